@@ -1,0 +1,178 @@
+"""ctypes view of the C ABI in include/*.h (librxcu.so).
+
+The library is loaded from the package directory only; a missing library raises instead of falling back to
+any other implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "librxcu.so"
+
+f3 = C.c_float * 3
+u3 = C.c_uint * 3
+
+
+class float3(C.Structure):
+    """float3 passed by value."""
+
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
+class uint3(C.Structure):
+    """uint3 passed by value."""
+
+    _fields_ = [("x", C.c_uint), ("y", C.c_uint), ("z", C.c_uint)]
+
+MAX_POLY_NUM = MAX_BOX_NUM = MAX_SPHERE_NUM = 10
+
+
+class rxSimParams(C.Structure):
+    """rx_cu_common.cuh:142-237 (include/rx_cu_common.h)."""
+
+    _fields_ = [
+        ("Gravity", f3), ("GlobalDamping", C.c_float), ("ParticleRadius", C.c_float),
+        ("BoundaryMax", f3), ("BoundaryMin", f3),
+        ("GridSize", u3), ("NumCells", C.c_uint), ("WorldOrigin", f3), ("WorldMax", f3), ("CellWidth", f3),
+        ("NumBodies", C.c_uint), ("MaxParticlesPerCell", C.c_uint),
+        ("GridSizeB", u3), ("NumCellsB", C.c_uint), ("WorldOriginB", f3), ("WorldMaxB", f3), ("CellWidthB", f3),
+        ("NumBodiesB", C.c_uint),
+        ("EffectiveRadius", C.c_float), ("Mass", C.c_float), ("VorticityConfinement", C.c_float),
+        ("Buoyancy", C.c_float), ("Density", C.c_float), ("Pressure", C.c_float),
+        ("Tension", C.c_float), ("Viscosity", C.c_float), ("GasStiffness", C.c_float),
+        ("Volume", C.c_float), ("KernelParticles", C.c_float), ("Restitution", C.c_float),
+        ("Threshold", C.c_float), ("InitDensity", C.c_float), ("Dt", C.c_float),
+        ("Wpoly6", C.c_float), ("GWpoly6", C.c_float), ("LWpoly6", C.c_float),
+        ("Wspiky", C.c_float), ("GWspiky", C.c_float), ("LWspiky", C.c_float),
+        ("Wvisc", C.c_float), ("GWvisc", C.c_float), ("LWvisc", C.c_float),
+        ("Wd2", C.c_float), ("Wd3", C.c_float), ("GWd2", C.c_float), ("GWd3", C.c_float), ("Wd1", C.c_float),
+        ("NoiseScl", C.c_float), ("NoiseEthr", C.c_float), ("NoiseMag", C.c_float),
+        ("AP", C.c_int), ("AP_K", C.c_float), ("AP_N", C.c_float), ("AP_Q", C.c_float), ("AP_WQ", C.c_float),
+        ("PolyNum", C.c_uint), ("PolyVel", f3 * MAX_POLY_NUM),
+        ("BoxNum", C.c_uint), ("BoxCen", f3 * MAX_BOX_NUM), ("BoxExt", f3 * MAX_BOX_NUM),
+        ("BoxRot", (f3 * 3) * MAX_BOX_NUM), ("BoxInvRot", (f3 * 3) * MAX_BOX_NUM), ("BoxFlg", C.c_uint * MAX_BOX_NUM),
+        ("SphereNum", C.c_uint), ("SphereCen", f3 * MAX_SPHERE_NUM), ("SphereRad", C.c_float * MAX_SPHERE_NUM),
+        ("SphereFlg", C.c_uint * MAX_SPHERE_NUM),
+    ]
+
+
+class rxParticleCell(C.Structure):
+    """rx_cu_common.cuh:241-259; passed by value."""
+
+    _fields_ = [
+        ("dSortedPos", C.c_void_p), ("dSortedVel", C.c_void_p), ("dSortedIndex", C.c_void_p),
+        ("dGridParticleHash", C.c_void_p), ("dCellStart", C.c_void_p), ("dCellEnd", C.c_void_p),
+        ("uNumParticles", C.c_uint), ("uNumCells", C.c_uint), ("uNumArdGrid", C.c_uint),
+        ("dSortedPolyIdx", C.c_void_p), ("dGridPolyHash", C.c_void_p), ("dPolyCellStart", C.c_void_p),
+        ("dPolyCellEnd", C.c_void_p), ("uNumPolyHash", C.c_uint),
+    ]
+
+
+class rxSphEnv(C.Structure):
+    """include/rx_sph_host.h."""
+
+    _fields_ = [
+        ("max_particles", C.c_int), ("boundary_cen", f3), ("boundary_ext", f3), ("h", C.c_float),
+        ("rest_dens", C.c_float), ("mass", C.c_float), ("spacing", C.c_float), ("particle_radius", C.c_float),
+        ("dt", C.c_float), ("viscosity", C.c_float), ("gas_k", C.c_float), ("gravity", f3),
+        ("epsilon", C.c_float), ("eta", C.c_float), ("min_iter", C.c_int), ("max_iter", C.c_int),
+        ("use_ap", C.c_int), ("ap_k", C.c_float), ("ap_n", C.c_float), ("ap_q", C.c_float),
+        ("xsph_c", C.c_float), ("cell_width", C.c_float), ("expansion", C.c_float),
+    ]
+
+
+vp = C.c_void_p
+fp = C.POINTER(C.c_float)
+u32 = C.c_uint
+i32 = C.c_int
+flt = C.c_float
+
+# name -> (restype, argtypes).  Every symbol declared in include/*.h is listed here.
+SIGNATURES: dict[str, tuple] = {
+    # rx_cu_funcs.h
+    "CuInit": (None, [i32, vp]),
+    "CuSetDevice": (None, [i32]),
+    "CuDeviceProp": (None, []),
+    "CuAllocateArray": (None, [C.POINTER(vp), i32]),
+    "CuSetArrayValue": (None, [vp, i32, C.c_size_t]),
+    "CuFreeArray": (None, [vp]),
+    "CuCopyArrayD2D": (None, [vp, vp, i32]),
+    "CuThreadSync": (None, []),
+    "CuCopyArrayFromDevice": (None, [vp, vp, vp, i32]),
+    "CuCopyArrayToDevice": (None, [vp, vp, i32, i32]),
+    "CuRegisterGLBufferObject": (None, [u32, vp]),
+    "CuUnregisterGLBufferObject": (None, [vp]),
+    "CuMapGLBufferObject": (vp, [vp]),
+    "CuUnmapGLBufferObject": (None, [vp]),
+    "CuScan": (None, [vp, vp, u32]),
+    "CuSort": (None, [vp, vp, u32]),
+    "CuScanf": (None, [vp, vp, u32]),
+    "CuSPHInit": (None, [i32]),
+    "CuSPHClean": (None, []),
+    "CuSetParameters": (None, [C.POINTER(rxSimParams)]),
+    "CuClearData": (None, []),
+    "CuCalcHash": (None, [vp, vp, vp, vp, i32]),
+    "CuReorderDataAndFindCellStart": (None, [rxParticleCell, vp, vp]),
+    "CuCalcHashB": (None, [vp, vp, vp, float3, float3, uint3, i32]),
+    "CuReorderDataAndFindCellStartB": (None, [rxParticleCell, vp]),
+    "CuSphDensity": (None, [vp, vp, rxParticleCell]),
+    "CuSphForces": (None, [vp, vp, vp, rxParticleCell, flt]),
+    "CuSphNormal": (None, [vp, vp, rxParticleCell]),
+    "CuSphIntegrate": (None, [vp, vp, vp, vp, vp, flt, u32]),
+    "CuSphCheckDelete": (None, [vp, vp, vp, fp, fp, fp, u32]),
+    "CuSphCheckDeleteX": (None, [vp, vp, vp, flt, fp, u32]),
+    "CuSphDetectNeighborParticles": (None, [vp, rxParticleCell, flt, i32]),
+    "CuSphDetectSurfaceParticles": (None, [vp, vp, rxParticleCell, vp, flt]),
+    "CuPbdSphDensity": (None, [vp, rxParticleCell]),
+    "CuPbdSphExternalForces": (None, [vp, vp, rxParticleCell, flt]),
+    "CuPbdSphScalingFactor": (None, [vp, vp, vp, flt, rxParticleCell]),
+    "CuPbdSphPositionCorrection": (None, [vp, vp, vp, rxParticleCell]),
+    "CuPbdSphCorrectPosition": (None, [vp, vp, u32]),
+    "CuPbdSphUpdatePosition": (None, [vp, vp, vp, flt, u32]),
+    "CuPbdSphUpdateVelocity": (None, [vp, vp, vp, flt, u32]),
+    "CuXSphViscosity": (None, [vp, vp, vp, vp, flt, rxParticleCell]),
+    "CuPbdSphCalDensityFluctuation": (flt, [vp, vp, vp, flt, u32]),
+    "CuPbdSphIntegrate": (None, [vp, vp, vp, vp, vp, vp, flt, u32]),
+    "CuPbdSphCheckDelete": (None, [vp, vp, vp, fp, fp, fp, u32]),
+    "CuPbdSphCheckDeleteX": (None, [vp, vp, vp, flt, fp, u32]),
+    "CuPbdSphNormal": (None, [vp, vp, rxParticleCell]),
+    "CuGetLastErrorCode": (i32, []),
+    "CuGetLastErrorString": (C.c_char_p, []),
+    "CuClearLastError": (None, []),
+    "CuPbdSphDensityErrorStats": (None, [vp, flt, u32, fp, fp]),
+    # rx_sph_host.h
+    "rxSphEnvDefault": (None, [C.POINTER(rxSphEnv)]),
+    "rxSetupSimParams": (i32, [C.POINTER(rxSphEnv), C.POINTER(rxSimParams)]),
+    "rxAbiLayout": (C.c_long, [i32]),
+    "rxSceneBox": (i32, [vp, vp, i32, fp, i32, i32, i32, flt, flt, flt, u32]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Loads librxcu.so from the package directory (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(lib: C.CDLL | None = None) -> None:
+    """Raises RuntimeError when the library recorded an error since the last check."""
+    lib = lib or load()
+    code = lib.CuGetLastErrorCode()
+    if code:
+        msg = lib.CuGetLastErrorString().decode(errors="replace")
+        lib.CuClearLastError()
+        raise RuntimeError(f"rx_cu error {code}: {msg}")

@@ -1,0 +1,107 @@
+"""Step drivers written against the extern "C" ABI alone, in the call order of the reference host classes.
+
+They restate rxSPH_GPU::Update (rx_sph_gpu.cpp:756-923) and rxPBDSPH_GPU::Update (rx_sph_pbd_gpu.cpp:790-957)
+one ABI call at a time, so a test can check the drop-in boundary end to end.  The fused solvers of the
+library (include/rx_sph_host.h) are the fast path; these drivers are the reference call sequence.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._abi import check, load, rxParticleCell
+from .device import DevBuf
+
+
+class GridBuffers:
+    """Device arrays of rxParticleCell plus the per-particle hash/index (rxSPH_GPU::Allocate, rx_sph_gpu.cpp:391-472)."""
+
+    def __init__(self, n: int, ncells: int, lib=None):
+        self.lib = lib or load()
+        self.n, self.ncells = n, ncells
+        self.hash = DevBuf(4 * n)
+        self.index = DevBuf(4 * n)
+        self.cstart = DevBuf(4 * ncells)
+        self.cend = DevBuf(4 * ncells)
+        self.spos = DevBuf(16 * n)
+        self.svel = DevBuf(16 * n)
+
+    def cell(self) -> rxParticleCell:
+        c = rxParticleCell()
+        c.dSortedPos, c.dSortedVel = self.spos.ptr, self.svel.ptr
+        c.dSortedIndex, c.dGridParticleHash = self.index.ptr, self.hash.ptr
+        c.dCellStart, c.dCellEnd = self.cstart.ptr, self.cend.ptr
+        c.uNumParticles, c.uNumCells = self.n, self.ncells
+        return c
+
+    def set_object_to_cell(self, dpos: int, dvel: int, dattr: int | None = None) -> rxParticleCell:
+        """rxSPH_GPU::setObjectToCell: hash, sort, reorder + cell bounds."""
+        lib = self.lib
+        lib.CuCalcHash(self.hash.ptr, self.index.ptr, dpos, dattr, self.n)
+        lib.CuSort(self.hash.ptr, self.index.ptr, self.n)
+        c = self.cell()
+        lib.CuReorderDataAndFindCellStart(c, dpos, dvel)
+        check(lib)
+        return c
+
+
+def sph_step(scene, pos4: np.ndarray, vel4: np.ndarray, steps: int = 1):
+    """WCSPH steps through the ABI: hash -> sort -> reorder -> density -> forces -> integrate."""
+    lib = load()
+    sp = scene.params()
+    n = pos4.shape[0]
+    lib.CuSetParameters(C.byref(sp))
+    g = GridBuffers(n, sp.NumCells, lib)
+    dpos, dvel = DevBuf.from_array(pos4), DevBuf.from_array(vel4)
+    dens, pres, frc = DevBuf(4 * n), DevBuf(4 * n), DevBuf(16 * n)
+    for _ in range(steps):
+        c = g.set_object_to_cell(dpos.ptr, dvel.ptr)
+        lib.CuSphDensity(dens.ptr, pres.ptr, c)
+        lib.CuSphForces(dens.ptr, pres.ptr, frc.ptr, c, scene.dt)
+        lib.CuSphIntegrate(dpos.ptr, dvel.ptr, frc.ptr, dens.ptr, None, scene.dt, n)
+        check(lib)
+    return (dpos.download(np.float32).reshape(n, 4), dvel.download(np.float32).reshape(n, 4),
+            dens.download(np.float32))
+
+
+def pbf_step(scene, pos4: np.ndarray, vel4: np.ndarray, steps: int = 1):
+    """PBF steps through the ABI in the order of rxPBDSPH_GPU::Update with a fixed iteration count and the
+    XSPH pass of the workload; returns (pos4, vel4, dens, max_err per iteration of the last step)."""
+    lib = load()
+    sp = scene.params()
+    n = pos4.shape[0]
+    lib.CuSetParameters(C.byref(sp))
+    g = GridBuffers(n, sp.NumCells, lib)
+    dpos, dvel = DevBuf.from_array(pos4), DevBuf.from_array(vel4)
+    dens, frc = DevBuf(4 * n), DevBuf(16 * n)
+    ppos, pvel = DevBuf(16 * n), DevBuf(16 * n)
+    scl, dp = DevBuf(4 * n), DevBuf(16 * n)
+    errs = []
+    mx, mean = C.c_float(), C.c_float()
+    for _ in range(steps):
+        errs = []
+        c = g.set_object_to_cell(dpos.ptr, dvel.ptr)
+        lib.CuPbdSphDensity(dens.ptr, c)
+        lib.CuPbdSphExternalForces(dens.ptr, frc.ptr, c, scene.dt)
+        lib.CuPbdSphIntegrate(dpos.ptr, dvel.ptr, frc.ptr, None, ppos.ptr, pvel.ptr, scene.dt, n)
+        for _it in range(scene.iterations):
+            c = g.set_object_to_cell(ppos.ptr, pvel.ptr)
+            lib.CuPbdSphScalingFactor(ppos.ptr, dens.ptr, scl.ptr, scene.epsilon, c)
+            lib.CuPbdSphDensityErrorStats(dens.ptr, sp.Density, n, C.byref(mx), C.byref(mean))
+            errs.append(mx.value)
+            lib.CuPbdSphPositionCorrection(ppos.ptr, scl.ptr, dp.ptr, c)
+            lib.CuPbdSphCorrectPosition(ppos.ptr, dp.ptr, n)
+            pvel.fill(0)
+            frc.fill(0)
+            lib.CuPbdSphIntegrate(ppos.ptr, pvel.ptr, frc.ptr, None, ppos.ptr, pvel.ptr, scene.dt, n)
+        lib.CuPbdSphUpdateVelocity(ppos.ptr, dpos.ptr, pvel.ptr, scene.dt, n)
+        lib.CuCopyArrayD2D(dpos.ptr, ppos.ptr, 16 * n)
+        if scene.xsph_c != 0.0:
+            c = g.set_object_to_cell(dpos.ptr, pvel.ptr)
+            lib.CuXSphViscosity(dpos.ptr, pvel.ptr, dvel.ptr, dens.ptr, scene.xsph_c, c)
+        else:
+            lib.CuCopyArrayD2D(dvel.ptr, pvel.ptr, 16 * n)
+        check(lib)
+    return (dpos.download(np.float32).reshape(n, 4), dvel.download(np.float32).reshape(n, 4),
+            dens.download(np.float32), errs)

@@ -1,0 +1,583 @@
+// rx_sph.hip -- WCSPH and PBF neighbour kernels behind the extern "C" ABI (rx_cu_funcs.cuh:70-128).
+//
+// One thread per particle in sorted (cell) order.  The 27-cell neighbourhood is walked as 9 rows of
+// consecutive cells: cells (x-1..x+1, y, z) have consecutive hashes, so their particles occupy one contiguous
+// slot range of the sorted arrays, and lanes of a wavefront (consecutive sorted particles, hence neighbouring
+// cells) stream overlapping ranges out of L1/L2.  Results are written in original particle order, which is
+// the contract of these entry points.
+#include <stdio.h>
+
+#include "rx_common.h"
+#include "../../include/rx_cu_funcs.h"
+
+namespace {
+
+constexpr int kTpb = 256;
+
+// Visits the slots of all particles in the cells overlapped by [p - h, p + h] (the cell range of
+// rx_sph_kernel.cu:266-278), row by row.
+template <class F>
+__device__ inline void for_each_candidate(float3 p, const SphK& k, const uint* __restrict__ cs, const uint* __restrict__ ce, F&& f) {
+    int3 g0 = grid_pos(f3(p.x - k.h, p.y - k.h, p.z - k.h), k);
+    int3 g1 = grid_pos(f3(p.x + k.h, p.y + k.h, p.z + k.h), k);
+    int nx = g1.x - g0.x + 1;
+    for (int z = g0.z; z <= g1.z; ++z)
+        for (int y = g0.y; y <= g1.y; ++y) {
+            uint s, e;
+            row_range(cs, ce, grid_hash(make_int3(g0.x, y, z), k), nx, s, e);
+            for (uint j = s; j < e; ++j) f(j);
+        }
+}
+
+__device__ inline float pow_n(float x, float n) { return (n == 4.0f) ? (x * x) * (x * x) : powf(x, n); }
+
+// ------------------------------------------------------------------------------------------ WCSPH
+// rx_sph_kernel.cu:216-288 (sphCalDensity)
+__global__ __launch_bounds__(kTpb) void k_sph_density(float* __restrict__ dens, float* __restrict__ pres, rxParticleCell c, SphK k, int write_pres) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.uNumParticles) return;
+    float3 p = f3(c.dSortedPos[i]);
+    float d = 0.0f;
+    for_each_candidate(p, k, c.dCellStart, c.dCellEnd, [&](uint j) {
+        float3 r = p - f3(c.dSortedPos[j]);
+        float r2 = dot3(r, r);
+        if (r2 <= k.h2) {
+            float q = k.h2 - r2;
+            d += k.mass * k.wpoly6 * q * q * q;
+        }
+    });
+    uint o = c.dSortedIndex[i];
+    dens[o] = d;
+    if (write_pres) pres[o] = k.gas_k * (d - k.rho0);
+}
+
+// rx_sph_kernel.cu:394-488 (sphCalForces) with the CPU path's pair rule (rx_sph_cpu.cpp:584: j != i, r > 0).
+__global__ __launch_bounds__(kTpb) void k_sph_force(const float* __restrict__ dens, const float* __restrict__ pres,
+                                                    float4* __restrict__ frc, rxParticleCell c, SphK k) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.uNumParticles) return;
+    float3 p0 = f3(c.dSortedPos[i]);
+    float3 v0 = f3(c.dSortedVel[i]);
+    uint o = c.dSortedIndex[i];
+    float d0 = dens[o];
+    float prsi = pres[o] / (d0 * d0);
+    float3 fp = f3(0, 0, 0), fv = f3(0, 0, 0);
+    for_each_candidate(p0, k, c.dCellStart, c.dCellEnd, [&](uint j) {
+        if (j == i) return;
+        float3 rij = p0 - f3(c.dSortedPos[j]);
+        float r2 = dot3(rij, rij);
+        if (r2 <= k.h2 && r2 > 0.0f) {
+            float r = sqrtf(r2);
+            uint oj = c.dSortedIndex[j];
+            float dj = dens[oj];
+            float prsj = pres[oj] / (dj * dj);
+            float q = k.h - r;
+            fp += (k.mass * (prsi + prsj) * k.gwspiky * q * q / r) * rij;
+            fv += (k.mass / dj * k.lwvisc * q) * (f3(c.dSortedVel[j]) - v0);
+        }
+    });
+    float3 a = (-1.0f) * fp + k.visc * fv + k.gravity;
+    frc[o] = make_float4(a.x, a.y, a.z, 0.0f);
+}
+
+// rx_sph_kernel.cu:301-376 (sphCalNormal)
+__global__ __launch_bounds__(kTpb) void k_sph_normal(float4* __restrict__ nrm, const float* __restrict__ dens, rxParticleCell c, SphK k) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.uNumParticles) return;
+    float3 p0 = f3(c.dSortedPos[i]);
+    float3 n = f3(0, 0, 0);
+    for_each_candidate(p0, k, c.dCellStart, c.dCellEnd, [&](uint j) {
+        if (j == i) return;
+        float3 rij = p0 - f3(c.dSortedPos[j]);
+        float r2 = dot3(rij, rij);
+        float r = sqrtf(r2);
+        if (r <= k.h && r > 0.0001f) {
+            float d1 = dens[c.dSortedIndex[j]];
+            float q = k.h2 - r2;
+            n += (k.mass / d1 * k.gwpoly6 * q * q) * rij;
+        }
+    });
+    float l = sqrtf(dot3(n, n));
+    if (l > 0.0f) n = (1.0f / l) * n;
+    nrm[c.dSortedIndex[i]] = make_float4(n.x, n.y, n.z, 0.0f);
+}
+
+// Wall rule of the CPU path (rxSolidBox::GetDistanceR, rx_sph_solid.cpp:244-262): keep the centre at least
+// ParticleRadius inside the box; velocity is not modified.
+__device__ inline float3 wall_clamp(float3 x, const SphK& k) {
+    x.x = fminf(fmaxf(x.x, k.wall_lo.x), k.wall_hi.x);
+    x.y = fminf(fmaxf(x.y, k.wall_lo.y), k.wall_hi.y);
+    x.z = fminf(fmaxf(x.z, k.wall_lo.z), k.wall_hi.z);
+    return x;
+}
+
+// rx_sph_kernel.cu:814-837 (sphIntegrate) with the CPU wall rule (rx_sph_cpu.cpp:1040-1077).
+__global__ void k_sph_integrate(float4* __restrict__ pos, float4* __restrict__ vel, const float4* __restrict__ acc,
+                                const int* __restrict__ attr, float dt, uint n, SphK k) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (attr && attr[i] == -1) return;
+    float3 x = f3(pos[i]), v = f3(vel[i]), a = f3(acc[i]);
+    v += dt * a;
+    x += dt * v;
+    x = wall_clamp(x, k);
+    pos[i] = make_float4(x.x, x.y, x.z, pos[i].w);
+    vel[i] = make_float4(v.x, v.y, v.z, vel[i].w);
+}
+
+// rx_sph_kernel.cu:904-940 (checkDelete / checkDeleteX)
+__global__ void k_check_delete(float4* pos, float4* vel, int* attr, float3 mn, float3 mx, float xmax, int mode, float3 far, uint n) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (attr[i] == -1) return;
+    float3 x = f3(pos[i]);
+    bool del = mode == 0 ? ((x.x > mn.x && x.x < mx.x) && (x.y > mn.y && x.y < mx.y) && (x.z > mn.z && x.z < mx.z)) : (x.x > xmax);
+    if (del) {
+        pos[i] = make_float4(far.x, far.y, far.z, 0.0f);
+        vel[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        attr[i] = -1;
+    }
+}
+
+// rx_sph_kernel.cu:1134-1208 (detectNeighborParticles).  As in the reference the test radius is the kernel
+// support h (params.EffectiveRadius); the radius argument of the entry point is not used.  The distance is
+// computed without contraction so that membership at r == h is the plain IEEE result.
+__global__ __launch_bounds__(kTpb) void k_detect_neighbors(int* __restrict__ nb, rxParticleCell c, SphK k, int nmax) {
+#pragma clang fp contract(off)
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.uNumParticles) return;
+    uint o = c.dSortedIndex[i];
+    float3 p = f3(c.dSortedPos[i]);
+    int cnt = 0;
+    int3 g0 = grid_pos(f3(p.x - k.h, p.y - k.h, p.z - k.h), k);
+    int3 g1 = grid_pos(f3(p.x + k.h, p.y + k.h, p.z + k.h), k);
+    bool full = false;
+    for (int z = g0.z; z <= g1.z && !full; ++z)
+        for (int y = g0.y; y <= g1.y && !full; ++y)
+            for (int x = g0.x; x <= g1.x && !full; ++x) {
+                uint gh = grid_hash(make_int3(x, y, z), k);
+                uint s = c.dCellStart[gh];
+                if (s == kEmpty) continue;
+                uint e = c.dCellEnd[gh];
+                for (uint j = s; j < e; ++j) {
+                    float3 q = f3(c.dSortedPos[j]);
+                    float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+                    float r = sqrtf(dx * dx + dy * dy + dz * dz);
+                    if (r <= k.h) {
+                        nb[(size_t)o * nmax + cnt] = (int)c.dSortedIndex[j];
+                        cnt++;
+                        if (cnt > nmax - 1) break;
+                    }
+                }
+                if (cnt > 29) full = true;  // 30 = maxNeight (rx_sph_kernel.cu:1184)
+            }
+    for (int t = cnt; t < nmax - 1; ++t) nb[(size_t)o * nmax + t] = -1;
+    nb[(size_t)o * nmax + nmax - 1] = cnt;
+}
+
+// rx_sph_kernel.cu:1214-1290 (CalDistToNormalizedMassCenter + detectSurfaceParticles), kept as written
+// (including its loop bound of count - 1 neighbours).
+__global__ void k_detect_surface(const int* __restrict__ nb, int* __restrict__ surf, rxParticleCell c, const float4* __restrict__ pos,
+                                 float radius, int nmax, SphK k) {
+    uint idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= c.uNumParticles) return;
+    uint o = c.dSortedIndex[idx];
+    float3 p0 = f3(pos[o]);
+    int nn = nb[(size_t)(o + 1) * nmax - 1];
+    float3 sp = f3(0, 0, 0);
+    float sm = 0.0f;
+    for (int t = 0; t < nn - 1; ++t) {
+        int j = nb[(size_t)o * nmax + t];
+        float3 p1 = f3(pos[j]);
+        sp += k.mass * (p0 - p1);
+        sm += k.mass;
+    }
+    float d = 1.17549435e-38f;  // FLT_MIN
+    if (sm > 0.0f) {
+        sp = (1.0f / sm) * sp;
+        d = sqrtf(dot3(sp, sp));
+    }
+    if (nn <= 3) {
+        surf[o] = nn;
+    } else if (surf[o] > 0) {
+        surf[o] = (d < 0.25f * radius) ? 0 : nn;
+    } else {
+        surf[o] = (d > 0.35f * radius || d < 0.0f) ? nn : 0;
+    }
+}
+
+// ------------------------------------------------------------------------------------------ PBF
+// rx_pbdsph_kernel.cu:71-101
+__global__ __launch_bounds__(kTpb) void k_pbf_ext_force(const float* __restrict__ dens, float4* __restrict__ frc, rxParticleCell c, SphK k) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.uNumParticles) return;
+    float3 p0 = f3(c.dSortedPos[i]);
+    float3 v0 = f3(c.dSortedVel[i]);
+    uint o = c.dSortedIndex[i];
+    float d0 = dens[o];
+    float3 fv = f3(0, 0, 0);
+    for_each_candidate(p0, k, c.dCellStart, c.dCellEnd, [&](uint j) {
+        if (j == i) return;
+        float3 rij = p0 - f3(c.dSortedPos[j]);
+        float r2 = dot3(rij, rij);
+        if (r2 <= k.h2 && r2 > 0.0f) {
+            float r = sqrtf(r2);
+            fv += (k.lwvisc * (k.h - r)) * (f3(c.dSortedVel[j]) - v0);
+        }
+    });
+    // rx_sph_pbd.cpp:676: viscosity uses the particle's own density
+    float3 a = (k.visc * k.mass / d0) * fv + k.gravity;
+    frc[o] = make_float4(a.x, a.y, a.z, 0.0f);
+}
+
+// rx_pbdsph_kernel.cu:253-303 with the self term of rx_sph_pbd.cpp:767-784:
+// sd = sum_{k != i} |grad W_ik / rho0|^2 + |sum_j grad W_ij / rho0|^2.
+__global__ __launch_bounds__(kTpb) void k_pbf_lambda(float* __restrict__ dens, float* __restrict__ scl, float eps, rxParticleCell c, SphK k) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.uNumParticles) return;
+    float3 p = f3(c.dSortedPos[i]);
+    float d = 0.0f, sd = 0.0f;
+    float3 gs = f3(0, 0, 0);
+    for_each_candidate(p, k, c.dCellStart, c.dCellEnd, [&](uint j) {
+        float3 rij = p - f3(c.dSortedPos[j]);
+        float r2 = dot3(rij, rij);
+        if (r2 <= k.h2) {
+            float q2 = k.h2 - r2;
+            d += k.mass * k.wpoly6 * q2 * q2 * q2;
+            if (r2 > 0.0f) {
+                float r = sqrtf(r2);
+                float q = k.h - r;
+                float3 g = (k.gwspiky * q * q / r * k.inv_rho0) * rij;
+                gs += g;
+                sd += dot3(g, g);
+            }
+        }
+    });
+    sd += dot3(gs, gs);
+    float C = d * k.inv_rho0 - 1.0f;
+    uint o = c.dSortedIndex[i];
+    scl[o] = -C / (sd + eps);
+    dens[o] = d;
+}
+
+// rx_pbdsph_kernel.cu:374-407 / rx_sph_pbd.cpp:925-1026
+__global__ __launch_bounds__(kTpb) void k_pbf_dp(const float* __restrict__ scl, float4* __restrict__ dp, rxParticleCell c, SphK k) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.uNumParticles) return;
+    float3 p = f3(c.dSortedPos[i]);
+    float si = scl[c.dSortedIndex[i]];
+    float dt2 = k.dt * k.dt;
+    float3 acc = f3(0, 0, 0);
+    for_each_candidate(p, k, c.dCellStart, c.dCellEnd, [&](uint j) {
+        float3 rij = p - f3(c.dSortedPos[j]);
+        float r2 = dot3(rij, rij);
+        if (r2 <= k.h2 && r2 > 0.0f) {
+            float r = sqrtf(r2);
+            float sc = 0.0f;
+            if (k.ap) {
+                float q2 = k.h2 - r2;
+                float ww = k.wpoly6 * q2 * q2 * q2 * k.inv_ap_wq;
+                sc = -k.ap_k * pow_n(ww, k.ap_n) * dt2;
+            }
+            float q = k.h - r;
+            float sj = scl[c.dSortedIndex[j]];
+            acc += ((si + sj + sc) * k.gwspiky * q * q / r * k.inv_rho0) * rij;
+        }
+    });
+    dp[c.dSortedIndex[i]] = make_float4(acc.x, acc.y, acc.z, 0.0f);
+}
+
+__global__ void k_add4(float4* __restrict__ a, const float4* __restrict__ b, uint n) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 x = a[i], d = b[i];
+    a[i] = make_float4(x.x + d.x, x.y + d.y, x.z + d.z, x.w + d.w);
+}
+
+// rx_pbdsph_kernel.cu:1066-1078 (pbdsphUpdateVelocity): new_vel = (pos - new_pos)/dt.
+__global__ void k_pbf_update_velocity(const float4* __restrict__ pos, const float4* __restrict__ new_pos, float4* __restrict__ new_vel, float dt, uint n) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float3 v = (1.0f / dt) * (f3(pos[i]) - f3(new_pos[i]));
+    new_vel[i] = make_float4(v.x, v.y, v.z, 0.0f);
+}
+// pbdsphUpdatePosition: new_vel = (new_pos - pos)/dt; pos = new_pos.
+__global__ void k_pbf_update_position(float4* __restrict__ pos, const float4* __restrict__ new_pos, float4* __restrict__ new_vel, float dt, uint n) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float3 x1 = f3(new_pos[i]);
+    float3 v = (1.0f / dt) * (x1 - f3(pos[i]));
+    new_vel[i] = make_float4(v.x, v.y, v.z, 0.0f);
+    pos[i] = make_float4(x1.x, x1.y, x1.z, pos[i].w);
+}
+
+// rx_pbdsph_kernel.cu:1130-1162 (xsphVisocosity)
+__global__ __launch_bounds__(kTpb) void k_xsph(const float4* __restrict__ vel, float4* __restrict__ nvel, const float* __restrict__ dens,
+                                               float cx, rxParticleCell c, SphK k) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.uNumParticles) return;
+    float3 p = f3(c.dSortedPos[i]);
+    uint o = c.dSortedIndex[i];
+    float3 v0 = f3(vel[o]);
+    float3 acc = f3(0, 0, 0);
+    for_each_candidate(p, k, c.dCellStart, c.dCellEnd, [&](uint j) {
+        float3 rij = p - f3(c.dSortedPos[j]);
+        float r2 = dot3(rij, rij);
+        if (r2 <= k.h2) {
+            uint oj = c.dSortedIndex[j];
+            float q = k.h2 - r2;
+            acc += (k.mass / dens[oj] * k.wpoly6 * q * q * q) * (f3(vel[oj]) - v0);
+        }
+    });
+    float3 v = v0 + cx * acc;
+    nvel[o] = make_float4(v.x, v.y, v.z, 0.0f);
+}
+
+// rx_pbdsph_kernel.cu:948-971 (pbdsphIntegrate) with the CPU wall rule.
+__global__ void k_pbf_integrate(const float4* __restrict__ pos, const float4* __restrict__ vel, const float4* __restrict__ acc,
+                                const int* __restrict__ attr, float4* __restrict__ npos, float4* __restrict__ nvel, float dt, uint n, SphK k) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 x4 = pos[i];
+    if (attr && attr[i] == -1) {
+        npos[i] = x4;
+        nvel[i] = vel[i];
+        return;
+    }
+    float3 x = f3(x4), v = f3(vel[i]), a = f3(acc[i]);
+    v += dt * a;
+    x += dt * v;
+    x = wall_clamp(x, k);
+    npos[i] = make_float4(x.x, x.y, x.z, x4.w);
+    nvel[i] = make_float4(v.x, v.y, v.z, 0.0f);
+}
+
+// rx_pbdsph_kernel.cu:431-441
+__global__ void k_dens_fluct(float* __restrict__ err, const float* __restrict__ dens, float r0, uint n) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float e = dens[i] - r0;
+    err[i] = (e >= 0.0f ? e : 0.0f) / r0;
+}
+
+// Max positive and mean absolute relative density error: per-block partials, then one block.
+__global__ __launch_bounds__(256) void k_err_partial(const float* __restrict__ dens, float r0, uint n, float* __restrict__ pmax, double* __restrict__ psum) {
+    __shared__ float smax[256];
+    __shared__ double ssum[256];
+    float mx = 0.0f;
+    double sm = 0.0;
+    for (uint i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float e = (dens[i] - r0) / r0;
+        mx = fmaxf(mx, e);
+        sm += fabs((double)e);
+    }
+    smax[threadIdx.x] = mx;
+    ssum[threadIdx.x] = sm;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + s]);
+            ssum[threadIdx.x] += ssum[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        pmax[blockIdx.x] = smax[0];
+        psum[blockIdx.x] = ssum[0];
+    }
+}
+__global__ __launch_bounds__(256) void k_err_final(const float* __restrict__ pmax, const double* __restrict__ psum, int nb, uint n, float* __restrict__ out) {
+    __shared__ float smax[256];
+    __shared__ double ssum[256];
+    float mx = 0.0f;
+    double sm = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) {
+        mx = fmaxf(mx, pmax[i]);
+        sm += psum[i];
+    }
+    smax[threadIdx.x] = mx;
+    ssum[threadIdx.x] = sm;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + s]);
+            ssum[threadIdx.x] += ssum[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = smax[0];
+        out[1] = (float)(n ? ssum[0] / n : 0.0);
+    }
+}
+
+bool need_params(const char* fn) {
+    if (!rx_params_set()) {
+        rx_set_error(3, "%s: CuSetParameters must be called first", fn);
+        return false;
+    }
+    return true;
+}
+
+void finish(const char* fn) {
+    RX_HIP(hipGetLastError());
+    RX_HIP(hipStreamSynchronize(rx_stream()));
+}
+
+}  // namespace
+
+// ============================================================================================ extern "C"
+extern "C" {
+
+void CuSphDensity(float* dDens, float* dPres, rxParticleCell cell) {
+    if (!need_params("CuSphDensity") || !cell.uNumParticles) return;
+    k_sph_density<<<rx_blocks(cell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>(dDens, dPres, cell, rx_make_consts(rx_params()), 1);
+    finish("CuSphDensity");
+}
+
+void CuSphForces(float* dDens, float* dPres, float* dFrc, rxParticleCell cell, float dt) {
+    if (!need_params("CuSphForces") || !cell.uNumParticles) return;
+    k_sph_force<<<rx_blocks(cell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>(dDens, dPres, (float4*)dFrc, cell, rx_make_consts(rx_params()));
+    finish("CuSphForces");
+}
+
+void CuSphNormal(float* dNrms, float* dDens, rxParticleCell cell) {
+    if (!need_params("CuSphNormal") || !cell.uNumParticles) return;
+    k_sph_normal<<<rx_blocks(cell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>((float4*)dNrms, dDens, cell, rx_make_consts(rx_params()));
+    finish("CuSphNormal");
+}
+
+void CuSphIntegrate(float* pos, float* vel, float* frc, float* dens, int* attr, float dt, uint numParticles) {
+    if (!need_params("CuSphIntegrate") || !numParticles) return;
+    k_sph_integrate<<<rx_blocks(numParticles, kTpb), kTpb, 0, rx_stream()>>>((float4*)pos, (float4*)vel, (const float4*)frc, attr, dt,
+                                                                            numParticles, rx_make_consts(rx_params()));
+    finish("CuSphIntegrate");
+}
+
+void CuSphCheckDelete(float* pos, float* vel, int* attr, float minp[3], float maxp[3], float far_point[3], uint nprts) {
+    if (!nprts) return;
+    k_check_delete<<<rx_blocks(nprts, kTpb), kTpb, 0, rx_stream()>>>((float4*)pos, (float4*)vel, attr, f3(minp[0], minp[1], minp[2]),
+                                                                    f3(maxp[0], maxp[1], maxp[2]), 0.0f, 0,
+                                                                    f3(far_point[0], far_point[1], far_point[2]), nprts);
+    finish("CuSphCheckDelete");
+}
+void CuSphCheckDeleteX(float* pos, float* vel, int* attr, float xmax, float farpoint[3], uint nprts) {
+    if (!nprts) return;
+    k_check_delete<<<rx_blocks(nprts, kTpb), kTpb, 0, rx_stream()>>>((float4*)pos, (float4*)vel, attr, f3(0, 0, 0), f3(0, 0, 0), xmax, 1,
+                                                                    f3(farpoint[0], farpoint[1], farpoint[2]), nprts);
+    finish("CuSphCheckDeleteX");
+}
+void CuPbdSphCheckDelete(float* pos, float* vel, int* attr, float minp[3], float maxp[3], float far_point[3], uint nprts) {
+    CuSphCheckDelete(pos, vel, attr, minp, maxp, far_point, nprts);
+}
+void CuPbdSphCheckDeleteX(float* pos, float* vel, int* attr, float xmax, float farpoint[3], uint nprts) {
+    CuSphCheckDeleteX(pos, vel, attr, xmax, farpoint, nprts);
+}
+
+void CuSphDetectNeighborParticles(int* neights, rxParticleCell celldata, float radius, int prtNum) {
+    if (!need_params("CuSphDetectNeighborParticles") || prtNum <= 0) return;
+    celldata.uNumParticles = (uint)prtNum;
+    k_detect_neighbors<<<rx_blocks(prtNum, kTpb), kTpb, 0, rx_stream()>>>(neights, celldata, rx_make_consts(rx_params()), 30);
+    finish("CuSphDetectNeighborParticles");
+}
+
+void CuSphDetectSurfaceParticles(int* neights, int* surface, rxParticleCell celldata, float* pos, float radius) {
+    if (!need_params("CuSphDetectSurfaceParticles") || !celldata.uNumParticles) return;
+    k_detect_surface<<<rx_blocks(celldata.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>(neights, surface, celldata, (const float4*)pos,
+                                                                                       radius, 30, rx_make_consts(rx_params()));
+    finish("CuSphDetectSurfaceParticles");
+}
+
+void CuPbdSphDensity(float* dDens, rxParticleCell cell) {
+    if (!need_params("CuPbdSphDensity") || !cell.uNumParticles) return;
+    k_sph_density<<<rx_blocks(cell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>(dDens, nullptr, cell, rx_make_consts(rx_params()), 0);
+    finish("CuPbdSphDensity");
+}
+
+void CuPbdSphExternalForces(float* dDens, float* dFrc, rxParticleCell cell, float dt) {
+    if (!need_params("CuPbdSphExternalForces") || !cell.uNumParticles) return;
+    k_pbf_ext_force<<<rx_blocks(cell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>(dDens, (float4*)dFrc, cell, rx_make_consts(rx_params()));
+    finish("CuPbdSphExternalForces");
+}
+
+void CuPbdSphScalingFactor(float* dPos, float* dDens, float* dScl, float eps, rxParticleCell cell) {
+    if (!need_params("CuPbdSphScalingFactor") || !cell.uNumParticles) return;
+    k_pbf_lambda<<<rx_blocks(cell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>(dDens, dScl, eps, cell, rx_make_consts(rx_params()));
+    finish("CuPbdSphScalingFactor");
+}
+
+void CuPbdSphPositionCorrection(float* dPos, float* dScl, float* dDp, rxParticleCell cell) {
+    if (!need_params("CuPbdSphPositionCorrection") || !cell.uNumParticles) return;
+    k_pbf_dp<<<rx_blocks(cell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>(dScl, (float4*)dDp, cell, rx_make_consts(rx_params()));
+    finish("CuPbdSphPositionCorrection");
+}
+
+void CuPbdSphCorrectPosition(float* dPos, float* dDp, uint nprts) {
+    if (!nprts) return;
+    k_add4<<<rx_blocks(nprts, kTpb), kTpb, 0, rx_stream()>>>((float4*)dPos, (const float4*)dDp, nprts);
+    finish("CuPbdSphCorrectPosition");
+}
+
+void CuPbdSphUpdatePosition(float* pos, float* new_pos, float* new_vel, float dt, uint nprts) {
+    if (!nprts) return;
+    k_pbf_update_position<<<rx_blocks(nprts, kTpb), kTpb, 0, rx_stream()>>>((float4*)pos, (const float4*)new_pos, (float4*)new_vel, dt, nprts);
+    finish("CuPbdSphUpdatePosition");
+}
+
+void CuPbdSphUpdateVelocity(float* pos, float* new_pos, float* new_vel, float dt, uint nprts) {
+    if (!nprts) return;
+    k_pbf_update_velocity<<<rx_blocks(nprts, kTpb), kTpb, 0, rx_stream()>>>((const float4*)pos, (const float4*)new_pos, (float4*)new_vel, dt, nprts);
+    finish("CuPbdSphUpdateVelocity");
+}
+
+void CuXSphViscosity(float* dPos, float* dVel, float* dNewVel, float* dDens, float c, rxParticleCell cell) {
+    if (!need_params("CuXSphViscosity") || !cell.uNumParticles) return;
+    k_xsph<<<rx_blocks(cell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>((const float4*)dVel, (float4*)dNewVel, dDens, c, cell,
+                                                                         rx_make_consts(rx_params()));
+    finish("CuXSphViscosity");
+}
+
+float CuPbdSphCalDensityFluctuation(float* dErrScan, float* dErr, float* dDens, float rest_dens, uint nprts) {
+    if (!nprts) return 0.0f;
+    hipStream_t s = rx_stream();
+    k_dens_fluct<<<rx_blocks(nprts, kTpb), kTpb, 0, s>>>(dErr, dDens, rest_dens, nprts);
+    rx_scan_f32(dErr, dErrScan, nprts, s);
+    float last[2] = {0, 0};
+    RX_HIP(hipMemcpyAsync(&last[0], dErrScan + nprts - 1, sizeof(float), hipMemcpyDeviceToHost, s));
+    RX_HIP(hipMemcpyAsync(&last[1], dErr + nprts - 1, sizeof(float), hipMemcpyDeviceToHost, s));
+    RX_HIP(hipStreamSynchronize(s));
+    return (last[0] + last[1]) / (float)nprts;
+}
+
+void CuPbdSphIntegrate(float* pos, float* vel, float* acc, int* attr, float* new_pos, float* new_vel, float dt, uint numParticles) {
+    if (!need_params("CuPbdSphIntegrate") || !numParticles) return;
+    k_pbf_integrate<<<rx_blocks(numParticles, kTpb), kTpb, 0, rx_stream()>>>((const float4*)pos, (const float4*)vel, (const float4*)acc, attr,
+                                                                            (float4*)new_pos, (float4*)new_vel, dt, numParticles,
+                                                                            rx_make_consts(rx_params()));
+    finish("CuPbdSphIntegrate");
+}
+
+void CuPbdSphNormal(float* dNrms, float* dDens, rxParticleCell cell) { CuSphNormal(dNrms, dDens, cell); }
+
+void CuPbdSphDensityErrorStats(float* dDens, float rest_dens, uint nprts, float* host_max_pos, float* host_mean_abs) {
+    if (!nprts) {
+        if (host_max_pos) *host_max_pos = 0.0f;
+        if (host_mean_abs) *host_mean_abs = 0.0f;
+        return;
+    }
+    hipStream_t s = rx_stream();
+    int nb = (int)std::min<uint>(rx_blocks(nprts, 256), 1024);
+    char* scr = (char*)rx_scratch(5, (sizeof(float) + sizeof(double)) * nb + 64);
+    double* psum = (double*)scr;
+    float* pmax = (float*)(scr + sizeof(double) * nb);
+    float* out = pmax + nb;
+    k_err_partial<<<nb, 256, 0, s>>>(dDens, rest_dens, nprts, pmax, psum);
+    k_err_final<<<1, 256, 0, s>>>(pmax, psum, nb, nprts, out);
+    float h[2];
+    RX_HIP(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, s));
+    RX_HIP(hipStreamSynchronize(s));
+    if (host_max_pos) *host_max_pos = h[0];
+    if (host_mean_abs) *host_mean_abs = h[1];
+}
+
+}  // extern "C"

@@ -1,0 +1,123 @@
+"""Scene set-up: the seeded workloads of the particle path and their rxSimParams.
+
+Each workload is a dam-break style block of particles on a jittered lattice inside a box (the scene of
+rxParticleSystemBase::AddBox, rx_ps.cpp:195-261, with a fixed seed).  Gravity acts along -y as in the
+reference application (rx_fltk_glcanvas.cpp:193).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._abi import check, load, rxSimParams, rxSphEnv
+
+
+@dataclass
+class Scene:
+    name: str
+    box_min: tuple
+    box_max: tuple
+    h: float
+    spacing: float
+    block: tuple            # (nx, ny, nz)
+    block_origin: tuple
+    jitter: float           # absolute, per axis
+    vel_sigma: float
+    seed: int
+    dt: float
+    rest_dens: float = 998.29
+    gas_k: float = 3.0
+    viscosity: float = 1.0e-3
+    gravity: tuple = (0.0, -9.8, 0.0)
+    # PBF
+    epsilon: float = 600.0
+    iterations: int = 4
+    use_ap: int = 1
+    ap_k: float = 0.1
+    ap_n: float = 4.0
+    ap_q: float = 0.2
+    xsph_c: float = 0.0
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return int(np.prod(self.block))
+
+    def env(self) -> rxSphEnv:
+        lib = load()
+        e = rxSphEnv()
+        lib.rxSphEnvDefault(C.byref(e))
+        e.max_particles = self.n
+        for k in range(3):
+            e.boundary_cen[k] = 0.5 * (self.box_min[k] + self.box_max[k])
+            e.boundary_ext[k] = 0.5 * (self.box_max[k] - self.box_min[k])
+            e.gravity[k] = self.gravity[k]
+        e.h = self.h
+        e.rest_dens = self.rest_dens
+        e.mass = 0.0
+        e.spacing = self.spacing
+        e.particle_radius = 0.0
+        e.dt = self.dt
+        e.viscosity = self.viscosity
+        e.gas_k = self.gas_k
+        e.epsilon = self.epsilon
+        e.min_iter = self.iterations
+        e.max_iter = self.iterations
+        e.use_ap = self.use_ap
+        e.ap_k, e.ap_n, e.ap_q = self.ap_k, self.ap_n, self.ap_q
+        e.xsph_c = self.xsph_c
+        e.cell_width = 0.0
+        e.expansion = 0.01
+        return e
+
+    def params(self) -> rxSimParams:
+        lib = load()
+        p = rxSimParams()
+        e = self.env()
+        if lib.rxSetupSimParams(C.byref(e), C.byref(p)) != 0:
+            check(lib)
+        p.NumBodies = self.n
+        return p
+
+    def particles(self) -> tuple[np.ndarray, np.ndarray]:
+        """(pos4, vel4) float32 arrays of shape (n, 4)."""
+        lib = load()
+        n = self.n
+        pos = np.zeros((n, 4), np.float32)
+        vel = np.zeros((n, 4), np.float32)
+        org = (C.c_float * 3)(*self.block_origin)
+        got = lib.rxSceneBox(pos.ctypes.data, vel.ctypes.data, n, org, *self.block, self.spacing, self.jitter,
+                             self.vel_sigma, self.seed)
+        assert got == n
+        return pos, vel
+
+
+# --------------------------------------------------------------------------------------------- workloads
+def wcsph_dam_break(nx: int = 32, ny: int = 32, nz: int = 32, seed: int = 1) -> Scene:
+    """WCSPH dam break: 32^3 block at spacing 0.5h, jitter 0.01h, unit box, h = 0.02, k = 3, mu = 3.5,
+    dt = 1e-3 (32,768 particles at the default size)."""
+    h = 0.02
+    return Scene("wcsph_dam_break", (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), h, 0.5 * h, (nx, ny, nz), (0.0, 0.0, 0.0),
+                 0.01 * h, 0.0, seed, 1.0e-3, gas_k=3.0, viscosity=3.5)
+
+
+def pbf_dam_break(nx: int = 64, ny: int = 64, nz: int = 64, seed: int = 2, box: tuple | None = None) -> Scene:
+    """Plain PBF: block at spacing 0.5h (h = 0.1) with jitter 0.01h in a 2:1:1 box, 4 Jacobi iterations,
+    eps = 600, s_corr k = 0.1 n = 4, XSPH c = 0.01, dt = 1/120, velocities N(0, 0.05)
+    (262,144 particles at the default size)."""
+    h = 0.1
+    s = 0.5 * h
+    if box is None:
+        ly = max(ny, nz) * s
+        box = (2.0 * ly, ly, ly)
+    return Scene("pbf_dam_break", (0.0, 0.0, 0.0), box, h, s, (nx, ny, nz), (0.0, 0.0, 0.0), 0.01 * h, 0.05, seed,
+                 1.0 / 120.0, epsilon=600.0, iterations=4, use_ap=1, ap_k=0.1, ap_n=4.0, ap_q=0.2, xsph_c=0.01)
+
+
+def pbf_1m(seed: int = 5) -> Scene:
+    """Learned-warm-start PBF workload: 128 x 128 x 64 = 1,048,576 particles, plain-PBF parameters."""
+    sc = pbf_dam_break(128, 128, 64, seed=seed, box=(12.8, 6.4, 6.4))
+    sc.name = "pbf_1m"
+    return sc

@@ -1,0 +1,161 @@
+"""ctypes bindings of the CPU oracle (liboracle.so).  TEST INFRASTRUCTURE ONLY.
+
+The oracle is the checker of the HIP path: tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+bench.py use it; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("h", C.c_double), ("mass", C.c_double), ("rest_dens", C.c_double), ("gas_k", C.c_double),
+        ("viscosity", C.c_double), ("gravity", C.c_double * 3), ("radius", C.c_double),
+        ("bmin", C.c_double * 3), ("bmax", C.c_double * 3), ("origin", C.c_double * 3), ("cell", C.c_double * 3),
+        ("gsize", C.c_int * 3), ("eps", C.c_double), ("use_ap", C.c_int), ("ap_k", C.c_double),
+        ("ap_n", C.c_double), ("ap_q", C.c_double), ("xsph_c", C.c_double), ("iterations", C.c_int),
+        ("dt", C.c_double),
+    ]
+
+
+class OrcPbfStats(C.Structure):
+    _fields_ = [("max_err", C.c_double * 16), ("mean_abs", C.c_double * 16), ("iterations", C.c_int)]
+
+
+class OrcSmParams(C.Structure):
+    _fields_ = [
+        ("alpha", C.c_double), ("beta", C.c_double), ("dt", C.c_double), ("gravity", C.c_double * 3),
+        ("bmin", C.c_double * 3), ("bmax", C.c_double * 3), ("iterations", C.c_int),
+        ("volume_conservation", C.c_int), ("restitution", C.c_double),
+    ]
+
+
+_lib = None
+P = C.c_void_p
+D = C.c_double
+I = C.c_int
+F = C.c_float
+
+_SIGS = {
+    "orc_kernel_coef": (D, [I, D, I]),
+    "orc_kernel_poly6": (D, [D, D, D]),
+    "orc_kernel_spiky_grad": (D, [D, D, D]),
+    "orc_kernel_visc_lap": (D, [D, D, D]),
+    "orc_rest_density": (D, [D, D, D]),
+    "orc_calc_hash_f": (None, [P, P, I, P, P, P]),
+    "orc_stable_sort": (None, [P, P, I]),
+    "orc_cell_bounds": (None, [P, I, I, P, P]),
+    "orc_neighbors_brute_f": (C.c_long, [P, I, F, I, P, P]),
+    "orc_sph_density_f": (None, [P, I, P, P, P]),
+    "orc_sph_force_f": (None, [P, P, P, P, I, P, P]),
+    "orc_sph_step_f": (None, [P, P, I, P, F, P, P, P]),
+    "orc_sph_step_d": (None, [P, P, I, P, D, P]),
+    "orc_pbf_density_f": (None, [P, I, P, P]),
+    "orc_pbf_ext_force_f": (None, [P, P, P, I, P, P]),
+    "orc_pbf_scaling_factor_f": (None, [P, I, P, P, P]),
+    "orc_pbf_position_correction_f": (None, [P, P, I, P, F, P]),
+    "orc_pbf_xsph_f": (None, [P, P, P, I, P, F, P]),
+    "orc_pbf_step_f": (None, [P, P, I, P, P, P, P]),
+    "orc_pbf_step_d": (None, [P, P, I, P, P, P]),
+    "orc_jacobi_eigen3": (I, [P, P, D, I]),
+    "orc_polar_decomposition": (None, [P, P, P]),
+    "orc_sm_step_f": (None, [P, P, I, P, P, I, P, P]),
+    "orc_sm_step_d": (None, [P, P, I, P, P, I, P, P]),
+    "orc_heat_diffuse_f": (None, [P, P, P, I, P, F, F, P]),
+    "orc_mlp_forward_d": (None, [P, I, I, P, P, I, P, P, I, P, P, P]),
+    "orc_mlp_backward_d": (None, [P, P, I, I, P, P, I, P, P, I, P, P, P, P, P, P, P, P, P]),
+}
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def ptr(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def params_from(sp, scene=None) -> OrcParams:
+    """OrcParams carrying exactly the float32 values of an rxSimParams (plus the PBF knobs of a Scene)."""
+    o = OrcParams()
+    o.h = sp.EffectiveRadius
+    o.mass = sp.Mass
+    o.rest_dens = sp.Density
+    o.gas_k = sp.GasStiffness
+    o.viscosity = sp.Viscosity
+    for k in range(3):
+        o.gravity[k] = sp.Gravity[k]
+        o.bmin[k] = sp.BoundaryMin[k]
+        o.bmax[k] = sp.BoundaryMax[k]
+        o.origin[k] = sp.WorldOrigin[k]
+        o.cell[k] = sp.CellWidth[k]
+        o.gsize[k] = sp.GridSize[k]
+    o.radius = sp.ParticleRadius
+    o.use_ap = sp.AP
+    o.ap_k = sp.AP_K
+    o.ap_n = sp.AP_N
+    o.ap_q = sp.AP_Q
+    o.dt = sp.Dt
+    if scene is not None:
+        o.eps = scene.epsilon
+        o.xsph_c = scene.xsph_c
+        o.iterations = scene.iterations
+    return o
+
+
+def sph_step(pos4: np.ndarray, vel4: np.ndarray, P: OrcParams, dt: float, double: bool = False):
+    """One WCSPH step of the CPU path; returns (pos4, vel4, dens)."""
+    lib = load()
+    n = pos4.shape[0]
+    if double:
+        pos = np.ascontiguousarray(pos4, np.float64).copy()
+        vel = np.ascontiguousarray(vel4, np.float64).copy()
+        dens = np.zeros(n, np.float64)
+        lib.orc_sph_step_d(ptr(pos), ptr(vel), n, C.byref(P), dt, ptr(dens))
+        return pos, vel, dens
+    pos = np.ascontiguousarray(pos4, np.float32).copy()
+    vel = np.ascontiguousarray(vel4, np.float32).copy()
+    dens = np.zeros(n, np.float32)
+    pres = np.zeros(n, np.float32)
+    acc = np.zeros((n, 4), np.float32)
+    lib.orc_sph_step_f(ptr(pos), ptr(vel), n, C.byref(P), dt, ptr(dens), ptr(pres), ptr(acc))
+    return pos, vel, dens
+
+
+def pbf_step(pos4: np.ndarray, vel4: np.ndarray, P: OrcParams, double: bool = False, want_lambda: bool = False):
+    """One PBF step of the CPU path; returns (pos4, vel4, dens, stats[, lambda_sum])."""
+    lib = load()
+    n = pos4.shape[0]
+    st = OrcPbfStats()
+    if double:
+        pos = np.ascontiguousarray(pos4, np.float64).copy()
+        vel = np.ascontiguousarray(vel4, np.float64).copy()
+        dens = np.zeros(n, np.float64)
+        lib.orc_pbf_step_d(ptr(pos), ptr(vel), n, C.byref(P), ptr(dens), C.byref(st))
+        return pos, vel, dens, st
+    pos = np.ascontiguousarray(pos4, np.float32).copy()
+    vel = np.ascontiguousarray(vel4, np.float32).copy()
+    dens = np.zeros(n, np.float32)
+    lam = np.zeros(n, np.float32)
+    lib.orc_pbf_step_f(ptr(pos), ptr(vel), n, C.byref(P), ptr(dens), ptr(lam), C.byref(st))
+    if want_lambda:
+        return pos, vel, dens, st, lam
+    return pos, vel, dens, st

@@ -1,0 +1,120 @@
+/*
+ * oracle.h -- CPU restatement of the reference's particle path.  TEST INFRASTRUCTURE ONLY.
+ *
+ * This library is the checker of the HIP path: only the tests, __graft_entry__.smoke() and the CPU-baseline
+ * leg of bench.py load it.  Nothing in the product links it.
+ *
+ * Each function follows a reference CPU routine (cited in oracle.cpp).  Storage precision is selectable:
+ * the *_f entry points store particle state in float (the reference's RXREAL) and evaluate kernels in
+ * double (as rx_kernel.h does); the *_d entry points store everything in double (the FP64 drift oracle).
+ * Particle arrays are float4/double4 AoS with stride 4 (the reference's DIM = 4).
+ */
+#ifndef ORACLE_H_
+#define ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct OrcParams {
+    /* fluid */
+    double h;             /* kernel support                                       */
+    double mass;          /* particle mass                                        */
+    double rest_dens;     /* rho0                                                 */
+    double gas_k;         /* WCSPH gas constant                                   */
+    double viscosity;     /* mu                                                   */
+    double gravity[3];
+    double radius;        /* particle radius = wall margin                        */
+    double bmin[3];       /* walls                                                */
+    double bmax[3];
+    /* neighbour grid (values are exactly representable in float)                */
+    double origin[3];
+    double cell[3];
+    int gsize[3];
+    /* PBF */
+    double eps;           /* relaxation epsilon                                   */
+    int use_ap;           /* s_corr on/off                                        */
+    double ap_k, ap_n, ap_q;
+    double xsph_c;        /* XSPH coefficient (0 = off)                           */
+    int iterations;       /* Jacobi iterations per step                           */
+    double dt;
+} OrcParams;
+
+/* Per-step PBF statistics. */
+typedef struct OrcPbfStats {
+    double max_err[16];   /* max_i max(rho_i - rho0, 0)/rho0 at the start of iteration k */
+    double mean_abs[16];  /* mean_i |rho_i - rho0|/rho0 at the start of iteration k      */
+    int iterations;
+} OrcPbfStats;
+
+/* ---- kernels (rx_kernel.h) */
+double orc_kernel_coef(int kind, double h, int type);   /* kind 0 poly6, 1 spiky, 2 visc; type 1 W, 2 grad, 3 lap */
+double orc_kernel_poly6(double r, double h, double a);
+double orc_kernel_spiky_grad(double r, double h, double a);  /* scalar factor a (h-r)^2 / r, 0 outside (0,h] */
+double orc_kernel_visc_lap(double r, double h, double a);
+double orc_rest_density(double mass, double h, double spacing);
+
+/* ---- grid (rx_nnsearch.h) */
+void orc_calc_hash_f(const float* pos4, const int* attr, int n, const OrcParams* P, uint32_t* hash, uint32_t* index);
+void orc_stable_sort(uint32_t* hash, uint32_t* index, int n);
+void orc_cell_bounds(const uint32_t* sorted_hash, int n, int ncells, uint32_t* start, uint32_t* end);
+/* Neighbours of every particle within r <= h (float distance test of the GPU), brute force; returns the
+ * total count and fills counts[n] and, when list != NULL, list[i*maxn + k] (ascending original index). */
+long orc_neighbors_brute_f(const float* pos4, int n, float h, int maxn, int* counts, int* list);
+
+/* ---- WCSPH (rx_sph_cpu.cpp) */
+void orc_sph_density_f(const float* pos4, int n, const OrcParams* P, float* dens, float* pres);
+void orc_sph_force_f(const float* pos4, const float* vel4, const float* dens, const float* pres, int n,
+                     const OrcParams* P, float* acc4);
+void orc_sph_step_f(float* pos4, float* vel4, int n, const OrcParams* P, float dt, float* dens, float* pres, float* acc4);
+void orc_sph_step_d(double* pos4, double* vel4, int n, const OrcParams* P, double dt, double* dens);
+
+/* ---- PBF (rx_sph_pbd.cpp) */
+void orc_pbf_density_f(const float* pos4, int n, const OrcParams* P, float* dens);
+void orc_pbf_ext_force_f(const float* pos4, const float* vel4, const float* dens, int n, const OrcParams* P, float* acc4);
+void orc_pbf_scaling_factor_f(const float* pos4, int n, const OrcParams* P, float* dens, float* lambda);
+void orc_pbf_position_correction_f(const float* pos4, const float* lambda, int n, const OrcParams* P, float dt, float* dp4);
+void orc_pbf_xsph_f(const float* pos4, const float* vel4, const float* dens, int n, const OrcParams* P, float c, float* out_vel4);
+/* One PBF step of the reference CPU path with a fixed number of Jacobi iterations, collisions and the
+ * optional XSPH pass; lambda_sum (may be NULL) receives sum_k lambda_k; dens the last lambda-pass density. */
+void orc_pbf_step_f(float* pos4, float* vel4, int n, const OrcParams* P, float* dens, float* lambda_sum, OrcPbfStats* st);
+void orc_pbf_step_d(double* pos4, double* vel4, int n, const OrcParams* P, double* dens, OrcPbfStats* st);
+
+/* ---- shape matching (ShapeMatching.h / ShapeMatching.cpp / ElasticObject_OP.cpp / Ice_Convolution_Normal.cpp) */
+int orc_jacobi_eigen3(double* a9, double* v9, double eps, int iter_max);
+void orc_polar_decomposition(const double* A9, double* R9, double* S9);
+typedef struct OrcSmParams {
+    double alpha, beta, dt;
+    double gravity[3];
+    double bmin[3], bmax[3];
+    int iterations;
+    int volume_conservation;
+    double restitution;   /* wall response factor (0.9 in the reference) */
+} OrcSmParams;
+/* One batched shape-matching step over all clusters (CSR: clusters c have members
+ * members[cl_start[c] .. cl_start[c+1]-1]); rest positions of the members come from rest_pos4. */
+void orc_sm_step_f(float* pos4, float* vel4, int n, const int* cl_start, const int* members, int nclusters,
+                   const float* rest_pos4, const OrcSmParams* S);
+void orc_sm_step_d(double* pos4, double* vel4, int n, const int* cl_start, const int* members, int nclusters,
+                   const double* rest_pos4, const OrcSmParams* S);
+
+/* ---- heat (HeatTransfar.cpp / HeatTransfer.cu, SPH diffusion) */
+void orc_heat_diffuse_f(const float* pos4, const float* dens, const float* temp, int n, const OrcParams* P,
+                        float diffusivity, float dt, float* temp_out);
+
+/* ---- MLP (learned lambda warm start), FP64 */
+void orc_mlp_forward_d(const double* X, int nrows, int nin, const double* W1, const double* b1, int nh1,
+                       const double* W2, const double* b2, int nh2, const double* W3, const double* b3, double* Y);
+/* Gradients of L = 0.5/nrows * sum (Y - T)^2 with respect to all weights. */
+void orc_mlp_backward_d(const double* X, const double* T, int nrows, int nin,
+                        const double* W1, const double* b1, int nh1, const double* W2, const double* b2, int nh2,
+                        const double* W3, const double* b3,
+                        double* gW1, double* gb1, double* gW2, double* gb2, double* gW3, double* gb3, double* loss);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ORACLE_H_ */

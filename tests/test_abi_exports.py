@@ -1,0 +1,54 @@
+"""The C-ABI library loads, exports every symbol of include/*.h, and its struct layout matches the bindings."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        txt = h.read_text()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"^\s*(?:[A-Za-z_][\w\s\*]*?)\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", txt, flags=re.M):
+            name = m.group(1)
+            if name not in ("static_assert",):
+                names.add(name)
+    return names
+
+
+def test_every_declared_symbol_is_exported(lib):
+    from fltk_sph_ice_amd._abi import SIGNATURES
+
+    declared = _declared()
+    assert len(declared) > 50
+    missing = [n for n in sorted(declared) if not hasattr(lib, n)]
+    assert not missing, missing
+    unbound = sorted(declared - set(SIGNATURES))
+    assert not unbound, f"declared but not bound in _abi.SIGNATURES: {unbound}"
+
+
+def test_struct_layout_matches_bindings(lib):
+    from fltk_sph_ice_amd._abi import rxParticleCell, rxSimParams
+
+    assert lib.rxAbiLayout(0) == C.sizeof(rxSimParams)
+    assert lib.rxAbiLayout(1) == C.sizeof(rxParticleCell)
+    assert lib.rxAbiLayout(2) == rxSimParams.EffectiveRadius.offset
+    assert lib.rxAbiLayout(3) == rxSimParams.AP.offset
+    assert lib.rxAbiLayout(4) == rxSimParams.BoxNum.offset
+    assert lib.rxAbiLayout(5) == rxSimParams.SphereFlg.offset
+    assert lib.rxAbiLayout(6) == rxParticleCell.uNumParticles.offset
+    assert lib.rxAbiLayout(7) == rxParticleCell.uNumPolyHash.offset
+
+
+def test_reference_struct_size():
+    # rx_cu_common.cuh:142-237 with float3 = 12 bytes, all members 4-byte aligned:
+    # 308 bytes of scalar/vector members up to AP_WQ, PolyNum + PolyVel[10] (124), BoxNum + BoxCen/BoxExt[10]
+    # (244) + BoxRot/BoxInvRot[10] (720) + BoxFlg[10] (40), SphereNum + SphereCen[10] + SphereRad[10] +
+    # SphereFlg[10] (204) = 1640 bytes, no padding.
+    from fltk_sph_ice_amd._abi import rxSimParams
+
+    assert C.sizeof(rxSimParams) == 1640
