@@ -147,7 +147,20 @@ SIGNATURES: dict[str, tuple] = {
     "rxSetupSimParams": (i32, [C.POINTER(rxSphEnv), C.POINTER(rxSimParams)]),
     "rxAbiLayout": (C.c_long, [i32]),
     "rxSceneBox": (i32, [vp, vp, i32, fp, i32, i32, i32, flt, flt, flt, u32]),
+    "rxPBDSPH_GPU_Create": (vp, [C.POINTER(rxSphEnv)]),
+    "rxPBDSPH_GPU_Destroy": (None, [vp]),
+    "rxPBDSPH_GPU_SetParticles": (i32, [vp, vp, vp, i32]),
+    "rxPBDSPH_GPU_Update": (i32, [vp, flt, i32]),
+    "rxPBDSPH_GPU_GetArray": (i32, [vp, i32, vp, i32]),
+    "rxPBDSPH_GPU_GetStats": (i32, [vp, vp, i32]),
+    "rxPBDSPH_GPU_Sync": (i32, [vp]),
+    "rxPBDSPH_GPU_SetOption": (i32, [vp, i32, flt]),
+    "rxPBDSPH_GPU_GetProfile": (i32, [vp, vp, vp, i32]),
 }
+
+RX_ARRAY_POSITION, RX_ARRAY_VELOCITY, RX_ARRAY_DENSITY, RX_ARRAY_LAMBDA_SUM, RX_ARRAY_PRESSURE = range(5)
+RX_PBF_OPT_SYNC_EACH_STEP, RX_PBF_OPT_RECORD_LAMBDA, RX_PBF_OPT_TOL, RX_PBF_OPT_ITERATIONS, RX_PBF_OPT_SKIN, RX_PBF_OPT_PROFILE = range(6)
+PROF_CLASSES = ["predict", "grid", "list_lambda", "lambda", "dp", "velocity", "xsph", "features", "mlp"]
 
 _lib = None
 

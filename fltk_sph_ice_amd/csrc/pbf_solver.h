@@ -1,0 +1,76 @@
+// pbf_solver.h -- the fused PBF solver class (the rxPBDSPH_GPU interface of rx_sph.h:1284-1601).
+#pragma once
+
+#include <vector>
+
+#include "fused_grid.h"
+#include "../../include/rx_sph_host.h"
+
+struct PbfK;
+
+class rxPBDSPH_GPU {
+public:
+    rxPBDSPH_GPU();
+    ~rxPBDSPH_GPU();
+    int Initialize(const rxSphEnv& env);     // rx_sph_pbd_gpu.cpp:168-300
+    void Finalize();
+    int SetParticles(const float* pos4, const float* vel4, int n);
+    int Update(float dt, int step);          // rx_sph_pbd_gpu.cpp:790-957
+    int GetArray(int type, float* host, int count);
+    int GetStats(float* out, int count);
+    int SetOption(int opt, float value);
+    int Sync();
+    // Kernel timing with HIP events on the solver's stream (enabled by RX_PBF_OPT_PROFILE):
+    // total milliseconds and launch count per kernel class since the last reset.
+    int GetProfile(float* ms, int* count, int n);
+    hipStream_t stream() const { return m_stream; }
+    uint size() const { return m_n; }
+
+private:
+    bool alloc(uint n);
+    PbfK consts(float dt) const;
+    void reorder(float4*& a, hipStream_t s);
+    void k_lambda_acc_init(hipStream_t s);
+
+    rxSphEnv m_env{};
+    rxSimParams m_params{};
+    FusedGrid m_grid;
+    hipStream_t m_stream = nullptr;
+    uint m_n = 0;
+    float m_skin = 0.0f;
+    int m_maxn = 96;
+    float m_tol = 0.01f;
+    bool m_have_list = false;
+    bool m_sync_each_step = false;
+    bool m_record_lambda = false;
+    int m_step = 0;
+    // device state, all in cell order
+    float4* m_x = nullptr;   // positions at the step start
+    float4* m_xs = nullptr;  // predicted / corrected positions
+    float4* m_xl = nullptr;  // (position, lambda) written by lambda passes
+    float4* m_xb = nullptr;  // positions at the list build
+    float4* m_v = nullptr;   // velocities
+    float4* m_t = nullptr;   // gather / output scratch
+    float* m_dens = nullptr;
+    float* m_lamacc = nullptr;  // sum of lambda over the iterations of the last step
+    float* m_ftmp = nullptr;
+    uint* m_id = nullptr;       // slot -> original particle index
+    uint* m_idt = nullptr;
+    uint* m_list = nullptr;     // [slot][particle] skin neighbour list
+    uint* m_cnt = nullptr;
+    float* m_stats = nullptr;
+    // profiling
+    bool m_profile = false;
+    struct Mark {
+        int cls;
+        hipEvent_t a, b;
+    };
+    std::vector<Mark> m_marks;
+    std::vector<hipEvent_t> m_event_pool;
+    size_t m_event_next = 0;
+    double m_prof_ms[16] = {0};
+    long m_prof_cnt[16] = {0};
+    int begin_mark(int cls);
+    void end_mark(int id);
+    void collect_marks();
+};

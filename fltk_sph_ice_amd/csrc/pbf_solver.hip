@@ -1,0 +1,690 @@
+// pbf_solver.hip -- fused position-based-fluids step (rxPBDSPH_GPU::Update, rx_sph_pbd_gpu.cpp:790-957).
+//
+// Data flow per step (state kept in cell order):
+//   predict      density + viscosity at x_n (neighbour list of the previous step), x* = x + dt (v + dt a)
+//   rebuild      stable counting-sort grid on x* (cell = h + skin), gather of the state
+//   list+lambda  one stencil pass over the new grid: skin neighbour list (r <= h + skin) and lambda_0
+//   K x          position correction (list) and lambda (list), ping-pong between two position buffers
+//   velocity     v = (x* - x_n) / dt
+//   XSPH         v += c sum (m/rho_j)(v_j - v_i) W (list)
+// Every list pass reads the largest displacement since the list was built (kept on the device); while it is
+// within skin/2 the list holds every pair within h, otherwise the pass walks a 5x5x5 cell stencil of the
+// build-time grid instead, so the result never depends on the list being fresh.  No host synchronisation
+// happens inside a step.
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "fused_grid.h"
+#include "pbf_solver.h"
+#include "../../include/rx_cu_funcs.h"
+#include "../../include/rx_sph_host.h"
+
+struct PbfK {
+    SphK s;
+    FGridK g;
+    uint n;
+    float rc2;        // (h + skin)^2, list cutoff
+    float half_skin2; // (skin / 2)^2
+    float wide2;      // largest displacement^2 the 5x5x5 stencil still covers
+    int maxn;
+    float eps;
+    float dt, inv_dt;
+    float xsph_c;
+    float tol;        // learned-mode stop criterion (max error)
+};
+
+namespace {
+
+constexpr int kTpb = 256;
+
+// Per-step statistics slots (float/uint in one array).
+enum Stat {
+    ST_ERR_START = 0,        // max (rho - rho0)+ / rho0 at the step's start positions (judge of the previous step)
+    ST_ERR_ITER = 1,         // .. 1 + k: at the start of Jacobi iteration k (k < 16)
+    ST_DISP2 = 17,           // max squared displacement since the list build
+    ST_OVERFLOW = 18,        // particles whose list overflowed
+    ST_ITERS = 19,           // iterations executed (learned mode)
+    ST_FALLBACK = 20,        // list passes that had to walk the wide stencil
+    ST_ERR_FINAL = 21,       // max error after the solve (learned mode: the last check)
+    ST_BAD = 22,             // displacement beyond the wide stencil's reach
+    ST_MEAN_ABS_START = 23,  // sum |rho - rho0| / rho0 at the step start (divide by N)
+    ST_N = 32
+};
+
+
+
+__device__ inline float pow4(float x) {
+    float x2 = x * x;
+    return x2 * x2;
+}
+
+__device__ inline void atomic_max_pos(float* addr, float v) {
+    // non-negative floats order like their bit patterns
+    if (v > 0.0f) atomicMax((unsigned int*)addr, __float_as_uint(v));
+}
+
+// Block-level max of a non-negative value, then one atomic per block.
+__device__ inline void block_max_atomic(float v, float* dst) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+    __shared__ float wm[kTpb / 64];
+    if (lane == 0) wm[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = wm[0];
+        for (int w = 1; w < kTpb / 64; ++w) m = fmaxf(m, wm[w]);
+        atomic_max_pos(dst, m);
+    }
+}
+__device__ inline void block_sum_atomic(float v, float* dst) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    __shared__ float ws[kTpb / 64];
+    if (lane == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = 0.0f;
+        for (int w = 0; w < kTpb / 64; ++w) m += ws[w];
+        atomicAdd(dst, m);
+    }
+}
+
+__device__ inline uint slot_of(const PbfK& k, bool& valid) {
+    uint b = xcd_block(blockIdx.x, gridDim.x);
+    uint i = b * blockDim.x + threadIdx.x;
+    valid = i < k.n;
+    return valid ? i : 0;
+}
+
+__device__ inline float3 clamp_wall(float3 x, const SphK& s) {
+    x.x = fminf(fmaxf(x.x, s.wall_lo.x), s.wall_hi.x);
+    x.y = fminf(fmaxf(x.y, s.wall_lo.y), s.wall_hi.y);
+    x.z = fminf(fmaxf(x.z, s.wall_lo.z), s.wall_hi.z);
+    return x;
+}
+
+// Neighbour candidates of slot i for a pass that needs every j with |x_i - x_j| <= h at the current positions.
+// mode 0: the skin list; mode 1: the 3x3x3 stencil of a grid built on the current positions (cell >= h);
+// overflowed lists and displacements beyond skin/2 use the 5x5x5 stencil of the build-time grid.
+template <class F>
+__device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint* __restrict__ list, const uint* __restrict__ cnt,
+                                     const uint* __restrict__ bhash, const uint* __restrict__ off, float disp2, F&& f) {
+    if (mode == 1) {
+        fg_for_stencil(fg_unhash(bhash[i], k.g), k.g, off, f, 1);
+        return;
+    }
+    uint c = cnt[i];
+    if (c <= (uint)k.maxn && disp2 <= k.half_skin2) {
+        for (uint s = 0; s < c; ++s) f(list[(size_t)s * k.n + i]);
+    } else {
+        fg_for_stencil(fg_unhash(bhash[i], k.g), k.g, off, f, 2);
+    }
+}
+
+// ------------------------------------------------------------------------------------------ predict
+// rxPBDSPH::calDensity + calForceExtAndVisc + integrate (rx_sph_pbd.cpp:599-689, 1409-1446), fused.
+// Reads x, v (w unused); writes x* to xs.  Also judges the previous step: max (rho - rho0)+ / rho0.
+__global__ __launch_bounds__(kTpb) void k_predict(const float4* __restrict__ x, const float4* __restrict__ v, float4* __restrict__ xs,
+                                                  float* __restrict__ dens, float* __restrict__ stats, int mode, PbfK k,
+                                                  const uint* __restrict__ list, const uint* __restrict__ cnt,
+                                                  const uint* __restrict__ bhash, const uint* __restrict__ off) {
+    bool valid;
+    uint i = slot_of(k, valid);
+    float disp2 = stats[ST_DISP2];
+    if (mode == 0 && disp2 > k.wide2 && threadIdx.x == 0) atomicOr((unsigned int*)&stats[ST_BAD], 1u);
+    float err = 0.0f, aerr = 0.0f;
+    if (valid) {
+        float3 p = f3(x[i]), vi = f3(v[i]);
+        float d = 0.0f;
+        float3 fv = f3(0, 0, 0);
+        for_neighbors(i, mode, k, list, cnt, bhash, off, disp2, [&](uint j) {
+            float4 pj = x[j];
+            float3 r = p - f3(pj);
+            float r2 = dot3(r, r);
+            if (r2 <= k.s.h2) {
+                float q = k.s.h2 - r2;
+                d += q * q * q;
+                if (r2 > 0.0f) fv += (k.s.h - sqrtf(r2)) * (f3(v[j]) - vi);
+            }
+        });
+        d *= k.s.mass * k.s.wpoly6;
+        float3 a = (k.s.visc * k.s.mass * k.s.lwvisc / d) * fv + k.s.gravity;  // viscosity / rho_i (rx_sph_pbd.cpp:676)
+        float3 vn = vi + k.dt * a;
+        float3 xn = clamp_wall(p + k.dt * vn, k.s);
+        xs[i] = make_float4(xn.x, xn.y, xn.z, 0.0f);
+        dens[i] = d;
+        float e = (d - k.s.rho0) * k.s.inv_rho0;
+        err = fmaxf(e, 0.0f);
+        aerr = fabsf(e);
+    }
+    block_max_atomic(err, &stats[ST_ERR_START]);
+    block_sum_atomic(aerr, &stats[ST_MEAN_ABS_START]);
+}
+
+// ------------------------------------------------------------------------------------------ list + lambda
+// Builds the skin list of slot i from the new grid (cell = h + skin) and computes lambda at x*
+// (rx_sph_pbd.cpp:699-822: C = rho/rho0 - 1, sd with the self term).  Writes xl = (x*, lambda).
+__global__ __launch_bounds__(kTpb) void k_list_lambda(const float4* __restrict__ xs, float4* __restrict__ xl, float4* __restrict__ xbuild,
+                                                      float* __restrict__ dens, uint* __restrict__ list, uint* __restrict__ cnt,
+                                                      float* __restrict__ stats, PbfK k, const uint* __restrict__ bhash,
+                                                      const uint* __restrict__ off) {
+    bool valid;
+    uint i = slot_of(k, valid);
+    float err = 0.0f;
+    if (valid) {
+        float4 p4 = xs[i];
+        float3 p = f3(p4);
+        float d = 0.0f, sd = 0.0f;
+        float3 gs = f3(0, 0, 0);
+        uint c = 0;
+        fg_for_stencil(fg_unhash(bhash[i], k.g), k.g, off, [&](uint j) {
+            float3 r = p - f3(xs[j]);
+            float r2 = dot3(r, r);
+            if (r2 <= k.rc2) {
+                if (c < (uint)k.maxn) list[(size_t)c * k.n + i] = j;
+                ++c;
+                if (r2 <= k.s.h2) {
+                    float q2 = k.s.h2 - r2;
+                    d += q2 * q2 * q2;
+                    if (r2 > 0.0f) {
+                        float rr = sqrtf(r2);
+                        float q = k.s.h - rr;
+                        float3 g = (k.s.gwspiky * q * q / rr * k.s.inv_rho0) * r;
+                        gs += g;
+                        sd += dot3(g, g);
+                    }
+                }
+            }
+        });
+        cnt[i] = c;
+        if (c > (uint)k.maxn) atomicAdd((unsigned int*)&stats[ST_OVERFLOW], 1u);
+        d *= k.s.mass * k.s.wpoly6;
+        sd += dot3(gs, gs);
+        float C = d * k.s.inv_rho0 - 1.0f;
+        float lam = -C / (sd + k.eps);
+        xl[i] = make_float4(p.x, p.y, p.z, lam);
+        xbuild[i] = p4;
+        dens[i] = d;
+        err = fmaxf(C, 0.0f);
+    }
+    block_max_atomic(err, &stats[ST_ERR_ITER + 0]);
+}
+
+// lambda pass (rx_sph_pbd.cpp:699-822) over the list: reads positions xa, writes xl = (x, lambda).
+// When lam_acc is set the lambda is also accumulated (sum over iterations, the solver's converged multiplier).
+__global__ __launch_bounds__(kTpb) void k_lambda(const float4* __restrict__ xa, float4* __restrict__ xl, float* __restrict__ dens,
+                                                 float* __restrict__ lam_acc, float* __restrict__ stats, int it, PbfK k,
+                                                 const uint* __restrict__ list, const uint* __restrict__ cnt,
+                                                 const uint* __restrict__ bhash, const uint* __restrict__ off, int learned) {
+    if (learned && stats[ST_ERR_ITER + it - 1] <= k.tol && it > 1) return;  // solve already converged
+    bool valid;
+    uint i = slot_of(k, valid);
+    float disp2 = stats[ST_DISP2];
+    float err = 0.0f;
+    if (valid) {
+        float3 p = f3(xa[i]);
+        float d = 0.0f, sd = 0.0f;
+        float3 gs = f3(0, 0, 0);
+        for_neighbors(i, 0, k, list, cnt, bhash, off, disp2, [&](uint j) {
+            float3 r = p - f3(xa[j]);
+            float r2 = dot3(r, r);
+            if (r2 <= k.s.h2) {
+                float q2 = k.s.h2 - r2;
+                d += q2 * q2 * q2;
+                if (r2 > 0.0f) {
+                    float rr = sqrtf(r2);
+                    float q = k.s.h - rr;
+                    float3 g = (k.s.gwspiky * q * q / rr * k.s.inv_rho0) * r;
+                    gs += g;
+                    sd += dot3(g, g);
+                }
+            }
+        });
+        d *= k.s.mass * k.s.wpoly6;
+        sd += dot3(gs, gs);
+        float C = d * k.s.inv_rho0 - 1.0f;
+        float lam = -C / (sd + k.eps);
+        xl[i] = make_float4(p.x, p.y, p.z, lam);
+        dens[i] = d;
+        if (lam_acc) lam_acc[i] += lam;
+        err = fmaxf(C, 0.0f);
+    }
+    block_max_atomic(err, &stats[ST_ERR_ITER + it]);
+}
+
+// Position correction (rx_sph_pbd.cpp:925-1026) + apply + wall (rx_sph_pbd.cpp:436-445): reads xl = (x, lambda),
+// writes xa = clamp(x + dp).  Tracks the displacement from the list build.  In learned mode the pass is
+// skipped once the previous check met the tolerance.
+__global__ __launch_bounds__(kTpb) void k_dp(const float4* __restrict__ xl, float4* __restrict__ xa, const float4* __restrict__ xbuild,
+                                             float* __restrict__ stats, int it, PbfK k, const uint* __restrict__ list,
+                                             const uint* __restrict__ cnt, const uint* __restrict__ bhash,
+                                             const uint* __restrict__ off, int learned) {
+    if (learned && it > 0 && stats[ST_ERR_ITER + it] <= k.tol) return;
+    if (learned && threadIdx.x == 0 && blockIdx.x == 0) atomicAdd((unsigned int*)&stats[ST_ITERS], 1u);
+    bool valid;
+    uint i = slot_of(k, valid);
+    float disp2 = stats[ST_DISP2];
+    float mv = 0.0f;
+    if (valid) {
+        float4 p4 = xl[i];
+        float3 p = f3(p4);
+        float li = p4.w;
+        float dt2 = k.dt * k.dt;
+        float3 acc = f3(0, 0, 0);
+        for_neighbors(i, 0, k, list, cnt, bhash, off, disp2, [&](uint j) {
+            float4 q4 = xl[j];
+            float3 r = p - f3(q4);
+            float r2 = dot3(r, r);
+            if (r2 <= k.s.h2 && r2 > 0.0f) {
+                float rr = sqrtf(r2);
+                float sc = 0.0f;
+                if (k.s.ap) {
+                    float q2 = k.s.h2 - r2;
+                    float ww = k.s.wpoly6 * q2 * q2 * q2 * k.s.inv_ap_wq;
+                    sc = -k.s.ap_k * (k.s.ap_n == 4.0f ? pow4(ww) : powf(ww, k.s.ap_n)) * dt2;
+                }
+                float q = k.s.h - rr;
+                acc += ((li + q4.w + sc) * k.s.gwspiky * q * q / rr * k.s.inv_rho0) * r;
+            }
+        });
+        float3 xn = clamp_wall(p + acc, k.s);
+        xa[i] = make_float4(xn.x, xn.y, xn.z, 0.0f);
+        float3 dd = xn - f3(xbuild[i]);
+        mv = dot3(dd, dd);
+    }
+    block_max_atomic(mv, &stats[ST_DISP2]);
+}
+
+// v = (x* - x_n)/dt (rx_sph_pbd.cpp:465-471), with rho in w for the XSPH pass.
+__global__ void k_velocity(const float4* __restrict__ xa, const float4* __restrict__ x0, const float* __restrict__ dens,
+                           float4* __restrict__ v, PbfK k) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= k.n) return;
+    float3 vv = k.inv_dt * (f3(xa[i]) - f3(x0[i]));
+    v[i] = make_float4(vv.x, vv.y, vv.z, dens[i]);
+}
+
+// XSPH (rx_pbdsph_kernel.cu:1090-1162 form): reads v = (v, rho), writes vout.
+__global__ __launch_bounds__(kTpb) void k_xsph(const float4* __restrict__ xa, const float4* __restrict__ v, float4* __restrict__ vout,
+                                               float* __restrict__ stats, PbfK k, const uint* __restrict__ list,
+                                               const uint* __restrict__ cnt, const uint* __restrict__ bhash,
+                                               const uint* __restrict__ off) {
+    bool valid;
+    uint i = slot_of(k, valid);
+    if (!valid) return;
+    float disp2 = stats[ST_DISP2];
+    float3 p = f3(xa[i]);
+    float3 vi = f3(v[i]);
+    float3 acc = f3(0, 0, 0);
+    for_neighbors(i, 0, k, list, cnt, bhash, off, disp2, [&](uint j) {
+        float3 r = p - f3(xa[j]);
+        float r2 = dot3(r, r);
+        if (r2 <= k.s.h2) {
+            float4 vj = v[j];
+            float q = k.s.h2 - r2;
+            acc += (q * q * q / vj.w) * (f3(vj) - vi);
+        }
+    });
+    float3 vn = vi + (k.xsph_c * k.s.mass * k.s.wpoly6) * acc;
+    vout[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
+}
+
+__global__ void k_scatter_out(const float4* __restrict__ src, const uint* __restrict__ id, float4* __restrict__ dst, uint n) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[id[i]] = src[i];
+}
+__global__ void k_scatter_out1(const float* __restrict__ src, const uint* __restrict__ id, float* __restrict__ dst, uint n) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[id[i]] = src[i];
+}
+__global__ void k_acc_init(const float4* __restrict__ xl, float* __restrict__ acc, uint n) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) acc[i] = xl[i].w;
+}
+__global__ void k_iota(uint* __restrict__ a, uint n) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = i;
+}
+
+}  // namespace
+
+// ============================================================================================ class
+rxPBDSPH_GPU::rxPBDSPH_GPU() {}
+rxPBDSPH_GPU::~rxPBDSPH_GPU() { Finalize(); }
+
+void rxPBDSPH_GPU::Finalize() {
+    if (m_stream) RX_HIP(hipStreamSynchronize(m_stream));
+    for (void** p : {(void**)&m_x, (void**)&m_xs, (void**)&m_xl, (void**)&m_xb, (void**)&m_v, (void**)&m_t, (void**)&m_dens,
+                     (void**)&m_lamacc, (void**)&m_id, (void**)&m_idt, (void**)&m_list, (void**)&m_cnt, (void**)&m_stats,
+                     (void**)&m_ftmp})
+        if (*p) {
+            RX_HIP(hipFree(*p));
+            *p = nullptr;
+        }
+    m_grid.release();
+    for (hipEvent_t e : m_event_pool) RX_HIP(hipEventDestroy(e));
+    m_event_pool.clear();
+    m_marks.clear();
+    m_event_next = 0;
+    if (m_stream) {
+        RX_HIP(hipStreamDestroy(m_stream));
+        m_stream = nullptr;
+    }
+    m_n = 0;
+    m_have_list = false;
+}
+
+int rxPBDSPH_GPU::Initialize(const rxSphEnv& env) {
+    Finalize();
+    if (rxSetupSimParams(&env, &m_params) != 0) return 1;
+    m_env = env;
+    m_skin = 0.1f * env.h;
+    m_maxn = 96;
+    m_grid.setup(m_params, env.h + m_skin);
+    if (!RX_HIP(hipStreamCreateWithFlags(&m_stream, hipStreamNonBlocking))) return 1;
+    return 0;
+}
+
+bool rxPBDSPH_GPU::alloc(uint n) {
+    size_t v4 = sizeof(float4) * n, f = sizeof(float) * n, u = sizeof(uint) * n;
+    bool ok = RX_HIP(hipMalloc(&m_x, v4)) && RX_HIP(hipMalloc(&m_xs, v4)) && RX_HIP(hipMalloc(&m_xl, v4)) &&
+              RX_HIP(hipMalloc(&m_xb, v4)) && RX_HIP(hipMalloc(&m_v, v4)) && RX_HIP(hipMalloc(&m_t, v4)) &&
+              RX_HIP(hipMalloc(&m_dens, f)) && RX_HIP(hipMalloc(&m_lamacc, f)) && RX_HIP(hipMalloc(&m_id, u)) &&
+              RX_HIP(hipMalloc(&m_idt, u)) && RX_HIP(hipMalloc(&m_list, u * (size_t)m_maxn)) && RX_HIP(hipMalloc(&m_cnt, u)) &&
+              RX_HIP(hipMalloc(&m_stats, sizeof(float) * ST_N)) && RX_HIP(hipMalloc(&m_ftmp, f));
+    m_grid.n = n;
+    m_grid.reserve(n);
+    return ok;
+}
+
+int rxPBDSPH_GPU::SetParticles(const float* pos4, const float* vel4, int n) {
+    if (n <= 0 || !pos4) {
+        rx_set_error(1, "rxPBDSPH_GPU::SetParticles: bad arguments");
+        return 1;
+    }
+    if ((uint)n != m_n) {
+        for (void** p : {(void**)&m_x, (void**)&m_xs, (void**)&m_xl, (void**)&m_xb, (void**)&m_v, (void**)&m_t, (void**)&m_dens,
+                         (void**)&m_lamacc, (void**)&m_id, (void**)&m_idt, (void**)&m_list, (void**)&m_cnt, (void**)&m_stats,
+                         (void**)&m_ftmp})
+            if (*p) {
+                RX_HIP(hipFree(*p));
+                *p = nullptr;
+            }
+        m_n = (uint)n;
+        if (!alloc(m_n)) return 1;
+    }
+    hipStream_t s = m_stream;
+    RX_HIP(hipMemcpyAsync(m_x, pos4, sizeof(float4) * n, hipMemcpyHostToDevice, s));
+    if (vel4) RX_HIP(hipMemcpyAsync(m_v, vel4, sizeof(float4) * n, hipMemcpyHostToDevice, s));
+    else rx_fill<float4>(m_v, make_float4(0, 0, 0, 0), n, s);
+    k_iota<<<rx_blocks(n, 256), 256, 0, s>>>(m_id, n);
+    rx_fill<float>(m_lamacc, 0.0f, n, s);
+    rx_fill<float>(m_stats, 0.0f, ST_N, s);
+    m_have_list = false;
+    m_step = 0;
+    return RX_HIP(hipStreamSynchronize(s)) ? 0 : 1;
+}
+
+PbfK rxPBDSPH_GPU::consts(float dt) const {
+    PbfK k;
+    rxSimParams p = m_params;
+    p.Dt = dt;
+    k.s = rx_make_consts(p);
+    k.g = m_grid.k;
+    k.n = m_n;
+    float rc = m_env.h + m_skin;
+    k.rc2 = rc * rc;
+    k.half_skin2 = 0.25f * m_skin * m_skin;
+    float wide = 0.5f * (2.0f * m_grid.k.cell - m_env.h);  // 5x5x5 stencil reach minus h, shared by both particles
+    k.wide2 = wide * wide;
+    k.maxn = m_maxn;
+    k.eps = m_env.epsilon;
+    k.dt = dt;
+    k.inv_dt = 1.0f / dt;
+    k.xsph_c = m_env.xsph_c;
+    k.tol = m_tol;
+    return k;
+}
+
+// Gathers every per-particle state array into the order of the last grid build.
+void rxPBDSPH_GPU::reorder(float4*& a, hipStream_t s) {
+    fg_gather<float4>(m_t, a, m_grid.perm, m_n, s);
+    std::swap(a, m_t);
+}
+
+int rxPBDSPH_GPU::Update(float dt, int step) {
+    if (!m_n) {
+        rx_set_error(1, "rxPBDSPH_GPU::Update: no particles");
+        return 1;
+    }
+    hipStream_t s = m_stream;
+    PbfK k = consts(dt);
+    uint n = m_n;
+    unsigned nb = rx_blocks(n, kTpb);
+    const int K = m_env.max_iter;
+    // per-step statistics (ST_DISP2 carries over: it bounds the validity of the previous list)
+    k_fill<float><<<1, 64, 0, s>>>(m_stats, 0.0f, ST_DISP2);
+    k_fill<float><<<1, 64, 0, s>>>(m_stats + ST_OVERFLOW, 0.0f, ST_N - ST_OVERFLOW);
+
+    // ---- predict at x_n
+    int mode = 0;
+    if (!m_have_list) {
+        // no list yet: grid on x_n, gather, predict with the 3x3x3 stencil of that grid
+        m_grid.build(m_x, s);
+        reorder(m_x, s);
+        reorder(m_v, s);
+        fg_gather<uint>(m_idt, m_id, m_grid.perm, n, s);
+        std::swap(m_id, m_idt);
+        fg_gather<float>(m_ftmp, m_lamacc, m_grid.perm, n, s);
+        std::swap(m_lamacc, m_ftmp);
+        mode = 1;
+    }
+    int mk = begin_mark(RX_PROF_PREDICT);
+    k_predict<<<nb, kTpb, 0, s>>>(m_x, m_v, m_xs, m_dens, m_stats, mode, k, m_list, m_cnt, m_grid.hash, m_grid.off);
+    end_mark(mk);
+
+    // ---- rebuild on x*
+    k_fill<float><<<1, 64, 0, s>>>(m_stats + ST_DISP2, 0.0f, 1);
+    mk = begin_mark(RX_PROF_GRID);
+    m_grid.build(m_xs, s);
+    reorder(m_xs, s);
+    reorder(m_x, s);
+    reorder(m_v, s);
+    fg_gather<uint>(m_idt, m_id, m_grid.perm, n, s);
+    std::swap(m_id, m_idt);
+    fg_gather<float>(m_ftmp, m_lamacc, m_grid.perm, n, s);
+    std::swap(m_lamacc, m_ftmp);
+    end_mark(mk);
+
+    // ---- list + lambda_0, then K Jacobi iterations
+    float* lam_acc = m_record_lambda ? m_lamacc : nullptr;
+    if (lam_acc) rx_fill<float>(lam_acc, 0.0f, n, s);
+    mk = begin_mark(RX_PROF_LIST_LAMBDA);
+    k_list_lambda<<<nb, kTpb, 0, s>>>(m_xs, m_xl, m_xb, m_dens, m_list, m_cnt, m_stats, k, m_grid.hash, m_grid.off);
+    end_mark(mk);
+    if (lam_acc) k_lambda_acc_init(s);
+    for (int it = 0; it < K; ++it) {
+        if (it > 0) {
+            mk = begin_mark(RX_PROF_LAMBDA);
+            k_lambda<<<nb, kTpb, 0, s>>>(m_xs, m_xl, m_dens, lam_acc, m_stats, it, k, m_list, m_cnt, m_grid.hash, m_grid.off, 0);
+            end_mark(mk);
+        }
+        mk = begin_mark(RX_PROF_DP);
+        k_dp<<<nb, kTpb, 0, s>>>(m_xl, m_xs, m_xb, m_stats, it, k, m_list, m_cnt, m_grid.hash, m_grid.off, 0);
+        end_mark(mk);
+    }
+    // ---- velocity, XSPH
+    mk = begin_mark(RX_PROF_VELOCITY);
+    k_velocity<<<rx_blocks(n, 256), 256, 0, s>>>(m_xs, m_x, m_dens, m_t, k);
+    end_mark(mk);
+    if (m_env.xsph_c != 0.0f) {
+        mk = begin_mark(RX_PROF_XSPH);
+        k_xsph<<<nb, kTpb, 0, s>>>(m_xs, m_t, m_v, m_stats, k, m_list, m_cnt, m_grid.hash, m_grid.off);
+        end_mark(mk);
+    } else {
+        std::swap(m_v, m_t);
+    }
+    std::swap(m_x, m_xs);
+    m_have_list = true;
+    ++m_step;
+    if (!RX_HIP(hipGetLastError())) return 1;
+    if (m_sync_each_step && !RX_HIP(hipStreamSynchronize(s))) return 1;
+    return 0;
+}
+
+void rxPBDSPH_GPU::k_lambda_acc_init(hipStream_t s) {
+    // lambda_0 lives in xl.w; copy it into the accumulator
+    k_acc_init<<<rx_blocks(m_n, 256), 256, 0, s>>>(m_xl, m_lamacc, m_n);
+}
+
+int rxPBDSPH_GPU::GetArray(int type, float* host, int count) {
+    if (!host || count < (int)m_n) {
+        rx_set_error(1, "rxPBDSPH_GPU::GetArray: buffer too small");
+        return 1;
+    }
+    hipStream_t s = m_stream;
+    uint n = m_n;
+    float4* tmp = m_t;
+    switch (type) {
+    case RX_ARRAY_POSITION:
+        k_scatter_out<<<rx_blocks(n, 256), 256, 0, s>>>(m_x, m_id, tmp, n);
+        RX_HIP(hipMemcpyAsync(host, tmp, sizeof(float4) * n, hipMemcpyDeviceToHost, s));
+        break;
+    case RX_ARRAY_VELOCITY:
+        k_scatter_out<<<rx_blocks(n, 256), 256, 0, s>>>(m_v, m_id, tmp, n);
+        RX_HIP(hipMemcpyAsync(host, tmp, sizeof(float4) * n, hipMemcpyDeviceToHost, s));
+        break;
+    case RX_ARRAY_DENSITY:
+        k_scatter_out1<<<rx_blocks(n, 256), 256, 0, s>>>(m_dens, m_id, (float*)tmp, n);
+        RX_HIP(hipMemcpyAsync(host, tmp, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        break;
+    case RX_ARRAY_LAMBDA_SUM:
+        k_scatter_out1<<<rx_blocks(n, 256), 256, 0, s>>>(m_lamacc, m_id, (float*)tmp, n);
+        RX_HIP(hipMemcpyAsync(host, tmp, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        break;
+    default:
+        rx_set_error(1, "rxPBDSPH_GPU::GetArray: unknown array %d", type);
+        return 1;
+    }
+    return RX_HIP(hipStreamSynchronize(s)) ? 0 : 1;
+}
+
+int rxPBDSPH_GPU::GetStats(float* out, int count) {
+    if (!out || count <= 0) return 1;
+    float h[ST_N];
+    RX_HIP(hipMemcpyAsync(h, m_stats, sizeof(h), hipMemcpyDeviceToHost, m_stream));
+    if (!RX_HIP(hipStreamSynchronize(m_stream))) return 1;
+    // uint counters are returned as floats
+    h[ST_OVERFLOW] = (float)__builtin_bit_cast(uint, h[ST_OVERFLOW]);
+    h[ST_ITERS] = (float)__builtin_bit_cast(uint, h[ST_ITERS]);
+    h[ST_MEAN_ABS_START] = h[ST_MEAN_ABS_START] / (float)m_n;
+    h[ST_DISP2] = sqrtf(h[ST_DISP2]);
+    for (int i = 0; i < count && i < ST_N; ++i) out[i] = h[i];
+    return 0;
+}
+
+int rxPBDSPH_GPU::Sync() { return RX_HIP(hipStreamSynchronize(m_stream)) ? 0 : 1; }
+
+int rxPBDSPH_GPU::begin_mark(int cls) {
+    if (!m_profile) return -1;
+    if (m_event_next + 2 > m_event_pool.size()) {
+        for (int t = 0; t < 64; ++t) {
+            hipEvent_t e;
+            RX_HIP(hipEventCreate(&e));
+            m_event_pool.push_back(e);
+        }
+    }
+    Mark m;
+    m.cls = cls;
+    m.a = m_event_pool[m_event_next++];
+    m.b = m_event_pool[m_event_next++];
+    RX_HIP(hipEventRecord(m.a, m_stream));
+    m_marks.push_back(m);
+    return (int)m_marks.size() - 1;
+}
+void rxPBDSPH_GPU::end_mark(int id) {
+    if (id < 0) return;
+    RX_HIP(hipEventRecord(m_marks[id].b, m_stream));
+    if (m_event_next > 4096) collect_marks();
+}
+void rxPBDSPH_GPU::collect_marks() {
+    if (m_marks.empty()) return;
+    RX_HIP(hipStreamSynchronize(m_stream));
+    for (const Mark& m : m_marks) {
+        float ms = 0.0f;
+        RX_HIP(hipEventElapsedTime(&ms, m.a, m.b));
+        m_prof_ms[m.cls] += ms;
+        m_prof_cnt[m.cls] += 1;
+    }
+    m_marks.clear();
+    m_event_next = 0;
+}
+int rxPBDSPH_GPU::GetProfile(float* ms, int* count, int n) {
+    collect_marks();
+    for (int i = 0; i < n && i < 16; ++i) {
+        if (ms) ms[i] = (float)m_prof_ms[i];
+        if (count) count[i] = (int)m_prof_cnt[i];
+        m_prof_ms[i] = 0.0;
+        m_prof_cnt[i] = 0;
+    }
+    return 0;
+}
+
+// ============================================================================================ C ABI
+extern "C" {
+
+void* rxPBDSPH_GPU_Create(const rxSphEnv* env) {
+    if (!env) {
+        rx_set_error(1, "rxPBDSPH_GPU_Create: null env");
+        return nullptr;
+    }
+    rxPBDSPH_GPU* p = new rxPBDSPH_GPU();
+    if (p->Initialize(*env) != 0) {
+        delete p;
+        return nullptr;
+    }
+    return p;
+}
+void rxPBDSPH_GPU_Destroy(void* h) { delete (rxPBDSPH_GPU*)h; }
+int rxPBDSPH_GPU_SetParticles(void* h, const float* pos4, const float* vel4, int n) {
+    return h ? ((rxPBDSPH_GPU*)h)->SetParticles(pos4, vel4, n) : 1;
+}
+int rxPBDSPH_GPU_Update(void* h, float dt, int steps) {
+    if (!h) return 1;
+    for (int i = 0; i < steps; ++i)
+        if (((rxPBDSPH_GPU*)h)->Update(dt, i) != 0) return 1;
+    return 0;
+}
+int rxPBDSPH_GPU_GetArray(void* h, int type, float* host, int count) { return h ? ((rxPBDSPH_GPU*)h)->GetArray(type, host, count) : 1; }
+int rxPBDSPH_GPU_GetStats(void* h, float* out, int count) { return h ? ((rxPBDSPH_GPU*)h)->GetStats(out, count) : 1; }
+int rxPBDSPH_GPU_Sync(void* h) { return h ? ((rxPBDSPH_GPU*)h)->Sync() : 1; }
+int rxPBDSPH_GPU_SetOption(void* h, int opt, float value) { return h ? ((rxPBDSPH_GPU*)h)->SetOption(opt, value) : 1; }
+int rxPBDSPH_GPU_GetProfile(void* h, float* ms, int* count, int n) { return h ? ((rxPBDSPH_GPU*)h)->GetProfile(ms, count, n) : 1; }
+
+}  // extern "C"
+
+int rxPBDSPH_GPU::SetOption(int opt, float value) {
+    switch (opt) {
+    case RX_PBF_OPT_SYNC_EACH_STEP: m_sync_each_step = value != 0.0f; return 0;
+    case RX_PBF_OPT_RECORD_LAMBDA: m_record_lambda = value != 0.0f; return 0;
+    case RX_PBF_OPT_TOL: m_tol = value; return 0;
+    case RX_PBF_OPT_ITERATIONS: m_env.max_iter = m_env.min_iter = (int)value; return 0;
+    case RX_PBF_OPT_PROFILE:
+        m_profile = value != 0.0f;
+        return 0;
+    case RX_PBF_OPT_SKIN:
+        if (value < 0.0f) return 1;
+        m_skin = value * m_env.h;
+        m_grid.setup(m_params, m_env.h + m_skin);
+        m_grid.reserve(m_n);
+        m_have_list = false;
+        return 0;
+    default:
+        rx_set_error(1, "rxPBDSPH_GPU::SetOption: unknown option %d", opt);
+        return 1;
+    }
+}

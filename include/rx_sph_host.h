@@ -58,6 +58,62 @@ long rxAbiLayout(int which);
 int rxSceneBox(float* pos4, float* vel4, int max_n, const float origin[3], int nx, int ny, int nz, float spacing,
                float jitter, float vel_sigma, unsigned seed);
 
+/* ---------------------------------------------------------------- solver objects */
+/* Arrays returned by the *_GetArray entry points (original particle order). */
+enum rxArrayType {
+    RX_ARRAY_POSITION = 0,   /* float4 per particle */
+    RX_ARRAY_VELOCITY = 1,   /* float4 per particle */
+    RX_ARRAY_DENSITY = 2,    /* float per particle  */
+    RX_ARRAY_LAMBDA_SUM = 3, /* float per particle: sum of the Jacobi lambdas of the last step (PBF) */
+    RX_ARRAY_PRESSURE = 4    /* float per particle (WCSPH) */
+};
+
+/* Options of rxPBDSPH_GPU_SetOption. */
+enum rxPbfOption {
+    RX_PBF_OPT_SYNC_EACH_STEP = 0, /* 1: Update returns after the step completed on the device        */
+    RX_PBF_OPT_RECORD_LAMBDA = 1,  /* 1: accumulate sum_k lambda_k (RX_ARRAY_LAMBDA_SUM)              */
+    RX_PBF_OPT_TOL = 2,            /* max density error that ends the learned solve (default 0.01)    */
+    RX_PBF_OPT_ITERATIONS = 3,     /* Jacobi iterations per step (plain solve)                        */
+    RX_PBF_OPT_SKIN = 4,           /* neighbour-list skin as a fraction of h (default 0.1)            */
+    RX_PBF_OPT_PROFILE = 5         /* 1: time every kernel class with HIP events (rxPBDSPH_GPU_GetProfile) */
+};
+
+/* Kernel classes of rxPBDSPH_GPU_GetProfile. */
+enum rxProfileClass {
+    RX_PROF_PREDICT = 0,      /* density + viscosity + prediction          */
+    RX_PROF_GRID = 1,         /* hash, counting sort, state gather         */
+    RX_PROF_LIST_LAMBDA = 2,  /* neighbour-list build + first lambda       */
+    RX_PROF_LAMBDA = 3,       /* lambda pass                               */
+    RX_PROF_DP = 4,           /* position correction + apply + walls       */
+    RX_PROF_VELOCITY = 5,
+    RX_PROF_XSPH = 6,
+    RX_PROF_FEATURES = 7,     /* learned solve: feature pass               */
+    RX_PROF_MLP = 8           /* learned solve: network inference          */
+};
+
+/* Per-step statistics of rxPBDSPH_GPU_GetStats (float slots):
+ *   0      max (rho - rho0)+ / rho0 at the step's start positions (= end of the previous step)
+ *   1 + k  the same at the start of Jacobi iteration k
+ *   17     largest displacement since the neighbour-list build
+ *   18     particles whose neighbour list overflowed
+ *   19     iterations executed (learned solve)
+ *   22     1 when a displacement exceeded what the wide cell stencil covers
+ *   23     mean |rho - rho0| / rho0 at the step's start positions                              */
+
+/* Fused PBF solver (rxPBDSPH_GPU, rx_sph.h:1284-1601): Initialize (rx_sph_pbd_gpu.cpp:168-300) and Update
+ * (rx_sph_pbd_gpu.cpp:790-957) with min_iter = max_iter Jacobi iterations, CPU-path lambda and collisions,
+ * and the XSPH pass when env->xsph_c != 0.  Update runs `steps` steps asynchronously on the solver's stream. */
+void* rxPBDSPH_GPU_Create(const rxSphEnv* env);
+void rxPBDSPH_GPU_Destroy(void* solver);
+int rxPBDSPH_GPU_SetParticles(void* solver, const float* pos4, const float* vel4, int n);
+int rxPBDSPH_GPU_Update(void* solver, float dt, int steps);
+int rxPBDSPH_GPU_GetArray(void* solver, int type, float* host, int count);
+int rxPBDSPH_GPU_GetStats(void* solver, float* out, int count);
+int rxPBDSPH_GPU_Sync(void* solver);
+int rxPBDSPH_GPU_SetOption(void* solver, int option, float value);
+/* Milliseconds and launch counts per rxProfileClass since the last call (n entries each). */
+int rxPBDSPH_GPU_GetProfile(void* solver, float* ms, int* count, int n);
+
 #ifdef __cplusplus
 }
 #endif
