@@ -70,6 +70,16 @@ class rxParticleCell(C.Structure):
     ]
 
 
+class rxIceEnv(C.Structure):
+    """include/rx_ice_funcs.h."""
+
+    _fields_ = [
+        ("alpha", C.c_float), ("beta", C.c_float), ("dt", C.c_float), ("gravity", f3), ("box_min", f3),
+        ("box_max", f3), ("iterations", C.c_int), ("volume_conservation", C.c_int), ("restitution", C.c_float),
+        ("cluster_radius", C.c_float),
+    ]
+
+
 class rxSphEnv(C.Structure):
     """include/rx_sph_host.h."""
 
@@ -156,6 +166,34 @@ SIGNATURES: dict[str, tuple] = {
     "rxPBDSPH_GPU_Sync": (i32, [vp]),
     "rxPBDSPH_GPU_SetOption": (i32, [vp, i32, flt]),
     "rxPBDSPH_GPU_GetProfile": (i32, [vp, vp, vp, i32]),
+    "rxSPH_GPU_Create": (vp, [C.POINTER(rxSphEnv)]),
+    "rxSPH_GPU_Destroy": (None, [vp]),
+    "rxSPH_GPU_SetParticles": (i32, [vp, vp, vp, i32]),
+    "rxSPH_GPU_Update": (i32, [vp, flt, i32]),
+    "rxSPH_GPU_GetArray": (i32, [vp, i32, vp, i32]),
+    "rxSPH_GPU_Sync": (i32, [vp]),
+    "rxSPH_GPU_UseGraph": (i32, [vp, i32]),
+    # rx_ice_funcs.h
+    "rxIceEnvDefault": (None, [C.POINTER(rxIceEnv)]),
+    "rxIceSM_Create": (vp, [C.POINTER(rxIceEnv), vp, i32]),
+    "rxIceSM_Destroy": (None, [vp]),
+    "rxIceSM_SetState": (i32, [vp, vp, vp]),
+    "rxIceSM_GetState": (i32, [vp, vp, vp]),
+    "rxIceSM_Update": (i32, [vp, i32]),
+    "rxIceSM_Sync": (i32, [vp]),
+    "rxIceSM_ClusterInfo": (i32, [vp, C.POINTER(i32), C.POINTER(C.c_long), vp, i32]),
+    "rxIceSM_GetClusters": (i32, [vp, vp, vp]),
+    "rxIceSM_Autotune": (i32, [vp, vp]),
+    "rxIceSM_SetTiles": (i32, [vp, vp]),
+    "rxIceSM_UseGraph": (i32, [vp, i32]),
+    "rxIceSM_KernelTimes": (i32, [vp, fp, fp]),
+}
+
+# C++-linkage entry points (Ice_SM.h / IceObject.h signatures), resolved by their mangled names.
+CXX_SYMBOLS = {
+    "LaunchShapeMatchingGPU": ("_Z22LaunchShapeMatchingGPUiPfS_S_S_S_S_S_PiS0_f",
+                               (None, [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, flt])),
+    "LaunchInterPolationGPU": ("_Z22LaunchInterPolationGPUiPfS_S_S_", (None, [i32, vp, vp, vp, vp])),
 }
 
 RX_ARRAY_POSITION, RX_ARRAY_VELOCITY, RX_ARRAY_DENSITY, RX_ARRAY_LAMBDA_SUM, RX_ARRAY_PRESSURE = range(5)
@@ -177,6 +215,11 @@ def load() -> C.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    for name, (mangled, (res, args)) in CXX_SYMBOLS.items():
+        fn = getattr(lib, mangled)
+        fn.restype = res
+        fn.argtypes = args
+        setattr(lib, name, fn)
     _lib = lib
     return lib
 

@@ -71,6 +71,7 @@ struct FusedGrid {
     uint* tmp_hash = nullptr;
     uint* slot = nullptr;
     uint* tsrc = nullptr;
+    void* scan_ws = nullptr;  // own scan workspace, so that a rebuild can be captured into a graph
     size_t cap_n = 0, cap_c = 0;
 
     void setup(const rxSimParams& p, float cell);

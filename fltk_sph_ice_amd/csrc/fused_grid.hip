@@ -84,7 +84,9 @@ void FusedGrid::reserve(uint np) {
     }
     if ((size_t)k.ncells + 1 > cap_c) {
         if (off) RX_HIP(hipFree(off));
+        if (scan_ws) RX_HIP(hipFree(scan_ws));
         RX_HIP(hipMalloc(&off, sizeof(uint) * ((size_t)k.ncells + 2)));
+        RX_HIP(hipMalloc(&scan_ws, rx_scan_ws_bytes((size_t)k.ncells + 1)));
         cap_c = (size_t)k.ncells + 1;
     }
 }
@@ -95,6 +97,10 @@ void FusedGrid::release() {
             RX_HIP(hipFree(*p));
             *p = nullptr;
         }
+    if (scan_ws) {
+        RX_HIP(hipFree(scan_ws));
+        scan_ws = nullptr;
+    }
     cap_n = cap_c = 0;
 }
 
@@ -103,7 +109,7 @@ void FusedGrid::build(const float4* pos, hipStream_t s) {
     rx_fill<uint>(off, 0u, (size_t)k.ncells + 1, s);
     unsigned b = rx_blocks(n, 256);
     k_fg_hash_count<<<b, 256, 0, s>>>(pos, n, k, tmp_hash, slot, off);
-    rx_scan_u32(off, off, k.ncells, true, s);  // off[c] = first slot of cell c, off[ncells] = n
+    rx_scan_u32(off, off, k.ncells, true, s, scan_ws);  // off[c] = first slot of cell c, off[ncells] = n
     k_fg_scatter<<<b, 256, 0, s>>>(tmp_hash, slot, off, n, tsrc, perm /* as scratch for the cell ids */);
     // perm doubles as the cell-id scratch of the scatter; the rank pass reads it and writes hash/perm anew
     k_fg_rank<<<b, 256, 0, s>>>(tsrc, perm, off, n, slot /* perm out */, hash);

@@ -102,5 +102,8 @@ template <class T> inline void rx_fill(T* p, T v, size_t n, hipStream_t s) {
 
 // Device-wide exclusive scan of uint (n elements); out may alias in; also writes the total to out[n] when
 // write_total is set (out must then hold n + 1 elements).
-void rx_scan_u32(const uint* in, uint* out, size_t n, bool write_total, hipStream_t s);
+// ws: optional caller-owned workspace of rx_scan_ws_bytes(n) bytes (required inside graph capture, where
+// the library scratch must not be allocated); null uses library scratch.
+void rx_scan_u32(const uint* in, uint* out, size_t n, bool write_total, hipStream_t s, void* ws = nullptr);
+size_t rx_scan_ws_bytes(size_t n);
 void rx_scan_f32(const float* in, float* out, size_t n, hipStream_t s);

@@ -173,37 +173,49 @@ template <class T> __global__ void k_write_total(const T* __restrict__ last_in, 
 }
 
 // Recursive three-phase scan: per-tile sums, scan of the sums, rescan of the tiles with their offsets.
-// Scratch slots 8.. are used by the recursion levels.
-template <class T> void scan_impl(const T* in, T* out, size_t n, hipStream_t s, int level) {
+// The tile sums of every recursion level live in ws (rx_scan_ws_bytes), or in library scratch slots 8..
+// when ws is null.
+template <class T> void scan_impl(const T* in, T* out, size_t n, hipStream_t s, int level, char* ws) {
     if (n == 0) return;
     size_t tiles = (n + kScanTile - 1) / kScanTile;
     if (tiles == 1) {
         k_scan_apply<T><<<1, kScanTpb, 0, s>>>(in, out, nullptr, n);
         return;
     }
-    T* sums = (T*)rx_scratch(8 + level, sizeof(T) * tiles);
+    T* sums = ws ? (T*)ws : (T*)rx_scratch(8 + level, sizeof(T) * tiles);
+    char* next = ws ? ws + ((sizeof(T) * tiles + 255) & ~(size_t)255) : nullptr;
     k_scan_reduce<T><<<(unsigned)tiles, kScanTpb, 0, s>>>(in, sums, n);
-    scan_impl<T>(sums, sums, tiles, s, level + 1);
+    scan_impl<T>(sums, sums, tiles, s, level + 1, next);
     k_scan_apply<T><<<(unsigned)tiles, kScanTpb, 0, s>>>(in, out, sums, n);
 }
 }  // namespace
+
+size_t rx_scan_ws_bytes(size_t n) {
+    size_t b = 256;  // the saved last element
+    while (n > (size_t)kScanTile) {
+        n = (n + kScanTile - 1) / kScanTile;
+        b += ((sizeof(uint) * n + 255) & ~(size_t)255);
+    }
+    return b + 256;
+}
 
 __global__ void k_copy_last(const uint* __restrict__ in, uint* __restrict__ last, size_t n) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *last = in[n - 1];
 }
 
-void rx_scan_u32(const uint* in, uint* out, size_t n, bool write_total, hipStream_t s) {
+void rx_scan_u32(const uint* in, uint* out, size_t n, bool write_total, hipStream_t s, void* ws) {
+    char* w = (char*)ws;
     if (!write_total) {
-        scan_impl<uint>(in, out, n, s, 0);
+        scan_impl<uint>(in, out, n, s, 0, w ? w + 256 : nullptr);
         return;
     }
     // keep the last input element: the scan may overwrite it when out aliases in
-    uint* last = (uint*)rx_scratch(7, sizeof(uint));
+    uint* last = w ? (uint*)w : (uint*)rx_scratch(7, sizeof(uint));
     if (n) k_copy_last<<<1, 64, 0, s>>>(in, last, n);
-    scan_impl<uint>(in, out, n, s, 0);
+    scan_impl<uint>(in, out, n, s, 0, w ? w + 256 : nullptr);
     k_write_total<uint><<<1, 64, 0, s>>>(last, out, out + n, n);
 }
-void rx_scan_f32(const float* in, float* out, size_t n, hipStream_t s) { scan_impl<float>(in, out, n, s, 0); }
+void rx_scan_f32(const float* in, float* out, size_t n, hipStream_t s) { scan_impl<float>(in, out, n, s, 0, nullptr); }
 
 // ============================================================================================ grid kernels
 namespace {

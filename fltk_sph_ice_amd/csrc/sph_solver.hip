@@ -1,0 +1,282 @@
+// sph_solver.hip -- fused WCSPH step (rxSPH_GPU::Update, rx_sph_gpu.cpp:756-923; CPU path rxSPH::Update,
+// rx_sph_cpu.cpp:323-380).
+//
+// Per step, state kept in cell order:
+//   rebuild     stable counting-sort grid (cell = h) on x, gather x/v/id into the working buffers
+//   density     rho_i and p_i/rho_i^2 (float2 per particle) over the 27-cell stencil
+//   force       pressure + viscosity + gravity and the integration with the CPU wall rule, written back to
+//               the state buffers
+// The buffers never swap, so a step is a fixed launch sequence that is captured once into a hipGraph and
+// replayed (the 32k-particle workload is launch-latency bound).
+#include <vector>
+
+#include "fused_grid.h"
+#include "../../include/rx_cu_funcs.h"
+#include "../../include/rx_sph_host.h"
+
+namespace {
+
+constexpr int kTpb = 256;
+
+struct SphStepK {
+    SphK s;
+    FGridK g;
+    uint n;
+    float dt;
+};
+
+__device__ inline uint slot_xcd(uint n, bool& valid) {
+    uint b = xcd_block(blockIdx.x, gridDim.x);
+    uint i = b * blockDim.x + threadIdx.x;
+    valid = i < n;
+    return valid ? i : 0;
+}
+
+// rxSPH::calDensity + pressure (rx_sph_cpu.cpp:495-571)
+__global__ __launch_bounds__(kTpb) void k_density(const float4* __restrict__ x, float2* __restrict__ dp, const uint* __restrict__ bhash,
+                                                  const uint* __restrict__ off, SphStepK k) {
+    bool valid;
+    uint i = slot_xcd(k.n, valid);
+    if (!valid) return;
+    float3 p = f3(x[i]);
+    float d = 0.0f;
+    fg_for_stencil(fg_unhash(bhash[i], k.g), k.g, off, [&](uint j) {
+        float3 r = p - f3(x[j]);
+        float r2 = dot3(r, r);
+        if (r2 <= k.s.h2) {
+            float q = k.s.h2 - r2;
+            d += q * q * q;
+        }
+    });
+    d *= k.s.mass * k.s.wpoly6;
+    float pr = k.s.gas_k * (d - k.s.rho0);
+    dp[i] = make_float2(d, pr / (d * d));
+}
+
+// rxSPH::calForce + integrate (rx_sph_cpu.cpp:574-614, 1040-1077); writes the new state.
+__global__ __launch_bounds__(kTpb) void k_force_integrate(const float4* __restrict__ x, const float4* __restrict__ v,
+                                                          const float2* __restrict__ dp, const uint* __restrict__ bhash,
+                                                          const uint* __restrict__ off, float4* __restrict__ xo,
+                                                          float4* __restrict__ vo, SphStepK k) {
+    bool valid;
+    uint i = slot_xcd(k.n, valid);
+    if (!valid) return;
+    float3 p = f3(x[i]), vi = f3(v[i]);
+    float prsi = dp[i].y;
+    float3 fp = f3(0, 0, 0), fv = f3(0, 0, 0);
+    fg_for_stencil(fg_unhash(bhash[i], k.g), k.g, off, [&](uint j) {
+        float3 r = p - f3(x[j]);
+        float r2 = dot3(r, r);
+        if (r2 <= k.s.h2 && r2 > 0.0f) {
+            float rr = sqrtf(r2);
+            float2 dj = dp[j];
+            float q = k.s.h - rr;
+            fp += ((prsi + dj.y) * q * q / rr) * r;
+            fv += (q / dj.x) * (f3(v[j]) - vi);
+        }
+    });
+    float3 a = (-k.s.mass * k.s.gwspiky) * fp + (k.s.visc * k.s.mass * k.s.lwvisc) * fv + k.s.gravity;
+    float3 vn = vi + k.dt * a;
+    float3 xn = p + k.dt * vn;
+    xn.x = fminf(fmaxf(xn.x, k.s.wall_lo.x), k.s.wall_hi.x);
+    xn.y = fminf(fmaxf(xn.y, k.s.wall_lo.y), k.s.wall_hi.y);
+    xn.z = fminf(fmaxf(xn.z, k.s.wall_lo.z), k.s.wall_hi.z);
+    xo[i] = make_float4(xn.x, xn.y, xn.z, 0.0f);
+    vo[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
+}
+
+__global__ void k_scatter4(const float4* __restrict__ src, const uint* __restrict__ id, float4* __restrict__ dst, uint n) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[id[i]] = src[i];
+}
+__global__ void k_scatter_d(const float2* __restrict__ src, const uint* __restrict__ id, float* __restrict__ dst, int comp, float gas_k,
+                            float rho0, uint n) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float2 d = src[i];
+    dst[id[i]] = comp == 0 ? d.x : gas_k * (d.x - rho0);
+}
+__global__ void k_iota2(uint* __restrict__ a, uint n) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = i;
+}
+
+}  // namespace
+
+// ============================================================================================ class
+class rxSPH_GPU {
+public:
+    ~rxSPH_GPU() { Finalize(); }
+    int Initialize(const rxSphEnv& env) {
+        Finalize();
+        if (rxSetupSimParams(&env, &m_params) != 0) return 1;
+        m_env = env;
+        m_grid.setup(m_params, env.h);
+        return RX_HIP(hipStreamCreateWithFlags(&m_stream, hipStreamNonBlocking)) ? 0 : 1;
+    }
+    void Finalize() {
+        if (m_stream) RX_HIP(hipStreamSynchronize(m_stream));
+        drop_graph();
+        for (void** p : {(void**)&m_x, (void**)&m_v, (void**)&m_xw, (void**)&m_vw, (void**)&m_dp, (void**)&m_id, (void**)&m_idw})
+            if (*p) {
+                RX_HIP(hipFree(*p));
+                *p = nullptr;
+            }
+        m_grid.release();
+        if (m_stream) {
+            RX_HIP(hipStreamDestroy(m_stream));
+            m_stream = nullptr;
+        }
+        m_n = 0;
+    }
+    int SetParticles(const float* pos4, const float* vel4, int n) {
+        if (n <= 0 || !pos4) {
+            rx_set_error(1, "rxSPH_GPU::SetParticles: bad arguments");
+            return 1;
+        }
+        if ((uint)n != m_n) {
+            drop_graph();
+            for (void** p : {(void**)&m_x, (void**)&m_v, (void**)&m_xw, (void**)&m_vw, (void**)&m_dp, (void**)&m_id, (void**)&m_idw})
+                if (*p) {
+                    RX_HIP(hipFree(*p));
+                    *p = nullptr;
+                }
+            m_n = (uint)n;
+            size_t v4 = sizeof(float4) * n;
+            if (!(RX_HIP(hipMalloc(&m_x, v4)) && RX_HIP(hipMalloc(&m_v, v4)) && RX_HIP(hipMalloc(&m_xw, v4)) &&
+                  RX_HIP(hipMalloc(&m_vw, v4)) && RX_HIP(hipMalloc(&m_dp, sizeof(float2) * n)) &&
+                  RX_HIP(hipMalloc(&m_id, sizeof(uint) * n)) && RX_HIP(hipMalloc(&m_idw, sizeof(uint) * n))))
+                return 1;
+            m_grid.n = m_n;
+            m_grid.reserve(m_n);
+        }
+        RX_HIP(hipMemcpyAsync(m_x, pos4, sizeof(float4) * n, hipMemcpyHostToDevice, m_stream));
+        if (vel4) RX_HIP(hipMemcpyAsync(m_v, vel4, sizeof(float4) * n, hipMemcpyHostToDevice, m_stream));
+        else rx_fill<float4>(m_v, make_float4(0, 0, 0, 0), n, m_stream);
+        k_iota2<<<rx_blocks(n, 256), 256, 0, m_stream>>>(m_id, n);
+        return RX_HIP(hipStreamSynchronize(m_stream)) ? 0 : 1;
+    }
+    // One step as a fixed launch sequence on m_stream.
+    void enqueue_step(float dt) {
+        hipStream_t s = m_stream;
+        SphStepK k;
+        rxSimParams p = m_params;
+        p.Dt = dt;
+        k.s = rx_make_consts(p);
+        k.g = m_grid.k;
+        k.n = m_n;
+        k.dt = dt;
+        m_grid.build(m_x, s);
+        fg_gather<float4>(m_xw, m_x, m_grid.perm, m_n, s);
+        fg_gather<float4>(m_vw, m_v, m_grid.perm, m_n, s);
+        fg_gather<uint>(m_idw, m_id, m_grid.perm, m_n, s);
+        RX_HIP(hipMemcpyAsync(m_id, m_idw, sizeof(uint) * m_n, hipMemcpyDeviceToDevice, s));
+        unsigned nb = rx_blocks(m_n, kTpb);
+        k_density<<<nb, kTpb, 0, s>>>(m_xw, m_dp, m_grid.hash, m_grid.off, k);
+        k_force_integrate<<<nb, kTpb, 0, s>>>(m_xw, m_vw, m_dp, m_grid.hash, m_grid.off, m_x, m_v, k);
+    }
+    int Update(float dt, int steps) {
+        if (!m_n) {
+            rx_set_error(1, "rxSPH_GPU::Update: no particles");
+            return 1;
+        }
+        for (int t = 0; t < steps; ++t) {
+            if (!m_use_graph) {
+                enqueue_step(dt);
+                continue;
+            }
+            if (!m_graph_exec || m_graph_dt != dt) {
+                // run this step eagerly, then capture the launch sequence for the following steps
+                drop_graph();
+                enqueue_step(dt);
+                hipGraph_t g;
+                if (!RX_HIP(hipStreamBeginCapture(m_stream, hipStreamCaptureModeThreadLocal))) return 1;
+                enqueue_step(dt);
+                if (!RX_HIP(hipStreamEndCapture(m_stream, &g))) return 1;
+                if (!RX_HIP(hipGraphInstantiate(&m_graph_exec, g, nullptr, nullptr, 0))) return 1;
+                RX_HIP(hipGraphDestroy(g));
+                m_graph_dt = dt;
+                continue;
+            }
+            if (!RX_HIP(hipGraphLaunch(m_graph_exec, m_stream))) return 1;
+        }
+        return RX_HIP(hipGetLastError()) ? 0 : 1;
+    }
+    int GetArray(int type, float* host, int count) {
+        if (!host || count < (int)m_n) {
+            rx_set_error(1, "rxSPH_GPU::GetArray: buffer too small");
+            return 1;
+        }
+        hipStream_t s = m_stream;
+        uint n = m_n;
+        switch (type) {
+        case RX_ARRAY_POSITION:
+        case RX_ARRAY_VELOCITY:
+            k_scatter4<<<rx_blocks(n, 256), 256, 0, s>>>(type == RX_ARRAY_POSITION ? m_x : m_v, m_id, m_vw, n);
+            RX_HIP(hipMemcpyAsync(host, m_vw, sizeof(float4) * n, hipMemcpyDeviceToHost, s));
+            break;
+        case RX_ARRAY_DENSITY:
+        case RX_ARRAY_PRESSURE:
+            // densities of the last step, indexed by the order of that step's grid (m_idw)
+            k_scatter_d<<<rx_blocks(n, 256), 256, 0, s>>>(m_dp, m_idw, (float*)m_vw, type == RX_ARRAY_DENSITY ? 0 : 1,
+                                                           m_params.GasStiffness, m_params.Density, n);
+            RX_HIP(hipMemcpyAsync(host, m_vw, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+            break;
+        default:
+            rx_set_error(1, "rxSPH_GPU::GetArray: unknown array %d", type);
+            return 1;
+        }
+        return RX_HIP(hipStreamSynchronize(s)) ? 0 : 1;
+    }
+    int Sync() { return RX_HIP(hipStreamSynchronize(m_stream)) ? 0 : 1; }
+    void UseGraph(bool on) {
+        if (!on) drop_graph();
+        m_use_graph = on;
+    }
+
+private:
+    void drop_graph() {
+        if (m_graph_exec) {
+            RX_HIP(hipGraphExecDestroy(m_graph_exec));
+            m_graph_exec = nullptr;
+        }
+    }
+    rxSphEnv m_env{};
+    rxSimParams m_params{};
+    FusedGrid m_grid;
+    hipStream_t m_stream = nullptr;
+    hipGraphExec_t m_graph_exec = nullptr;
+    float m_graph_dt = 0.0f;
+    bool m_use_graph = true;
+    uint m_n = 0;
+    float4 *m_x = nullptr, *m_v = nullptr, *m_xw = nullptr, *m_vw = nullptr;
+    float2* m_dp = nullptr;
+    uint *m_id = nullptr, *m_idw = nullptr;
+};
+
+extern "C" {
+
+void* rxSPH_GPU_Create(const rxSphEnv* env) {
+    if (!env) {
+        rx_set_error(1, "rxSPH_GPU_Create: null env");
+        return nullptr;
+    }
+    rxSPH_GPU* p = new rxSPH_GPU();
+    if (p->Initialize(*env) != 0) {
+        delete p;
+        return nullptr;
+    }
+    return p;
+}
+void rxSPH_GPU_Destroy(void* h) { delete (rxSPH_GPU*)h; }
+int rxSPH_GPU_SetParticles(void* h, const float* pos4, const float* vel4, int n) { return h ? ((rxSPH_GPU*)h)->SetParticles(pos4, vel4, n) : 1; }
+int rxSPH_GPU_Update(void* h, float dt, int steps) { return h ? ((rxSPH_GPU*)h)->Update(dt, steps) : 1; }
+int rxSPH_GPU_GetArray(void* h, int type, float* host, int count) { return h ? ((rxSPH_GPU*)h)->GetArray(type, host, count) : 1; }
+int rxSPH_GPU_Sync(void* h) { return h ? ((rxSPH_GPU*)h)->Sync() : 1; }
+int rxSPH_GPU_UseGraph(void* h, int on) {
+    if (!h) return 1;
+    ((rxSPH_GPU*)h)->UseGraph(on != 0);
+    return 0;
+}
+
+}  // extern "C"

@@ -121,3 +121,56 @@ def pbf_1m(seed: int = 5) -> Scene:
     sc = pbf_dam_break(128, 128, 64, seed=seed, box=(12.8, 6.4, 6.4))
     sc.name = "pbf_1m"
     return sc
+
+
+@dataclass
+class IceScene:
+    """An ice block for batched shape matching: jittered lattice at rest, one cluster per particle."""
+
+    name: str
+    block: tuple
+    spacing: float
+    origin: tuple
+    box_min: tuple
+    box_max: tuple
+    jitter: float
+    impulse_sigma: float
+    seed_pos: int
+    seed_vel: int
+    alpha: float = 0.9
+    beta: float = 0.3
+    dt: float = 1.0 / 120.0
+    iterations: int = 2
+    radius_spacings: float = 2.5
+    gravity: tuple = (0.0, -9.8, 0.0)
+
+    @property
+    def n(self) -> int:
+        return int(np.prod(self.block))
+
+    @property
+    def cluster_radius(self) -> float:
+        return self.radius_spacings * self.spacing
+
+    def particles(self):
+        lib = load()
+        n = self.n
+        pos = np.zeros((n, 4), np.float32)
+        vel = np.zeros((n, 4), np.float32)
+        dummy = np.zeros((n, 4), np.float32)
+        org = (C.c_float * 3)(*self.origin)
+        lib.rxSceneBox(pos.ctypes.data, None, n, org, *self.block, self.spacing, self.jitter, 0.0, self.seed_pos)
+        lib.rxSceneBox(dummy.ctypes.data, vel.ctypes.data, n, org, *self.block, self.spacing, 0.0, self.impulse_sigma,
+                       self.seed_vel)
+        return pos, vel
+
+
+def ice_block(nx: int = 64, ny: int = 64, nz: int = 32, seed_pos: int = 3, seed_vel: int = 4) -> IceScene:
+    """Batched ice shape matching: 64 x 64 x 32 = 131,072 particles on a jittered lattice (spacing 0.05,
+    jitter 1% of the spacing) 0.5 above the floor of a box; clusters from the neighbours within 2.5
+    spacings; alpha = 0.9, beta = 0.3, 2 shape-matching iterations, gravity plus an N(0, 0.1) impulse."""
+    s = 0.05
+    ext = (nx * s, ny * s, nz * s)
+    box_max = (2.0 * ext[0] + 1.0, ext[1] + 1.5, 2.0 * ext[2] + 1.0)
+    origin = (0.5 * (box_max[0] - ext[0]), 0.5, 0.5 * (box_max[2] - ext[2]))
+    return IceScene("ice_block", (nx, ny, nz), s, origin, (0.0, 0.0, 0.0), box_max, 0.01 * s, 0.1, seed_pos, seed_vel)

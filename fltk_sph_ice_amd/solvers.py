@@ -86,3 +86,134 @@ class PBFSolver:
             self.close()
         except Exception:
             pass
+
+
+class SPHSolver:
+    """rxSPH_GPU (fused WCSPH step, hipGraph replay after the first step) for a Scene."""
+
+    def __init__(self, scene, pos4: np.ndarray | None = None, vel4: np.ndarray | None = None, graph: bool = True):
+        self.lib = load()
+        self.scene = scene
+        env = scene.env()
+        self.h = self.lib.rxSPH_GPU_Create(C.byref(env))
+        if not self.h:
+            check(self.lib)
+            raise RuntimeError("rxSPH_GPU_Create failed")
+        self.lib.rxSPH_GPU_UseGraph(self.h, 1 if graph else 0)
+        if pos4 is None:
+            pos4, vel4 = scene.particles()
+        pos4 = np.ascontiguousarray(pos4, np.float32)
+        self.n = pos4.shape[0]
+        v = None if vel4 is None else np.ascontiguousarray(vel4, np.float32)
+        if self.lib.rxSPH_GPU_SetParticles(self.h, pos4.ctypes.data, None if v is None else v.ctypes.data, self.n):
+            check(self.lib)
+
+    def step(self, steps: int = 1, dt: float | None = None):
+        if self.lib.rxSPH_GPU_Update(self.h, self.scene.dt if dt is None else dt, steps):
+            check(self.lib)
+
+    def sync(self):
+        if self.lib.rxSPH_GPU_Sync(self.h):
+            check(self.lib)
+
+    def array(self, kind: int) -> np.ndarray:
+        vec = kind in (_abi.RX_ARRAY_POSITION, _abi.RX_ARRAY_VELOCITY)
+        out = np.zeros((self.n, 4) if vec else self.n, np.float32)
+        if self.lib.rxSPH_GPU_GetArray(self.h, kind, out.ctypes.data, self.n):
+            check(self.lib)
+        return out
+
+    def close(self):
+        if self.h:
+            self.lib.rxSPH_GPU_Destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class IceSM:
+    """rxIceSM: batched shape matching of one ice body (include/rx_ice_funcs.h)."""
+
+    NBINS = 5
+
+    def __init__(self, rest_pos4: np.ndarray, alpha=0.9, beta=0.3, dt=1.0 / 120.0, gravity=(0.0, -9.8, 0.0),
+                 box_min=(-10.0, -10.0, -10.0), box_max=(10.0, 10.0, 10.0), iterations=2, volume_conservation=1,
+                 restitution=0.9, cluster_radius=0.1, graph=True):
+        self.lib = load()
+        e = _abi.rxIceEnv()
+        self.lib.rxIceEnvDefault(C.byref(e))
+        e.alpha, e.beta, e.dt = alpha, beta, dt
+        for k in range(3):
+            e.gravity[k], e.box_min[k], e.box_max[k] = gravity[k], box_min[k], box_max[k]
+        e.iterations, e.volume_conservation, e.restitution, e.cluster_radius = iterations, volume_conservation, restitution, cluster_radius
+        self.env = e
+        rest = np.ascontiguousarray(rest_pos4, np.float32)
+        self.n = rest.shape[0]
+        self.h = self.lib.rxIceSM_Create(C.byref(e), rest.ctypes.data, self.n)
+        if not self.h:
+            check(self.lib)
+            raise RuntimeError("rxIceSM_Create failed")
+        self.lib.rxIceSM_UseGraph(self.h, 1 if graph else 0)
+
+    def set_state(self, pos4, vel4=None):
+        p = np.ascontiguousarray(pos4, np.float32)
+        v = None if vel4 is None else np.ascontiguousarray(vel4, np.float32)
+        if self.lib.rxIceSM_SetState(self.h, p.ctypes.data, None if v is None else v.ctypes.data):
+            check(self.lib)
+
+    def state(self):
+        p = np.zeros((self.n, 4), np.float32)
+        v = np.zeros((self.n, 4), np.float32)
+        if self.lib.rxIceSM_GetState(self.h, p.ctypes.data, v.ctypes.data):
+            check(self.lib)
+        return p, v
+
+    def step(self, steps: int = 1):
+        if self.lib.rxIceSM_Update(self.h, steps):
+            check(self.lib)
+
+    def sync(self):
+        if self.lib.rxIceSM_Sync(self.h):
+            check(self.lib)
+
+    def clusters(self):
+        nc = C.c_int()
+        nm = C.c_long()
+        hist = np.zeros(128, np.int32)
+        self.lib.rxIceSM_ClusterInfo(self.h, C.byref(nc), C.byref(nm), hist.ctypes.data, 128)
+        start = np.zeros(nc.value + 1, np.int32)
+        mem = np.zeros(max(nm.value, 1), np.int32)
+        self.lib.rxIceSM_GetClusters(self.h, start.ctypes.data, mem.ctypes.data)
+        return start, mem[: nm.value], hist
+
+    def autotune(self):
+        t = np.zeros(self.NBINS, np.int32)
+        if self.lib.rxIceSM_Autotune(self.h, t.ctypes.data):
+            check(self.lib)
+        return t
+
+    def set_tiles(self, tiles):
+        t = np.ascontiguousarray(tiles, np.int32)
+        if self.lib.rxIceSM_SetTiles(self.h, t.ctypes.data):
+            check(self.lib)
+
+    def kernel_times(self):
+        a, b = C.c_float(), C.c_float()
+        if self.lib.rxIceSM_KernelTimes(self.h, C.byref(a), C.byref(b)):
+            check(self.lib)
+        return a.value, b.value
+
+    def close(self):
+        if self.h:
+            self.lib.rxIceSM_Destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

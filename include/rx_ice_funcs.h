@@ -1,0 +1,80 @@
+/*
+ * rx_ice_funcs.h -- ice shape matching on the GPU.
+ *
+ * 1. The batched shape-matching object (no reference counterpart as a single object): every cluster of an
+ *    ice body in one persistent launch per shape-matching iteration, clusters binned by size with tile
+ *    shapes chosen by a built-in autotuner, the substep replayed from a hipGraph.  Semantics per step:
+ *      prediction and wall response of OrientedPrtObj::UpdateCluster_SM (ElasticObject_OP.cpp:556-609),
+ *      rxShapeMatching::shapeMatching (ShapeMatching.cpp:492-587; alpha, beta, volume conservation, unit
+ *      masses, polar decomposition by Jacobi on A^T A, ShapeMatching.h:242-387),
+ *      per-particle averaging of ConvoNormal::StepConvolution (Ice_Convolution_Normal.cpp:32-67),
+ *      repeated `iterations` times as Ice_CalcMethod_Iteration::StepObjMove (Ice_CalcMethod_Iteration.cpp:32-48),
+ *      velocity of OrientedParticle::UpdatePosAndVel (OrientedParticle.cpp:377-393).
+ *    Clusters: one per particle, {p} plus every particle within `cluster_radius` of p in the rest state
+ *    (IceObject::SetClusterMoveInfoFromNeight, IceObject.cpp:608-633).
+ *
+ * 2. The C++-linkage launch functions of Ice_SM.h:23-36 and IceObject.h:388-395, same names and arguments.
+ *
+ * Entry points returning int give 0 on success; CuGetLastErrorString (rx_cu_funcs.h) holds the message.
+ */
+#ifndef RX_ICE_FUNCS_H_
+#define RX_ICE_FUNCS_H_
+
+#include "rx_cu_common.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rxIceEnv {
+    float alpha;            /* stiffness, goal blend per iteration                                   */
+    float beta;             /* linear-deformation share of the goal transform                        */
+    float dt;
+    float gravity[3];
+    float box_min[3];       /* simulation space of the clusters (rxShapeMatching::SetSimulationSpace) */
+    float box_max[3];
+    int iterations;         /* shape-matching iterations per step                                    */
+    int volume_conservation;
+    float restitution;      /* wall response factor (0.9 in the reference)                           */
+    float cluster_radius;   /* neighbour radius that forms the clusters                              */
+} rxIceEnv;
+
+void rxIceEnvDefault(rxIceEnv* env);
+
+/* Creates the object for n particles at rest positions rest_pos4 (float4) and builds the clusters. */
+void* rxIceSM_Create(const rxIceEnv* env, const float* rest_pos4, int n);
+void rxIceSM_Destroy(void* sm);
+/* Current state (original order, float4). */
+int rxIceSM_SetState(void* sm, const float* pos4, const float* vel4);
+int rxIceSM_GetState(void* sm, float* pos4, float* vel4);
+/* `steps` steps, asynchronous on the object's stream. */
+int rxIceSM_Update(void* sm, int steps);
+int rxIceSM_Sync(void* sm);
+/* Number of clusters, members, and a size histogram (hist[k] = clusters with k members, k < nhist). */
+int rxIceSM_ClusterInfo(void* sm, int* nclusters, long* nmembers, int* hist, int nhist);
+/* Cluster CSR (start[nclusters+1], members) copied to the host. */
+int rxIceSM_GetClusters(void* sm, int* start, int* members);
+/* Times the candidate lanes-per-cluster tiles of every size bin and keeps the fastest; out_tiles receives
+ * the tile chosen for each of the 5 bins (<=8, <=16, <=32, <=64, >64 members). */
+int rxIceSM_Autotune(void* sm, int* out_tiles);
+int rxIceSM_SetTiles(void* sm, const int* tiles);
+int rxIceSM_UseGraph(void* sm, int on);
+/* Average milliseconds of the cluster kernel and of the averaging kernel over the last Update (HIP events). */
+int rxIceSM_KernelTimes(void* sm, float* cluster_ms, float* average_ms);
+
+#ifdef __cplusplus
+}
+
+/* Ice_SM.h:23-36 (LaunchShapeMatchingGPU): one rigid shape-matching pass per cluster (alpha = 1, beta = 0)
+ * as OrientedPrtObj::UpdateCluster_SM (ElasticObject_OP.cpp:551-720).  Layout of the reference: prtPos/prtVel
+ * float4 per particle; member slots i of cluster c are indxSet[2c] .. indxSet[2c+1] (inclusive);
+ * pIndxes[i] = particle of slot i; orgPos/curPos/vel 3 floats per slot; orgCm/curCm 3 floats per cluster.
+ * Gravity and walls come from the last CuSetParameters (Gravity, BoundaryMin/Max). */
+void LaunchShapeMatchingGPU(int prtNum, float* prtPos, float* prtVel, float* orgPos, float* curPos, float* orgCm, float* curCm,
+                            float* vel, int* pIndxes, int* d_IndxSet, float dt);
+/* IceObject.h:388-395 (LaunchInterPolationGPU): copies the solid positions/velocities into the fluid arrays
+ * (blend coefficient 1, IceObject.cpp:876-897). sldPrtPos/Vel hold 3 floats per particle, sphPrtPos/Vel 4. */
+void LaunchInterPolationGPU(int prtNum, float* sldPrtPos, float* sldPrtVel, float* sphPrtPos, float* sphPrtVel);
+#endif
+
+#endif /* RX_ICE_FUNCS_H_ */

@@ -114,6 +114,17 @@ int rxPBDSPH_GPU_SetOption(void* solver, int option, float value);
 /* Milliseconds and launch counts per rxProfileClass since the last call (n entries each). */
 int rxPBDSPH_GPU_GetProfile(void* solver, float* ms, int* count, int n);
 
+/* Fused WCSPH solver (rxSPH_GPU, rx_sph.h:432-773): Update (rx_sph_gpu.cpp:756-923) with the CPU path's
+ * force and wall rules (rx_sph_cpu.cpp:323-380).  After the first step each step replays a captured
+ * hipGraph (rxSPH_GPU_UseGraph(solver, 0) launches the kernels directly instead). */
+void* rxSPH_GPU_Create(const rxSphEnv* env);
+void rxSPH_GPU_Destroy(void* solver);
+int rxSPH_GPU_SetParticles(void* solver, const float* pos4, const float* vel4, int n);
+int rxSPH_GPU_Update(void* solver, float dt, int steps);
+int rxSPH_GPU_GetArray(void* solver, int type, float* host, int count);
+int rxSPH_GPU_Sync(void* solver);
+int rxSPH_GPU_UseGraph(void* solver, int on);
+
 #ifdef __cplusplus
 }
 #endif

@@ -21,14 +21,23 @@ def _declared():
 
 
 def test_every_declared_symbol_is_exported(lib):
-    from fltk_sph_ice_amd._abi import SIGNATURES
+    from fltk_sph_ice_amd._abi import CXX_SYMBOLS, LIB_PATH, SIGNATURES
 
     declared = _declared()
     assert len(declared) > 50
     missing = [n for n in sorted(declared) if not hasattr(lib, n)]
     assert not missing, missing
-    unbound = sorted(declared - set(SIGNATURES))
-    assert not unbound, f"declared but not bound in _abi.SIGNATURES: {unbound}"
+    unbound = sorted(declared - set(SIGNATURES) - set(CXX_SYMBOLS))
+    assert not unbound, f"declared but not bound in _abi: {unbound}"
+    # the C++-linkage launch functions keep the reference's C++ signatures (mangled names)
+    import subprocess
+
+    nm = subprocess.run(["nm", "-D", "--defined-only", str(LIB_PATH)], capture_output=True, text=True).stdout
+    for name, (mangled, _) in CXX_SYMBOLS.items():
+        assert mangled in nm, name
+    demangled = subprocess.run(["c++filt"], input="\n".join(m for m, _ in CXX_SYMBOLS.values()), capture_output=True,
+                               text=True).stdout
+    assert "LaunchShapeMatchingGPU(int, float*, float*, float*, float*, float*, float*, float*, int*, int*, float)" in demangled
 
 
 def test_struct_layout_matches_bindings(lib):

@@ -1,0 +1,136 @@
+"""Batched shape matching (rxIceSM) and the legacy launch function vs the oracle."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from fltk_sph_ice_amd import load, scene
+from fltk_sph_ice_amd.device import DevBuf
+from fltk_sph_ice_amd.solvers import IceSM
+
+pytestmark = pytest.mark.gpu
+
+
+def _make(sc, graph=True):
+    pos, vel = sc.particles()
+    sm = IceSM(pos, alpha=sc.alpha, beta=sc.beta, dt=sc.dt, gravity=sc.gravity, box_min=sc.box_min, box_max=sc.box_max,
+               iterations=sc.iterations, cluster_radius=sc.cluster_radius, graph=graph)
+    sm.set_state(pos, vel)
+    return sm, pos, vel
+
+
+def _oracle_steps(sc, pos, vel, cs, mem, steps):
+    S = oracle.OrcSmParams()
+    S.alpha, S.beta, S.dt = sc.alpha, sc.beta, sc.dt
+    for k in range(3):
+        S.gravity[k], S.bmin[k], S.bmax[k] = sc.gravity[k], sc.box_min[k], sc.box_max[k]
+    S.iterations, S.volume_conservation, S.restitution = sc.iterations, 1, 0.9
+    p, v = pos.copy(), vel.copy()
+    rest = pos.copy()
+    for _ in range(steps):
+        oracle.load().orc_sm_step_f(oracle.ptr(p), oracle.ptr(v), len(p), oracle.ptr(cs), oracle.ptr(mem), len(cs) - 1,
+                                    oracle.ptr(rest), C.byref(S))
+    return p, v
+
+
+def test_clusters_are_radius_neighbourhoods():
+    sc = scene.ice_block(10, 9, 7)
+    sm, pos, _ = _make(sc)
+    cs, mem, hist = sm.clusters()
+    d = np.linalg.norm(pos[:, None, :3].astype(np.float64) - pos[None, :, :3].astype(np.float64), axis=-1)
+    for i in range(sc.n):
+        ref = np.nonzero(d[i] <= sc.cluster_radius)[0]
+        assert np.array_equal(mem[cs[i]:cs[i + 1]], ref)
+    sizes = np.diff(cs)
+    print("cluster sizes: min", sizes.min(), "max", sizes.max())
+    assert hist.sum() == sc.n
+
+
+@pytest.mark.parametrize("steps", [1, 5])
+def test_batched_step_parity(steps):
+    sc = scene.ice_block(12, 10, 8)
+    sm, pos, vel = _make(sc)
+    cs, mem, _ = sm.clusters()
+    rp, rv = _oracle_steps(sc, pos, vel, cs, mem, steps)
+    sm.step(steps)
+    gp, gv = sm.state()
+    assert np.abs(gp[:, :3] - rp[:, :3]).max() <= 2e-5 * np.abs(rp[:, :3]).max()
+    assert np.abs(gv[:, :3] - rv[:, :3]).max() <= 2e-3 * max(np.abs(rv[:, :3]).max(), 1e-3)
+
+
+def test_autotuned_tiles_and_graph_give_same_result():
+    sc = scene.ice_block(14, 12, 10)
+    a, pos, vel = _make(sc, graph=True)
+    b, _, _ = _make(sc, graph=False)
+    tiles = a.autotune()
+    assert set(tiles.tolist()) <= {8, 16, 32, 64}
+    b.set_tiles([64, 64, 64, 64, 64])
+    a.set_state(pos, vel)
+    for _ in range(3):
+        a.step(2)
+        b.step(2)
+        a.sync()
+        b.sync()
+    pa, _ = a.state()
+    pb, _ = b.state()
+    assert np.abs(pa - pb).max() <= 1e-5 * np.abs(pb).max()
+
+
+def test_rigid_rotation_preserved():
+    sc = scene.ice_block(8, 8, 8)
+    sc.gravity = (0.0, 0.0, 0.0)
+    sc.alpha, sc.beta = 1.0, 0.3
+    sm, pos, _ = _make(sc)
+    th = 0.3
+    Q = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]], np.float64)
+    c = pos[:, :3].astype(np.float64).mean(0)
+    rot = pos.copy()
+    rot[:, :3] = ((pos[:, :3] - c) @ Q.T + c).astype(np.float32)
+    sm.set_state(rot, np.zeros_like(rot))
+    sm.step(1)
+    gp, gv = sm.state()
+    assert np.abs(gp[:, :3] - rot[:, :3]).max() < 2e-5
+
+
+def test_legacy_launch_shape_matching_disjoint_clusters(lib):
+    """LaunchShapeMatchingGPU (Ice_SM.h:23-36 layout) on disjoint clusters equals the oracle step with
+    alpha = 1, beta = 0 and one iteration (averaging is the identity for disjoint clusters)."""
+    sc = scene.wcsph_dam_break(4, 4, 4)
+    sp = sc.params()
+    sp.BoundaryMin[0] = sp.BoundaryMin[1] = sp.BoundaryMin[2] = -5.0
+    sp.BoundaryMax[0] = sp.BoundaryMax[1] = sp.BoundaryMax[2] = 5.0
+    sp.Gravity[1] = -9.8
+    lib.CuSetParameters(C.byref(sp))
+    rng = np.random.default_rng(7)
+    nc, k = 40, 9
+    n = nc * k
+    rest = np.zeros((n, 4), np.float32)
+    for c in range(nc):
+        rest[c * k:(c + 1) * k, :3] = rng.normal(0, 0.1, (k, 3)) + c * 1.0 - 2.0
+    pos = rest.copy()
+    pos[:, :3] += rng.normal(0, 0.01, (n, 3))
+    vel = np.zeros_like(pos)
+    vel[:, :3] = rng.normal(0, 0.2, (n, 3))
+    dt = np.float32(0.01)
+    cs = np.arange(0, n + 1, k, dtype=np.int32)
+    mem = np.arange(n, dtype=np.int32)
+    indx = np.stack([cs[:-1], cs[1:] - 1], 1).astype(np.int32).ravel()
+    org = rest[:, :3].astype(np.float32).ravel().copy()
+    ocm = np.stack([rest[c * k:(c + 1) * k, :3].astype(np.float64).mean(0) for c in range(nc)]).astype(np.float32).ravel()
+    bufs = {name: DevBuf.from_array(a) for name, a in
+            dict(pos=pos, vel=vel, org=org, cur=np.zeros(3 * n, np.float32), ocm=ocm, ccm=np.zeros(3 * nc, np.float32),
+                 v=np.zeros(3 * n, np.float32), pidx=mem, indx=indx).items()}
+    lib.LaunchShapeMatchingGPU(nc, bufs["pos"].ptr, bufs["vel"].ptr, bufs["org"].ptr, bufs["cur"].ptr, bufs["ocm"].ptr,
+                               bufs["ccm"].ptr, bufs["v"].ptr, bufs["pidx"].ptr, bufs["indx"].ptr, dt)
+    cur = bufs["cur"].download(np.float32).reshape(n, 3)
+    S = oracle.OrcSmParams()
+    S.alpha, S.beta, S.dt = 1.0, 0.0, float(dt)
+    for a in range(3):
+        S.gravity[a], S.bmin[a], S.bmax[a] = (0, -9.8, 0)[a], -5.0, 5.0
+    S.iterations, S.volume_conservation, S.restitution = 1, 1, 0.9
+    p, v = pos.copy(), vel.copy()
+    oracle.load().orc_sm_step_f(oracle.ptr(p), oracle.ptr(v), n, oracle.ptr(cs), oracle.ptr(mem), nc, oracle.ptr(rest), C.byref(S))
+    np.testing.assert_allclose(cur, p[:, :3], atol=2e-6)
+    vv = bufs["v"].download(np.float32).reshape(n, 3)
+    np.testing.assert_allclose(vv, v[:, :3], atol=5e-4)

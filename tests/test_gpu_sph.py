@@ -101,7 +101,11 @@ def test_one_step_parity_and_drift_report(lib):
     np.testing.assert_allclose(gpos[:, :3], rpos[:, :3], rtol=TOL, atol=TOL * 1.0)
     vscale = np.abs(rvel[:, :3]).max()
     assert np.abs(gvel[:, :3] - rvel[:, :3]).max() <= 1e-4 * vscale
-    # drift of 50 FP32 GPU steps against the FP64 oracle of the same steps (reported, not gated)
+    # drift of 50 FP32 GPU steps against the FP64 oracle of the same steps (reported, not gated).  The
+    # literal viscosity 3.5 is beyond the explicit stability limit of the reference's viscous term (no 1/rho_i:
+    # mu dt / h^2 = 8.75), so multi-step runs use mu = 3.5 / rho0.
+    sc.viscosity = 3.5 / 998.29
+    P = oracle.params_from(sc.params(), sc)
     p64, v64 = pos.astype(np.float64), vel.astype(np.float64)
     for _ in range(50):
         p64, v64, _ = oracle.sph_step(p64, v64, P, sc.dt, double=True)
