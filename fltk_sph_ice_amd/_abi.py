@@ -166,6 +166,8 @@ SIGNATURES: dict[str, tuple] = {
     "rxPBDSPH_GPU_Sync": (i32, [vp]),
     "rxPBDSPH_GPU_SetOption": (i32, [vp, i32, flt]),
     "rxPBDSPH_GPU_GetProfile": (i32, [vp, vp, vp, i32]),
+    "rxPBDSPH_GPU_SetNetwork": (i32, [vp, vp]),
+    "rxPBDSPH_GPU_FeatureScales": (i32, [vp, fp, fp]),
     "rxSPH_GPU_Create": (vp, [C.POINTER(rxSphEnv)]),
     "rxSPH_GPU_Destroy": (None, [vp]),
     "rxSPH_GPU_SetParticles": (i32, [vp, vp, vp, i32]),
@@ -187,6 +189,16 @@ SIGNATURES: dict[str, tuple] = {
     "rxIceSM_SetTiles": (i32, [vp, vp]),
     "rxIceSM_UseGraph": (i32, [vp, i32]),
     "rxIceSM_KernelTimes": (i32, [vp, fp, fp]),
+    # rx_mlp.h
+    "rxMLP_NumParams": (i32, []),
+    "rxMLP_Create": (vp, [u32]),
+    "rxMLP_Destroy": (None, [vp]),
+    "rxMLP_SetParams": (i32, [vp, vp]),
+    "rxMLP_GetParams": (i32, [vp, vp]),
+    "rxMLP_SetNormalization": (i32, [vp, vp, vp, flt, flt]),
+    "rxMLP_Forward": (i32, [vp, vp, i32, vp]),
+    "rxMLP_Gradients": (i32, [vp, vp, vp, i32, vp, fp]),
+    "rxMLP_Train": (i32, [vp, vp, vp, i32, i32, i32, flt, u32, vp]),
 }
 
 # C++-linkage entry points (Ice_SM.h / IceObject.h signatures), resolved by their mangled names.
@@ -196,8 +208,10 @@ CXX_SYMBOLS = {
     "LaunchInterPolationGPU": ("_Z22LaunchInterPolationGPUiPfS_S_S_", (None, [i32, vp, vp, vp, vp])),
 }
 
-RX_ARRAY_POSITION, RX_ARRAY_VELOCITY, RX_ARRAY_DENSITY, RX_ARRAY_LAMBDA_SUM, RX_ARRAY_PRESSURE = range(5)
-RX_PBF_OPT_SYNC_EACH_STEP, RX_PBF_OPT_RECORD_LAMBDA, RX_PBF_OPT_TOL, RX_PBF_OPT_ITERATIONS, RX_PBF_OPT_SKIN, RX_PBF_OPT_PROFILE = range(6)
+RX_ARRAY_POSITION, RX_ARRAY_VELOCITY, RX_ARRAY_DENSITY, RX_ARRAY_LAMBDA_SUM, RX_ARRAY_PRESSURE, RX_ARRAY_FEATURES = range(6)
+RX_PBF_MODE_PLAIN, RX_PBF_MODE_LEARNED = 0, 2
+(RX_PBF_OPT_SYNC_EACH_STEP, RX_PBF_OPT_RECORD_LAMBDA, RX_PBF_OPT_TOL, RX_PBF_OPT_ITERATIONS, RX_PBF_OPT_SKIN, RX_PBF_OPT_PROFILE,
+ RX_PBF_OPT_RECORD_FEATURES, RX_PBF_OPT_MODE, RX_PBF_OPT_ADAPTIVE, RX_PBF_OPT_WARM_SCALE) = range(10)
 PROF_CLASSES = ["predict", "grid", "list_lambda", "lambda", "dp", "velocity", "xsph", "features", "mlp"]
 
 _lib = None

@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "fused_grid.h"
+#include "mlp.h"
 #include "../../include/rx_sph_host.h"
 
 struct PbfK;
@@ -20,6 +21,11 @@ public:
     int GetStats(float* out, int count);
     int SetOption(int opt, float value);
     int Sync();
+    int SetNetwork(void* mlp_object);  // weights of an rxMLP object (copied)
+    void FeatureScales(float* sd_ref, float* nref) const {
+        if (sd_ref) *sd_ref = m_sd_ref;
+        if (nref) *nref = m_nref;
+    }
     // Kernel timing with HIP events on the solver's stream (enabled by RX_PBF_OPT_PROFILE):
     // total milliseconds and launch count per kernel class since the last reset.
     int GetProfile(float* ms, int* count, int n);
@@ -30,7 +36,7 @@ private:
     bool alloc(uint n);
     PbfK consts(float dt) const;
     void reorder(float4*& a, hipStream_t s);
-    void k_lambda_acc_init(hipStream_t s);
+    void lattice_refs();
 
     rxSphEnv m_env{};
     rxSimParams m_params{};
@@ -42,7 +48,13 @@ private:
     float m_tol = 0.01f;
     bool m_have_list = false;
     bool m_sync_each_step = false;
-    bool m_record_lambda = false;
+    bool m_record_features = false;
+    int m_mode = RX_PBF_MODE_PLAIN;
+    bool m_adaptive = false;
+    float m_warm_scale = 1.0f;
+    float m_sd_ref = 1.0f, m_nref = 1.0f;
+    MlpDev m_mlp{};
+    bool m_mlp_ready = false;
     int m_step = 0;
     // device state, all in cell order
     float4* m_x = nullptr;   // positions at the step start
@@ -54,6 +66,9 @@ private:
     float* m_dens = nullptr;
     float* m_lamacc = nullptr;  // sum of lambda over the iterations of the last step
     float* m_ftmp = nullptr;
+    float* m_dstart = nullptr;   // density at the step start
+    float* m_feat = nullptr;     // 16 features per particle (learned warm start)
+    float4* m_ftmp16 = nullptr;
     uint* m_id = nullptr;       // slot -> original particle index
     uint* m_idt = nullptr;
     uint* m_list = nullptr;     // [slot][particle] skin neighbour list

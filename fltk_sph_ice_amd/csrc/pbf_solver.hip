@@ -18,6 +18,7 @@
 
 #include "fused_grid.h"
 #include "pbf_solver.h"
+#include "mlp.h"
 #include "../../include/rx_cu_funcs.h"
 #include "../../include/rx_sph_host.h"
 
@@ -32,7 +33,13 @@ struct PbfK {
     float eps;
     float dt, inv_dt;
     float xsph_c;
-    float tol;        // learned-mode stop criterion (max error)
+    float tol;        // adaptive solve: stop once the max density error is at most tol
+    // feature normalisation of the learned warm start
+    float sd_ref;     // lambda denominator of an interior particle of the rest lattice
+    float inv_nref;   // 1 / neighbour count of the rest lattice
+    float inv_gref;   // 1 / sqrt(sd_ref)
+    float inv_h;
+    float inv_spacing;
 };
 
 namespace {
@@ -54,6 +61,10 @@ enum Stat {
 };
 
 
+
+// 1/sqrt(r2) with the hardware reciprocal square root (v_rsq_f32, within 1 ulp): one instruction in
+// place of a correctly rounded square root and division per neighbour pair.
+__device__ inline float rinv_of(float r2) { return __builtin_amdgcn_rsqf(r2); }
 
 __device__ inline float pow4(float x) {
     float x2 = x * x;
@@ -119,7 +130,14 @@ __device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint
     }
     uint c = cnt[i];
     if (c <= (uint)k.maxn && disp2 <= k.half_skin2) {
-        for (uint s = 0; s < c; ++s) f(list[(size_t)s * k.n + i]);
+        const uint* l = list + i;
+        uint s = 0;
+        for (; s + 1 < c; s += 2) {  // two independent gathers in flight
+            uint j0 = l[(size_t)s * k.n], j1 = l[(size_t)(s + 1) * k.n];
+            f(j0);
+            f(j1);
+        }
+        if (s < c) f(l[(size_t)s * k.n]);
     } else {
         fg_for_stencil(fg_unhash(bhash[i], k.g), k.g, off, f, 2);
     }
@@ -148,7 +166,7 @@ __global__ __launch_bounds__(kTpb) void k_predict(const float4* __restrict__ x, 
             if (r2 <= k.s.h2) {
                 float q = k.s.h2 - r2;
                 d += q * q * q;
-                if (r2 > 0.0f) fv += (k.s.h - sqrtf(r2)) * (f3(v[j]) - vi);
+                if (r2 > 0.0f) fv += (k.s.h - r2 * rinv_of(r2)) * (f3(v[j]) - vi);
             }
         });
         d *= k.s.mass * k.s.wpoly6;
@@ -167,11 +185,13 @@ __global__ __launch_bounds__(kTpb) void k_predict(const float4* __restrict__ x, 
 
 // ------------------------------------------------------------------------------------------ list + lambda
 // Builds the skin list of slot i from the new grid (cell = h + skin) and computes lambda at x*
-// (rx_sph_pbd.cpp:699-822: C = rho/rho0 - 1, sd with the self term).  Writes xl = (x*, lambda).
+// (rx_sph_pbd.cpp:699-822: C = rho/rho0 - 1, sd with the self term).  Writes xl = (x*, lambda).  When feat is
+// set it also writes the 16 features of the learned warm start (12 used, 4 zero padding) for slot i.
 __global__ __launch_bounds__(kTpb) void k_list_lambda(const float4* __restrict__ xs, float4* __restrict__ xl, float4* __restrict__ xbuild,
                                                       float* __restrict__ dens, uint* __restrict__ list, uint* __restrict__ cnt,
                                                       float* __restrict__ stats, PbfK k, const uint* __restrict__ bhash,
-                                                      const uint* __restrict__ off) {
+                                                      const uint* __restrict__ off, float* __restrict__ feat,
+                                                      const float4* __restrict__ v, const float* __restrict__ dstart) {
     bool valid;
     uint i = slot_of(k, valid);
     float err = 0.0f;
@@ -181,6 +201,10 @@ __global__ __launch_bounds__(kTpb) void k_list_lambda(const float4* __restrict__
         float d = 0.0f, sd = 0.0f;
         float3 gs = f3(0, 0, 0);
         uint c = 0;
+        // learned-start extras
+        float nn = 0.0f, wsum = 0.0f, wcs = 0.0f, ring = 0.0f, nring = 0.0f, div = 0.0f;
+        const float h2q = 0.25f * k.s.h2;
+        float3 vi = feat ? f3(v[i]) : f3(0, 0, 0);
         fg_for_stencil(fg_unhash(bhash[i], k.g), k.g, off, [&](uint j) {
             float3 r = p - f3(xs[j]);
             float r2 = dot3(r, r);
@@ -189,13 +213,26 @@ __global__ __launch_bounds__(kTpb) void k_list_lambda(const float4* __restrict__
                 ++c;
                 if (r2 <= k.s.h2) {
                     float q2 = k.s.h2 - r2;
-                    d += q2 * q2 * q2;
+                    float w = q2 * q2 * q2;
+                    d += w;
+                    float3 g = f3(0, 0, 0);
                     if (r2 > 0.0f) {
-                        float rr = sqrtf(r2);
-                        float q = k.s.h - rr;
-                        float3 g = (k.s.gwspiky * q * q / rr * k.s.inv_rho0) * r;
+                        float ri = rinv_of(r2);
+                        float q = k.s.h - r2 * ri;
+                        g = (k.s.gwspiky * k.s.inv_rho0 * q * q * ri) * r;
                         gs += g;
                         sd += dot3(g, g);
+                    }
+                    if (feat) {
+                        float cj = dstart[j] * k.s.inv_rho0 - 1.0f;
+                        nn += 1.0f;
+                        wsum += w;
+                        wcs += w * cj;
+                        if (r2 > h2q) {
+                            ring += cj;
+                            nring += 1.0f;
+                        }
+                        div += dot3(f3(v[j]) - vi, g);
                     }
                 }
             }
@@ -203,24 +240,48 @@ __global__ __launch_bounds__(kTpb) void k_list_lambda(const float4* __restrict__
         cnt[i] = c;
         if (c > (uint)k.maxn) atomicAdd((unsigned int*)&stats[ST_OVERFLOW], 1u);
         d *= k.s.mass * k.s.wpoly6;
-        sd += dot3(gs, gs);
+        float sgs = dot3(gs, gs);
+        sd += sgs;
         float C = d * k.s.inv_rho0 - 1.0f;
         float lam = -C / (sd + k.eps);
         xl[i] = make_float4(p.x, p.y, p.z, lam);
         xbuild[i] = p4;
         dens[i] = d;
         err = fmaxf(C, 0.0f);
+        if (feat) {
+            float wall = fminf(fminf(fminf(p.x - k.s.wall_lo.x, k.s.wall_hi.x - p.x), fminf(p.y - k.s.wall_lo.y, k.s.wall_hi.y - p.y)),
+                               fminf(p.z - k.s.wall_lo.z, k.s.wall_hi.z - p.z));
+            float f[16];
+            f[0] = C;
+            f[1] = lam * k.sd_ref;
+            f[2] = sd / k.sd_ref - 1.0f;
+            f[3] = nn * k.inv_nref - 1.0f;
+            f[4] = sqrtf(sgs) * k.inv_gref;
+            f[5] = gs.y * k.inv_gref;
+            f[6] = wsum > 0.0f ? wcs / wsum : 0.0f;       // W-weighted mean of the step-start C
+            f[7] = nring > 0.0f ? ring / nring : 0.0f;     // mean step-start C of the outer ring (h/2 < r <= h)
+            f[8] = dstart[i] * k.s.inv_rho0 - 1.0f;
+            f[9] = div * k.s.mass * k.dt;
+            f[10] = fminf(fmaxf(wall * k.inv_h, 0.0f), 1.0f) - 0.5f;
+            f[11] = vi.y * k.dt * k.inv_spacing;
+            f[12] = f[13] = f[14] = f[15] = 0.0f;
+            float4* fo = (float4*)(feat + (size_t)i * 16);
+            fo[0] = make_float4(f[0], f[1], f[2], f[3]);
+            fo[1] = make_float4(f[4], f[5], f[6], f[7]);
+            fo[2] = make_float4(f[8], f[9], f[10], f[11]);
+            fo[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
     }
     block_max_atomic(err, &stats[ST_ERR_ITER + 0]);
 }
 
 // lambda pass (rx_sph_pbd.cpp:699-822) over the list: reads positions xa, writes xl = (x, lambda).
-// When lam_acc is set the lambda is also accumulated (sum over iterations, the solver's converged multiplier).
+// Adaptive solve: once a check met the tolerance the remaining passes exit at once.
 __global__ __launch_bounds__(kTpb) void k_lambda(const float4* __restrict__ xa, float4* __restrict__ xl, float* __restrict__ dens,
-                                                 float* __restrict__ lam_acc, float* __restrict__ stats, int it, PbfK k,
-                                                 const uint* __restrict__ list, const uint* __restrict__ cnt,
-                                                 const uint* __restrict__ bhash, const uint* __restrict__ off, int learned) {
-    if (learned && stats[ST_ERR_ITER + it - 1] <= k.tol && it > 1) return;  // solve already converged
+                                                 float* __restrict__ stats, int it, PbfK k, const uint* __restrict__ list,
+                                                 const uint* __restrict__ cnt, const uint* __restrict__ bhash,
+                                                 const uint* __restrict__ off, int adaptive) {
+    if (adaptive && it > 1 && stats[ST_ERR_ITER + it - 1] <= k.tol) return;  // solve already converged
     bool valid;
     uint i = slot_of(k, valid);
     float disp2 = stats[ST_DISP2];
@@ -236,9 +297,9 @@ __global__ __launch_bounds__(kTpb) void k_lambda(const float4* __restrict__ xa, 
                 float q2 = k.s.h2 - r2;
                 d += q2 * q2 * q2;
                 if (r2 > 0.0f) {
-                    float rr = sqrtf(r2);
-                    float q = k.s.h - rr;
-                    float3 g = (k.s.gwspiky * q * q / rr * k.s.inv_rho0) * r;
+                    float ri = rinv_of(r2);
+                    float q = k.s.h - r2 * ri;
+                    float3 g = (k.s.gwspiky * k.s.inv_rho0 * q * q * ri) * r;
                     gs += g;
                     sd += dot3(g, g);
                 }
@@ -250,7 +311,6 @@ __global__ __launch_bounds__(kTpb) void k_lambda(const float4* __restrict__ xa, 
         float lam = -C / (sd + k.eps);
         xl[i] = make_float4(p.x, p.y, p.z, lam);
         dens[i] = d;
-        if (lam_acc) lam_acc[i] += lam;
         err = fmaxf(C, 0.0f);
     }
     block_max_atomic(err, &stats[ST_ERR_ITER + it]);
@@ -260,11 +320,11 @@ __global__ __launch_bounds__(kTpb) void k_lambda(const float4* __restrict__ xa, 
 // writes xa = clamp(x + dp).  Tracks the displacement from the list build.  In learned mode the pass is
 // skipped once the previous check met the tolerance.
 __global__ __launch_bounds__(kTpb) void k_dp(const float4* __restrict__ xl, float4* __restrict__ xa, const float4* __restrict__ xbuild,
-                                             float* __restrict__ stats, int it, PbfK k, const uint* __restrict__ list,
-                                             const uint* __restrict__ cnt, const uint* __restrict__ bhash,
-                                             const uint* __restrict__ off, int learned) {
-    if (learned && it > 0 && stats[ST_ERR_ITER + it] <= k.tol) return;
-    if (learned && threadIdx.x == 0 && blockIdx.x == 0) atomicAdd((unsigned int*)&stats[ST_ITERS], 1u);
+                                             float* __restrict__ lamacc, float* __restrict__ stats, int it, PbfK k,
+                                             const uint* __restrict__ list, const uint* __restrict__ cnt,
+                                             const uint* __restrict__ bhash, const uint* __restrict__ off, int adaptive) {
+    if (adaptive && it > 0 && stats[ST_ERR_ITER + it] <= k.tol) return;
+    if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd((unsigned int*)&stats[ST_ITERS], 1u);
     bool valid;
     uint i = slot_of(k, valid);
     float disp2 = stats[ST_DISP2];
@@ -280,19 +340,20 @@ __global__ __launch_bounds__(kTpb) void k_dp(const float4* __restrict__ xl, floa
             float3 r = p - f3(q4);
             float r2 = dot3(r, r);
             if (r2 <= k.s.h2 && r2 > 0.0f) {
-                float rr = sqrtf(r2);
+                float ri = rinv_of(r2);
                 float sc = 0.0f;
                 if (k.s.ap) {
                     float q2 = k.s.h2 - r2;
                     float ww = k.s.wpoly6 * q2 * q2 * q2 * k.s.inv_ap_wq;
                     sc = -k.s.ap_k * (k.s.ap_n == 4.0f ? pow4(ww) : powf(ww, k.s.ap_n)) * dt2;
                 }
-                float q = k.s.h - rr;
-                acc += ((li + q4.w + sc) * k.s.gwspiky * q * q / rr * k.s.inv_rho0) * r;
+                float q = k.s.h - r2 * ri;
+                acc += ((li + q4.w + sc) * (k.s.gwspiky * k.s.inv_rho0) * q * q * ri) * r;
             }
         });
         float3 xn = clamp_wall(p + acc, k.s);
         xa[i] = make_float4(xn.x, xn.y, xn.z, 0.0f);
+        lamacc[i] += li;  // the multiplier this step has applied so far
         float3 dd = xn - f3(xbuild[i]);
         mv = dot3(dd, dd);
     }
@@ -341,9 +402,11 @@ __global__ void k_scatter_out1(const float* __restrict__ src, const uint* __rest
     uint i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[id[i]] = src[i];
 }
-__global__ void k_acc_init(const float4* __restrict__ xl, float* __restrict__ acc, uint n) {
+__global__ void k_scatter_feat(const float4* __restrict__ src, const uint* __restrict__ id, float4* __restrict__ dst, uint n) {
     uint i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) acc[i] = xl[i].w;
+    if (i >= n) return;
+    uint o = id[i];
+    for (int t = 0; t < 4; ++t) dst[4 * (size_t)o + t] = src[4 * (size_t)i + t];
 }
 __global__ void k_iota(uint* __restrict__ a, uint n) {
     uint i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -360,7 +423,7 @@ void rxPBDSPH_GPU::Finalize() {
     if (m_stream) RX_HIP(hipStreamSynchronize(m_stream));
     for (void** p : {(void**)&m_x, (void**)&m_xs, (void**)&m_xl, (void**)&m_xb, (void**)&m_v, (void**)&m_t, (void**)&m_dens,
                      (void**)&m_lamacc, (void**)&m_id, (void**)&m_idt, (void**)&m_list, (void**)&m_cnt, (void**)&m_stats,
-                     (void**)&m_ftmp})
+                     (void**)&m_ftmp, (void**)&m_dstart, (void**)&m_feat, (void**)&m_ftmp16})
         if (*p) {
             RX_HIP(hipFree(*p));
             *p = nullptr;
@@ -385,6 +448,7 @@ int rxPBDSPH_GPU::Initialize(const rxSphEnv& env) {
     m_skin = 0.1f * env.h;
     m_maxn = 96;
     m_grid.setup(m_params, env.h + m_skin);
+    lattice_refs();
     if (!RX_HIP(hipStreamCreateWithFlags(&m_stream, hipStreamNonBlocking))) return 1;
     return 0;
 }
@@ -395,7 +459,9 @@ bool rxPBDSPH_GPU::alloc(uint n) {
               RX_HIP(hipMalloc(&m_xb, v4)) && RX_HIP(hipMalloc(&m_v, v4)) && RX_HIP(hipMalloc(&m_t, v4)) &&
               RX_HIP(hipMalloc(&m_dens, f)) && RX_HIP(hipMalloc(&m_lamacc, f)) && RX_HIP(hipMalloc(&m_id, u)) &&
               RX_HIP(hipMalloc(&m_idt, u)) && RX_HIP(hipMalloc(&m_list, u * (size_t)m_maxn)) && RX_HIP(hipMalloc(&m_cnt, u)) &&
-              RX_HIP(hipMalloc(&m_stats, sizeof(float) * ST_N)) && RX_HIP(hipMalloc(&m_ftmp, f));
+              RX_HIP(hipMalloc(&m_stats, sizeof(float) * ST_N)) && RX_HIP(hipMalloc(&m_ftmp, f)) &&
+              RX_HIP(hipMalloc(&m_dstart, f)) && RX_HIP(hipMalloc(&m_feat, v4 * 4)) &&
+              RX_HIP(hipMalloc(&m_ftmp16, v4 * 4));
     m_grid.n = n;
     m_grid.reserve(n);
     return ok;
@@ -409,7 +475,7 @@ int rxPBDSPH_GPU::SetParticles(const float* pos4, const float* vel4, int n) {
     if ((uint)n != m_n) {
         for (void** p : {(void**)&m_x, (void**)&m_xs, (void**)&m_xl, (void**)&m_xb, (void**)&m_v, (void**)&m_t, (void**)&m_dens,
                          (void**)&m_lamacc, (void**)&m_id, (void**)&m_idt, (void**)&m_list, (void**)&m_cnt, (void**)&m_stats,
-                         (void**)&m_ftmp})
+                         (void**)&m_ftmp, (void**)&m_dstart, (void**)&m_feat, (void**)&m_ftmp16})
             if (*p) {
                 RX_HIP(hipFree(*p));
                 *p = nullptr;
@@ -427,6 +493,29 @@ int rxPBDSPH_GPU::SetParticles(const float* pos4, const float* vel4, int n) {
     m_have_list = false;
     m_step = 0;
     return RX_HIP(hipStreamSynchronize(s)) ? 0 : 1;
+}
+
+// Lambda denominator and neighbour count of an interior particle of the rest lattice (spacing env.spacing):
+// the scales of the learned warm start's features.
+void rxPBDSPH_GPU::lattice_refs() {
+    double h = m_env.h, l = m_env.spacing > 0.0f ? m_env.spacing : 0.5 * h;
+    double gw = -45.0 / (3.14159265358979323846 * pow(h, 6.0)), r0 = m_params.Density;
+    int nmax = (int)ceil(h / l) + 1;
+    double sd = 0.0;
+    int cnt = 0;
+    for (int x = -nmax; x <= nmax; ++x)
+        for (int y = -nmax; y <= nmax; ++y)
+            for (int z = -nmax; z <= nmax; ++z) {
+                double r = sqrt((double)(x * x + y * y + z * z)) * l;
+                if (r > h) continue;
+                ++cnt;
+                if (r > 0.0) {
+                    double g = gw * (h - r) * (h - r) / r0;  // |grad W| / rho0
+                    sd += g * g;
+                }
+            }
+    m_sd_ref = (float)sd;
+    m_nref = (float)cnt;
 }
 
 PbfK rxPBDSPH_GPU::consts(float dt) const {
@@ -447,6 +536,11 @@ PbfK rxPBDSPH_GPU::consts(float dt) const {
     k.inv_dt = 1.0f / dt;
     k.xsph_c = m_env.xsph_c;
     k.tol = m_tol;
+    k.sd_ref = m_sd_ref;
+    k.inv_nref = 1.0f / m_nref;
+    k.inv_gref = 1.0f / sqrtf(m_sd_ref);
+    k.inv_h = 1.0f / m_env.h;
+    k.inv_spacing = 1.0f / (m_env.spacing > 0.0f ? m_env.spacing : 0.5f * m_env.h);
     return k;
 }
 
@@ -461,11 +555,16 @@ int rxPBDSPH_GPU::Update(float dt, int step) {
         rx_set_error(1, "rxPBDSPH_GPU::Update: no particles");
         return 1;
     }
+    if (m_mode == RX_PBF_MODE_LEARNED && !m_mlp_ready) {
+        rx_set_error(1, "rxPBDSPH_GPU::Update: learned mode needs network weights (rxPBDSPH_GPU_SetNetwork)");
+        return 1;
+    }
     hipStream_t s = m_stream;
     PbfK k = consts(dt);
     uint n = m_n;
     unsigned nb = rx_blocks(n, kTpb);
-    const int K = m_env.max_iter;
+    const int K = std::min(m_env.max_iter, 14);
+    const int adaptive = m_adaptive ? 1 : 0;
     // per-step statistics (ST_DISP2 carries over: it bounds the validity of the previous list)
     k_fill<float><<<1, 64, 0, s>>>(m_stats, 0.0f, ST_DISP2);
     k_fill<float><<<1, 64, 0, s>>>(m_stats + ST_OVERFLOW, 0.0f, ST_N - ST_OVERFLOW);
@@ -487,7 +586,7 @@ int rxPBDSPH_GPU::Update(float dt, int step) {
     k_predict<<<nb, kTpb, 0, s>>>(m_x, m_v, m_xs, m_dens, m_stats, mode, k, m_list, m_cnt, m_grid.hash, m_grid.off);
     end_mark(mk);
 
-    // ---- rebuild on x*
+    // ---- rebuild on x*; the step-start density follows the particles
     k_fill<float><<<1, 64, 0, s>>>(m_stats + ST_DISP2, 0.0f, 1);
     mk = begin_mark(RX_PROF_GRID);
     m_grid.build(m_xs, s);
@@ -496,27 +595,32 @@ int rxPBDSPH_GPU::Update(float dt, int step) {
     reorder(m_v, s);
     fg_gather<uint>(m_idt, m_id, m_grid.perm, n, s);
     std::swap(m_id, m_idt);
-    fg_gather<float>(m_ftmp, m_lamacc, m_grid.perm, n, s);
-    std::swap(m_lamacc, m_ftmp);
+    fg_gather<float>(m_dstart, m_dens, m_grid.perm, n, s);
+    rx_fill<float>(m_lamacc, 0.0f, n, s);
     end_mark(mk);
 
-    // ---- list + lambda_0, then K Jacobi iterations
-    float* lam_acc = m_record_lambda ? m_lamacc : nullptr;
-    if (lam_acc) rx_fill<float>(lam_acc, 0.0f, n, s);
-    mk = begin_mark(RX_PROF_LIST_LAMBDA);
-    k_list_lambda<<<nb, kTpb, 0, s>>>(m_xs, m_xl, m_xb, m_dens, m_list, m_cnt, m_stats, k, m_grid.hash, m_grid.off);
+    // ---- list + lambda_0 (the network's multiplier in learned mode), then up to K corrections
+    const bool feats = (m_mode == RX_PBF_MODE_LEARNED) || m_record_features;
+    mk = begin_mark(feats ? RX_PROF_FEATURES : RX_PROF_LIST_LAMBDA);
+    k_list_lambda<<<nb, kTpb, 0, s>>>(m_xs, m_xl, m_xb, m_dens, m_list, m_cnt, m_stats, k, m_grid.hash, m_grid.off,
+                                      feats ? m_feat : nullptr, m_v, m_dstart);
     end_mark(mk);
-    if (lam_acc) k_lambda_acc_init(s);
+    if (m_mode == RX_PBF_MODE_LEARNED) {
+        mk = begin_mark(RX_PROF_MLP);
+        rx_mlp_infer_to_w(m_feat, m_xl, n, m_mlp, m_warm_scale / k.sd_ref, s);
+        end_mark(mk);
+    }
     for (int it = 0; it < K; ++it) {
         if (it > 0) {
             mk = begin_mark(RX_PROF_LAMBDA);
-            k_lambda<<<nb, kTpb, 0, s>>>(m_xs, m_xl, m_dens, lam_acc, m_stats, it, k, m_list, m_cnt, m_grid.hash, m_grid.off, 0);
+            k_lambda<<<nb, kTpb, 0, s>>>(m_xs, m_xl, m_dens, m_stats, it, k, m_list, m_cnt, m_grid.hash, m_grid.off, adaptive);
             end_mark(mk);
         }
         mk = begin_mark(RX_PROF_DP);
-        k_dp<<<nb, kTpb, 0, s>>>(m_xl, m_xs, m_xb, m_stats, it, k, m_list, m_cnt, m_grid.hash, m_grid.off, 0);
+        k_dp<<<nb, kTpb, 0, s>>>(m_xl, m_xs, m_xb, m_lamacc, m_stats, it, k, m_list, m_cnt, m_grid.hash, m_grid.off, adaptive);
         end_mark(mk);
     }
+    // (the density of the final positions is judged by the next step's prediction pass, slot ST_ERR_START)
     // ---- velocity, XSPH
     mk = begin_mark(RX_PROF_VELOCITY);
     k_velocity<<<rx_blocks(n, 256), 256, 0, s>>>(m_xs, m_x, m_dens, m_t, k);
@@ -534,11 +638,6 @@ int rxPBDSPH_GPU::Update(float dt, int step) {
     if (!RX_HIP(hipGetLastError())) return 1;
     if (m_sync_each_step && !RX_HIP(hipStreamSynchronize(s))) return 1;
     return 0;
-}
-
-void rxPBDSPH_GPU::k_lambda_acc_init(hipStream_t s) {
-    // lambda_0 lives in xl.w; copy it into the accumulator
-    k_acc_init<<<rx_blocks(m_n, 256), 256, 0, s>>>(m_xl, m_lamacc, m_n);
 }
 
 int rxPBDSPH_GPU::GetArray(int type, float* host, int count) {
@@ -566,6 +665,14 @@ int rxPBDSPH_GPU::GetArray(int type, float* host, int count) {
         k_scatter_out1<<<rx_blocks(n, 256), 256, 0, s>>>(m_lamacc, m_id, (float*)tmp, n);
         RX_HIP(hipMemcpyAsync(host, tmp, sizeof(float) * n, hipMemcpyDeviceToHost, s));
         break;
+    case RX_ARRAY_FEATURES:
+        if (count < (int)(16 * n)) {
+            rx_set_error(1, "rxPBDSPH_GPU::GetArray: features need 16 floats per particle");
+            return 1;
+        }
+        k_scatter_feat<<<rx_blocks(n, 256), 256, 0, s>>>((const float4*)m_feat, m_id, m_ftmp16, n);
+        RX_HIP(hipMemcpyAsync(host, m_ftmp16, sizeof(float4) * 4 * n, hipMemcpyDeviceToHost, s));
+        break;
     default:
         rx_set_error(1, "rxPBDSPH_GPU::GetArray: unknown array %d", type);
         return 1;
@@ -588,6 +695,16 @@ int rxPBDSPH_GPU::GetStats(float* out, int count) {
 }
 
 int rxPBDSPH_GPU::Sync() { return RX_HIP(hipStreamSynchronize(m_stream)) ? 0 : 1; }
+
+int rxPBDSPH_GPU::SetNetwork(void* mlp) {
+    if (!rx_mlp_view(mlp, &m_mlp)) {
+        rx_set_error(1, "rxPBDSPH_GPU::SetNetwork: null network");
+        m_mlp_ready = false;
+        return 1;
+    }
+    m_mlp_ready = true;
+    return 0;
+}
 
 int rxPBDSPH_GPU::begin_mark(int cls) {
     if (!m_profile) return -1;
@@ -664,13 +781,29 @@ int rxPBDSPH_GPU_GetStats(void* h, float* out, int count) { return h ? ((rxPBDSP
 int rxPBDSPH_GPU_Sync(void* h) { return h ? ((rxPBDSPH_GPU*)h)->Sync() : 1; }
 int rxPBDSPH_GPU_SetOption(void* h, int opt, float value) { return h ? ((rxPBDSPH_GPU*)h)->SetOption(opt, value) : 1; }
 int rxPBDSPH_GPU_GetProfile(void* h, float* ms, int* count, int n) { return h ? ((rxPBDSPH_GPU*)h)->GetProfile(ms, count, n) : 1; }
+int rxPBDSPH_GPU_SetNetwork(void* h, void* mlp) { return h ? ((rxPBDSPH_GPU*)h)->SetNetwork(mlp) : 1; }
+int rxPBDSPH_GPU_FeatureScales(void* h, float* sd_ref, float* nref) {
+    if (!h) return 1;
+    ((rxPBDSPH_GPU*)h)->FeatureScales(sd_ref, nref);
+    return 0;
+}
 
 }  // extern "C"
 
 int rxPBDSPH_GPU::SetOption(int opt, float value) {
     switch (opt) {
     case RX_PBF_OPT_SYNC_EACH_STEP: m_sync_each_step = value != 0.0f; return 0;
-    case RX_PBF_OPT_RECORD_LAMBDA: m_record_lambda = value != 0.0f; return 0;
+    case RX_PBF_OPT_RECORD_LAMBDA: return 0;  // the applied multiplier is always accumulated
+    case RX_PBF_OPT_RECORD_FEATURES: m_record_features = value != 0.0f; return 0;
+    case RX_PBF_OPT_MODE:
+        if (value != RX_PBF_MODE_PLAIN && value != RX_PBF_MODE_LEARNED) {
+            rx_set_error(1, "rxPBDSPH_GPU::SetOption: unknown mode %g", value);
+            return 1;
+        }
+        m_mode = (int)value;
+        return 0;
+    case RX_PBF_OPT_ADAPTIVE: m_adaptive = value != 0.0f; return 0;
+    case RX_PBF_OPT_WARM_SCALE: m_warm_scale = value; return 0;
     case RX_PBF_OPT_TOL: m_tol = value; return 0;
     case RX_PBF_OPT_ITERATIONS: m_env.max_iter = m_env.min_iter = (int)value; return 0;
     case RX_PBF_OPT_PROFILE:

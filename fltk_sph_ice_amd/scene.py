@@ -91,6 +91,8 @@ class Scene:
         got = lib.rxSceneBox(pos.ctypes.data, vel.ctypes.data, n, org, *self.block, self.spacing, self.jitter,
                              self.vel_sigma, self.seed)
         assert got == n
+        if "v0" in self.extra:
+            vel[:, :3] += np.asarray(self.extra["v0"], np.float32)
         return pos, vel
 
 
@@ -174,3 +176,14 @@ def ice_block(nx: int = 64, ny: int = 64, nz: int = 32, seed_pos: int = 3, seed_
     box_max = (2.0 * ext[0] + 1.0, ext[1] + 1.5, 2.0 * ext[2] + 1.0)
     origin = (0.5 * (box_max[0] - ext[0]), 0.5, 0.5 * (box_max[2] - ext[2]))
     return IceScene("ice_block", (nx, ny, nz), s, origin, (0.0, 0.0, 0.0), box_max, 0.01 * s, 0.1, seed_pos, seed_vel)
+
+
+def pbf_splash(nx: int = 48, ny: int = 32, nz: int = 48, seed: int = 6, box: tuple = (6.4, 3.2, 3.2)) -> Scene:
+    """PBF splash: a block released from mid-height with a downward velocity of 2 (plus N(0, 0.05) noise) into
+    the same 2:1:1 box, plain-PBF parameters."""
+    sc = pbf_dam_break(nx, ny, nz, seed=seed, box=box)
+    sc.name = "pbf_splash"
+    s = sc.spacing
+    sc.block_origin = (0.5 * (box[0] - nx * s), box[1] - ny * s - 0.1, 0.5 * (box[2] - nz * s))
+    sc.extra["v0"] = (0.0, -2.0, 0.0)
+    return sc

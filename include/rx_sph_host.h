@@ -64,8 +64,15 @@ enum rxArrayType {
     RX_ARRAY_POSITION = 0,   /* float4 per particle */
     RX_ARRAY_VELOCITY = 1,   /* float4 per particle */
     RX_ARRAY_DENSITY = 2,    /* float per particle  */
-    RX_ARRAY_LAMBDA_SUM = 3, /* float per particle: sum of the Jacobi lambdas of the last step (PBF) */
-    RX_ARRAY_PRESSURE = 4    /* float per particle (WCSPH) */
+    RX_ARRAY_LAMBDA_SUM = 3, /* float per particle: sum of the multipliers the last step applied (PBF)  */
+    RX_ARRAY_PRESSURE = 4,   /* float per particle (WCSPH) */
+    RX_ARRAY_FEATURES = 5    /* 16 floats per particle: warm-start features of the last step (PBF) */
+};
+
+/* PBF solve modes (rxPBDSPH_GPU_SetOption(RX_PBF_OPT_MODE)). */
+enum rxPbfMode {
+    RX_PBF_MODE_PLAIN = 0,   /* reference Jacobi: lambda from scratch every iteration                    */
+    RX_PBF_MODE_LEARNED = 2  /* first correction uses the network's prediction from the step's features */
 };
 
 /* Options of rxPBDSPH_GPU_SetOption. */
@@ -75,7 +82,12 @@ enum rxPbfOption {
     RX_PBF_OPT_TOL = 2,            /* max density error that ends the learned solve (default 0.01)    */
     RX_PBF_OPT_ITERATIONS = 3,     /* Jacobi iterations per step (plain solve)                        */
     RX_PBF_OPT_SKIN = 4,           /* neighbour-list skin as a fraction of h (default 0.1)            */
-    RX_PBF_OPT_PROFILE = 5         /* 1: time every kernel class with HIP events (rxPBDSPH_GPU_GetProfile) */
+    RX_PBF_OPT_PROFILE = 5,        /* 1: time every kernel class with HIP events (rxPBDSPH_GPU_GetProfile) */
+    RX_PBF_OPT_RECORD_FEATURES = 6,/* 1: compute the warm-start features every step (RX_ARRAY_FEATURES)   */
+    RX_PBF_OPT_MODE = 7,           /* rxPbfMode                                                           */
+    RX_PBF_OPT_ADAPTIVE = 8,       /* 1: skip the remaining corrections once the max density error at the
+                                      start of an iteration is <= TOL                                    */
+    RX_PBF_OPT_WARM_SCALE = 9      /* stability factor on the network's multiplier (learned mode)          */
 };
 
 /* Kernel classes of rxPBDSPH_GPU_GetProfile. */
@@ -113,6 +125,10 @@ int rxPBDSPH_GPU_Sync(void* solver);
 int rxPBDSPH_GPU_SetOption(void* solver, int option, float value);
 /* Milliseconds and launch counts per rxProfileClass since the last call (n entries each). */
 int rxPBDSPH_GPU_GetProfile(void* solver, float* ms, int* count, int n);
+/* Network of the learned warm start (an rxMLP object, see rx_mlp.h); its weights are copied. */
+int rxPBDSPH_GPU_SetNetwork(void* solver, void* mlp);
+/* Feature scales of the scene: lambda denominator and neighbour count of the rest lattice. */
+int rxPBDSPH_GPU_FeatureScales(void* solver, float* sd_ref, float* nref);
 
 /* Fused WCSPH solver (rxSPH_GPU, rx_sph.h:432-773): Update (rx_sph_gpu.cpp:756-923) with the CPU path's
  * force and wall rules (rx_sph_cpu.cpp:323-380).  After the first step each step replays a captured
