@@ -206,12 +206,15 @@ CXX_SYMBOLS = {
     "LaunchShapeMatchingGPU": ("_Z22LaunchShapeMatchingGPUiPfS_S_S_S_S_S_PiS0_f",
                                (None, [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, flt])),
     "LaunchInterPolationGPU": ("_Z22LaunchInterPolationGPUiPfS_S_S_", (None, [i32, vp, vp, vp, vp])),
+    "LaunchHeatTransferGPU": ("_Z21LaunchHeatTransferGPUPfS_S_PKiS1_PKfPiS4_S1_fffffi",
+                              (None, [vp, vp, vp, vp, vp, vp, vp, vp, vp, flt, flt, flt, flt, flt, i32])),
 }
 
 RX_ARRAY_POSITION, RX_ARRAY_VELOCITY, RX_ARRAY_DENSITY, RX_ARRAY_LAMBDA_SUM, RX_ARRAY_PRESSURE, RX_ARRAY_FEATURES = range(6)
-RX_PBF_MODE_PLAIN, RX_PBF_MODE_LEARNED = 0, 2
+RX_PBF_MODE_PLAIN, RX_PBF_MODE_SCALED, RX_PBF_MODE_LEARNED = 0, 1, 2
 (RX_PBF_OPT_SYNC_EACH_STEP, RX_PBF_OPT_RECORD_LAMBDA, RX_PBF_OPT_TOL, RX_PBF_OPT_ITERATIONS, RX_PBF_OPT_SKIN, RX_PBF_OPT_PROFILE,
- RX_PBF_OPT_RECORD_FEATURES, RX_PBF_OPT_MODE, RX_PBF_OPT_ADAPTIVE, RX_PBF_OPT_WARM_SCALE) = range(10)
+ RX_PBF_OPT_RECORD_FEATURES, RX_PBF_OPT_MODE, RX_PBF_OPT_ADAPTIVE, RX_PBF_OPT_WARM_SCALE,
+ RX_PBF_OPT_JUDGE, RX_PBF_OPT_RESET_RUN) = range(12)
 PROF_CLASSES = ["predict", "grid", "list_lambda", "lambda", "dp", "velocity", "xsph", "features", "mlp"]
 
 _lib = None

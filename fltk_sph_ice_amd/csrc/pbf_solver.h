@@ -52,6 +52,7 @@ private:
     int m_mode = RX_PBF_MODE_PLAIN;
     bool m_adaptive = false;
     float m_warm_scale = 1.0f;
+    int m_judge = 0;
     float m_sd_ref = 1.0f, m_nref = 1.0f;
     MlpDev m_mlp{};
     bool m_mlp_ready = false;

@@ -28,12 +28,15 @@ struct PbfK {
     uint n;
     float rc2;        // (h + skin)^2, list cutoff
     float half_skin2; // (skin / 2)^2
+    float skin;       // list cutoff minus h
     float wide2;      // largest displacement^2 the 5x5x5 stencil still covers
     int maxn;
     float eps;
     float dt, inv_dt;
     float xsph_c;
-    float tol;        // adaptive solve: stop once the max density error is at most tol
+    float tol;        // adaptive solve: stop once the judged density error is at most tol
+    int judge;        // 0: max (rho - rho0)+ / rho0; 1: its mean over the particles
+    float lam0_scale; // factor on the first multiplier (scaled mode; 1 otherwise)
     // feature normalisation of the learned warm start
     float sd_ref;     // lambda denominator of an interior particle of the rest lattice
     float inv_nref;   // 1 / neighbour count of the rest lattice
@@ -57,8 +60,21 @@ enum Stat {
     ST_ERR_FINAL = 21,       // max error after the solve (learned mode: the last check)
     ST_BAD = 22,             // displacement beyond the wide stencil's reach
     ST_MEAN_ABS_START = 23,  // sum |rho - rho0| / rho0 at the step start (divide by N)
-    ST_N = 32
+    ST_MPOS_START = 24,      // sum (rho - rho0)+ / rho0 at the step start (divide by N)
+    ST_MPOS_ITER = 32,       // .. 32 + k: the same at the start of Jacobi iteration k (k < 16)
+    ST_RUN = 48,             // run accumulators from here on, kept across steps:
+    ST_RUN_MAX = 48,         //   max over the judged steps of the max error
+    ST_RUN_MPOS = 49,        //   max over the judged steps of the mean positive error
+    ST_RUN_ITERS = 50,       //   corrections executed (uint)
+    ST_RUN_STEPS = 51,       //   judged steps (uint)
+    ST_RUN_FAIL = 52,        //   judged steps whose judged error exceeded tol (uint)
+    ST_N = 64
 };
+
+// Error of iteration `it` under the solver's judge is within the tolerance.
+__device__ inline bool converged(const float* stats, int it, const PbfK& k) {
+    return k.judge == 0 ? stats[ST_ERR_ITER + it] <= k.tol : stats[ST_MPOS_ITER + it] <= k.tol * (float)k.n;
+}
 
 
 
@@ -119,17 +135,26 @@ __device__ inline float3 clamp_wall(float3 x, const SphK& s) {
 }
 
 // Neighbour candidates of slot i for a pass that needs every j with |x_i - x_j| <= h at the current positions.
-// mode 0: the skin list; mode 1: the 3x3x3 stencil of a grid built on the current positions (cell >= h);
-// overflowed lists and displacements beyond skin/2 use the 5x5x5 stencil of the build-time grid.
+// mode 0: the skin list; mode 1: the 3x3x3 stencil of a grid built on the current positions (cell >= h).
+// A pair within h now was within h + d_i + d_j <= h + d_i + D at the list build (d = displacement since the
+// build, D = its maximum over all particles), so the list of i holds every such j while d_i + D <= skin;
+// otherwise (or when the list overflowed) the pass walks the 5x5x5 stencil of the build-time grid, which
+// covers build distances below 2 cell widths.  `p` is the current position of i, `xb` the build positions.
 template <class F>
 __device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint* __restrict__ list, const uint* __restrict__ cnt,
-                                     const uint* __restrict__ bhash, const uint* __restrict__ off, float disp2, F&& f) {
+                                     const uint* __restrict__ bhash, const uint* __restrict__ off, float disp2, float3 p,
+                                     const float4* __restrict__ xb, F&& f) {
     if (mode == 1) {
         fg_for_stencil(fg_unhash(bhash[i], k.g), k.g, off, f, 1);
         return;
     }
     uint c = cnt[i];
-    if (c <= (uint)k.maxn && disp2 <= k.half_skin2) {
+    bool ok = c <= (uint)k.maxn;
+    if (ok && disp2 > k.half_skin2) {
+        float3 dd = p - f3(xb[i]);
+        ok = sqrtf(dot3(dd, dd)) + sqrtf(disp2) <= k.skin;
+    }
+    if (ok) {
         const uint* l = list + i;
         uint s = 0;
         for (; s + 1 < c; s += 2) {  // two independent gathers in flight
@@ -149,7 +174,8 @@ __device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint
 __global__ __launch_bounds__(kTpb) void k_predict(const float4* __restrict__ x, const float4* __restrict__ v, float4* __restrict__ xs,
                                                   float* __restrict__ dens, float* __restrict__ stats, int mode, PbfK k,
                                                   const uint* __restrict__ list, const uint* __restrict__ cnt,
-                                                  const uint* __restrict__ bhash, const uint* __restrict__ off) {
+                                                  const uint* __restrict__ bhash, const uint* __restrict__ off,
+                                                  const float4* __restrict__ xb) {
     bool valid;
     uint i = slot_of(k, valid);
     float disp2 = stats[ST_DISP2];
@@ -159,7 +185,7 @@ __global__ __launch_bounds__(kTpb) void k_predict(const float4* __restrict__ x, 
         float3 p = f3(x[i]), vi = f3(v[i]);
         float d = 0.0f;
         float3 fv = f3(0, 0, 0);
-        for_neighbors(i, mode, k, list, cnt, bhash, off, disp2, [&](uint j) {
+        for_neighbors(i, mode, k, list, cnt, bhash, off, disp2, p, xb, [&](uint j) {
             float4 pj = x[j];
             float3 r = p - f3(pj);
             float r2 = dot3(r, r);
@@ -181,6 +207,19 @@ __global__ __launch_bounds__(kTpb) void k_predict(const float4* __restrict__ x, 
     }
     block_max_atomic(err, &stats[ST_ERR_START]);
     block_sum_atomic(aerr, &stats[ST_MEAN_ABS_START]);
+    block_sum_atomic(err, &stats[ST_MPOS_START]);
+}
+
+// Folds the judge of the previous step (its final positions, measured by this step's prediction pass) into
+// the run accumulators.
+__global__ void k_fold_run(float* __restrict__ stats, PbfK k) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float mx = stats[ST_ERR_START], mp = stats[ST_MPOS_START] / (float)k.n;
+    stats[ST_RUN_MAX] = fmaxf(stats[ST_RUN_MAX], mx);
+    stats[ST_RUN_MPOS] = fmaxf(stats[ST_RUN_MPOS], mp);
+    uint* u = (uint*)stats;
+    u[ST_RUN_STEPS] += 1u;
+    if ((k.judge == 0 ? mx : mp) > k.tol) u[ST_RUN_FAIL] += 1u;
 }
 
 // ------------------------------------------------------------------------------------------ list + lambda
@@ -244,7 +283,7 @@ __global__ __launch_bounds__(kTpb) void k_list_lambda(const float4* __restrict__
         sd += sgs;
         float C = d * k.s.inv_rho0 - 1.0f;
         float lam = -C / (sd + k.eps);
-        xl[i] = make_float4(p.x, p.y, p.z, lam);
+        xl[i] = make_float4(p.x, p.y, p.z, lam * k.lam0_scale);
         xbuild[i] = p4;
         dens[i] = d;
         err = fmaxf(C, 0.0f);
@@ -273,6 +312,7 @@ __global__ __launch_bounds__(kTpb) void k_list_lambda(const float4* __restrict__
         }
     }
     block_max_atomic(err, &stats[ST_ERR_ITER + 0]);
+    block_sum_atomic(err, &stats[ST_MPOS_ITER + 0]);
 }
 
 // lambda pass (rx_sph_pbd.cpp:699-822) over the list: reads positions xa, writes xl = (x, lambda).
@@ -280,8 +320,8 @@ __global__ __launch_bounds__(kTpb) void k_list_lambda(const float4* __restrict__
 __global__ __launch_bounds__(kTpb) void k_lambda(const float4* __restrict__ xa, float4* __restrict__ xl, float* __restrict__ dens,
                                                  float* __restrict__ stats, int it, PbfK k, const uint* __restrict__ list,
                                                  const uint* __restrict__ cnt, const uint* __restrict__ bhash,
-                                                 const uint* __restrict__ off, int adaptive) {
-    if (adaptive && it > 1 && stats[ST_ERR_ITER + it - 1] <= k.tol) return;  // solve already converged
+                                                 const uint* __restrict__ off, int adaptive, const float4* __restrict__ xb) {
+    if (adaptive && it > 1 && converged(stats, it - 1, k)) return;  // solve already converged
     bool valid;
     uint i = slot_of(k, valid);
     float disp2 = stats[ST_DISP2];
@@ -290,7 +330,7 @@ __global__ __launch_bounds__(kTpb) void k_lambda(const float4* __restrict__ xa, 
         float3 p = f3(xa[i]);
         float d = 0.0f, sd = 0.0f;
         float3 gs = f3(0, 0, 0);
-        for_neighbors(i, 0, k, list, cnt, bhash, off, disp2, [&](uint j) {
+        for_neighbors(i, 0, k, list, cnt, bhash, off, disp2, p, xb, [&](uint j) {
             float3 r = p - f3(xa[j]);
             float r2 = dot3(r, r);
             if (r2 <= k.s.h2) {
@@ -314,6 +354,7 @@ __global__ __launch_bounds__(kTpb) void k_lambda(const float4* __restrict__ xa, 
         err = fmaxf(C, 0.0f);
     }
     block_max_atomic(err, &stats[ST_ERR_ITER + it]);
+    block_sum_atomic(err, &stats[ST_MPOS_ITER + it]);
 }
 
 // Position correction (rx_sph_pbd.cpp:925-1026) + apply + wall (rx_sph_pbd.cpp:436-445): reads xl = (x, lambda),
@@ -323,8 +364,11 @@ __global__ __launch_bounds__(kTpb) void k_dp(const float4* __restrict__ xl, floa
                                              float* __restrict__ lamacc, float* __restrict__ stats, int it, PbfK k,
                                              const uint* __restrict__ list, const uint* __restrict__ cnt,
                                              const uint* __restrict__ bhash, const uint* __restrict__ off, int adaptive) {
-    if (adaptive && it > 0 && stats[ST_ERR_ITER + it] <= k.tol) return;
-    if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd((unsigned int*)&stats[ST_ITERS], 1u);
+    if (adaptive && it > 0 && converged(stats, it, k)) return;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        atomicAdd((unsigned int*)&stats[ST_ITERS], 1u);
+        atomicAdd((unsigned int*)&stats[ST_RUN_ITERS], 1u);
+    }
     bool valid;
     uint i = slot_of(k, valid);
     float disp2 = stats[ST_DISP2];
@@ -335,7 +379,7 @@ __global__ __launch_bounds__(kTpb) void k_dp(const float4* __restrict__ xl, floa
         float li = p4.w;
         float dt2 = k.dt * k.dt;
         float3 acc = f3(0, 0, 0);
-        for_neighbors(i, 0, k, list, cnt, bhash, off, disp2, [&](uint j) {
+        for_neighbors(i, 0, k, list, cnt, bhash, off, disp2, p, xbuild, [&](uint j) {
             float4 q4 = xl[j];
             float3 r = p - f3(q4);
             float r2 = dot3(r, r);
@@ -373,7 +417,7 @@ __global__ void k_velocity(const float4* __restrict__ xa, const float4* __restri
 __global__ __launch_bounds__(kTpb) void k_xsph(const float4* __restrict__ xa, const float4* __restrict__ v, float4* __restrict__ vout,
                                                float* __restrict__ stats, PbfK k, const uint* __restrict__ list,
                                                const uint* __restrict__ cnt, const uint* __restrict__ bhash,
-                                               const uint* __restrict__ off) {
+                                               const uint* __restrict__ off, const float4* __restrict__ xb) {
     bool valid;
     uint i = slot_of(k, valid);
     if (!valid) return;
@@ -381,7 +425,7 @@ __global__ __launch_bounds__(kTpb) void k_xsph(const float4* __restrict__ xa, co
     float3 p = f3(xa[i]);
     float3 vi = f3(v[i]);
     float3 acc = f3(0, 0, 0);
-    for_neighbors(i, 0, k, list, cnt, bhash, off, disp2, [&](uint j) {
+    for_neighbors(i, 0, k, list, cnt, bhash, off, disp2, p, xb, [&](uint j) {
         float3 r = p - f3(xa[j]);
         float r2 = dot3(r, r);
         if (r2 <= k.s.h2) {
@@ -445,7 +489,7 @@ int rxPBDSPH_GPU::Initialize(const rxSphEnv& env) {
     Finalize();
     if (rxSetupSimParams(&env, &m_params) != 0) return 1;
     m_env = env;
-    m_skin = 0.1f * env.h;
+    m_skin = 0.15f * env.h;
     m_maxn = 96;
     m_grid.setup(m_params, env.h + m_skin);
     lattice_refs();
@@ -528,6 +572,7 @@ PbfK rxPBDSPH_GPU::consts(float dt) const {
     float rc = m_env.h + m_skin;
     k.rc2 = rc * rc;
     k.half_skin2 = 0.25f * m_skin * m_skin;
+    k.skin = m_skin;
     float wide = 0.5f * (2.0f * m_grid.k.cell - m_env.h);  // 5x5x5 stencil reach minus h, shared by both particles
     k.wide2 = wide * wide;
     k.maxn = m_maxn;
@@ -536,6 +581,8 @@ PbfK rxPBDSPH_GPU::consts(float dt) const {
     k.inv_dt = 1.0f / dt;
     k.xsph_c = m_env.xsph_c;
     k.tol = m_tol;
+    k.judge = m_judge;
+    k.lam0_scale = m_mode == RX_PBF_MODE_SCALED ? m_warm_scale : 1.0f;
     k.sd_ref = m_sd_ref;
     k.inv_nref = 1.0f / m_nref;
     k.inv_gref = 1.0f / sqrtf(m_sd_ref);
@@ -567,7 +614,7 @@ int rxPBDSPH_GPU::Update(float dt, int step) {
     const int adaptive = m_adaptive ? 1 : 0;
     // per-step statistics (ST_DISP2 carries over: it bounds the validity of the previous list)
     k_fill<float><<<1, 64, 0, s>>>(m_stats, 0.0f, ST_DISP2);
-    k_fill<float><<<1, 64, 0, s>>>(m_stats + ST_OVERFLOW, 0.0f, ST_N - ST_OVERFLOW);
+    k_fill<float><<<1, 64, 0, s>>>(m_stats + ST_OVERFLOW, 0.0f, ST_RUN - ST_OVERFLOW);
 
     // ---- predict at x_n
     int mode = 0;
@@ -583,8 +630,9 @@ int rxPBDSPH_GPU::Update(float dt, int step) {
         mode = 1;
     }
     int mk = begin_mark(RX_PROF_PREDICT);
-    k_predict<<<nb, kTpb, 0, s>>>(m_x, m_v, m_xs, m_dens, m_stats, mode, k, m_list, m_cnt, m_grid.hash, m_grid.off);
+    k_predict<<<nb, kTpb, 0, s>>>(m_x, m_v, m_xs, m_dens, m_stats, mode, k, m_list, m_cnt, m_grid.hash, m_grid.off, m_xb);
     end_mark(mk);
+    if (m_step > 0) k_fold_run<<<1, 64, 0, s>>>(m_stats, k);
 
     // ---- rebuild on x*; the step-start density follows the particles
     k_fill<float><<<1, 64, 0, s>>>(m_stats + ST_DISP2, 0.0f, 1);
@@ -613,7 +661,7 @@ int rxPBDSPH_GPU::Update(float dt, int step) {
     for (int it = 0; it < K; ++it) {
         if (it > 0) {
             mk = begin_mark(RX_PROF_LAMBDA);
-            k_lambda<<<nb, kTpb, 0, s>>>(m_xs, m_xl, m_dens, m_stats, it, k, m_list, m_cnt, m_grid.hash, m_grid.off, adaptive);
+            k_lambda<<<nb, kTpb, 0, s>>>(m_xs, m_xl, m_dens, m_stats, it, k, m_list, m_cnt, m_grid.hash, m_grid.off, adaptive, m_xb);
             end_mark(mk);
         }
         mk = begin_mark(RX_PROF_DP);
@@ -627,7 +675,7 @@ int rxPBDSPH_GPU::Update(float dt, int step) {
     end_mark(mk);
     if (m_env.xsph_c != 0.0f) {
         mk = begin_mark(RX_PROF_XSPH);
-        k_xsph<<<nb, kTpb, 0, s>>>(m_xs, m_t, m_v, m_stats, k, m_list, m_cnt, m_grid.hash, m_grid.off);
+        k_xsph<<<nb, kTpb, 0, s>>>(m_xs, m_t, m_v, m_stats, k, m_list, m_cnt, m_grid.hash, m_grid.off, m_xb);
         end_mark(mk);
     } else {
         std::swap(m_v, m_t);
@@ -689,6 +737,9 @@ int rxPBDSPH_GPU::GetStats(float* out, int count) {
     h[ST_OVERFLOW] = (float)__builtin_bit_cast(uint, h[ST_OVERFLOW]);
     h[ST_ITERS] = (float)__builtin_bit_cast(uint, h[ST_ITERS]);
     h[ST_MEAN_ABS_START] = h[ST_MEAN_ABS_START] / (float)m_n;
+    h[ST_MPOS_START] = h[ST_MPOS_START] / (float)m_n;
+    for (int t = 0; t < 16; ++t) h[ST_MPOS_ITER + t] = h[ST_MPOS_ITER + t] / (float)m_n;
+    for (int t : {ST_RUN_ITERS, ST_RUN_STEPS, ST_RUN_FAIL}) h[t] = (float)__builtin_bit_cast(uint, h[t]);
     h[ST_DISP2] = sqrtf(h[ST_DISP2]);
     for (int i = 0; i < count && i < ST_N; ++i) out[i] = h[i];
     return 0;
@@ -796,7 +847,7 @@ int rxPBDSPH_GPU::SetOption(int opt, float value) {
     case RX_PBF_OPT_RECORD_LAMBDA: return 0;  // the applied multiplier is always accumulated
     case RX_PBF_OPT_RECORD_FEATURES: m_record_features = value != 0.0f; return 0;
     case RX_PBF_OPT_MODE:
-        if (value != RX_PBF_MODE_PLAIN && value != RX_PBF_MODE_LEARNED) {
+        if (value != RX_PBF_MODE_PLAIN && value != RX_PBF_MODE_SCALED && value != RX_PBF_MODE_LEARNED) {
             rx_set_error(1, "rxPBDSPH_GPU::SetOption: unknown mode %g", value);
             return 1;
         }
@@ -805,6 +856,16 @@ int rxPBDSPH_GPU::SetOption(int opt, float value) {
     case RX_PBF_OPT_ADAPTIVE: m_adaptive = value != 0.0f; return 0;
     case RX_PBF_OPT_WARM_SCALE: m_warm_scale = value; return 0;
     case RX_PBF_OPT_TOL: m_tol = value; return 0;
+    case RX_PBF_OPT_JUDGE:
+        if (value != 0.0f && value != 1.0f) {
+            rx_set_error(1, "rxPBDSPH_GPU::SetOption: judge must be 0 (max) or 1 (mean)");
+            return 1;
+        }
+        m_judge = (int)value;
+        return 0;
+    case RX_PBF_OPT_RESET_RUN:
+        if (m_stats) k_fill<float><<<1, 64, 0, m_stream>>>(m_stats + ST_RUN, 0.0f, ST_N - ST_RUN);
+        return RX_HIP(hipGetLastError()) ? 0 : 1;
     case RX_PBF_OPT_ITERATIONS: m_env.max_iter = m_env.min_iter = (int)value; return 0;
     case RX_PBF_OPT_PROFILE:
         m_profile = value != 0.0f;

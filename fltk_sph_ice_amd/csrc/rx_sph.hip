@@ -179,6 +179,7 @@ __global__ __launch_bounds__(kTpb) void k_detect_neighbors(int* __restrict__ nb,
 // (including its loop bound of count - 1 neighbours).
 __global__ void k_detect_surface(const int* __restrict__ nb, int* __restrict__ surf, rxParticleCell c, const float4* __restrict__ pos,
                                  float radius, int nmax, SphK k) {
+#pragma clang fp contract(off)
     uint idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= c.uNumParticles) return;
     uint o = c.dSortedIndex[idx];
@@ -194,8 +195,8 @@ __global__ void k_detect_surface(const int* __restrict__ nb, int* __restrict__ s
     }
     float d = 1.17549435e-38f;  // FLT_MIN
     if (sm > 0.0f) {
-        sp = (1.0f / sm) * sp;
-        d = sqrtf(dot3(sp, sp));
+        sp = make_float3(sp.x / sm, sp.y / sm, sp.z / sm);
+        d = sqrtf(sp.x * sp.x + sp.y * sp.y + sp.z * sp.z);
     }
     if (nn <= 3) {
         surf[o] = nn;

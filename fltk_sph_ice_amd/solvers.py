@@ -12,7 +12,7 @@ from ._abi import check, load
 class PBFSolver:
     """rxPBDSPH_GPU (fused PBF step) for a Scene."""
 
-    STATS = 32
+    STATS = 64
 
     def __init__(self, scene, pos4: np.ndarray | None = None, vel4: np.ndarray | None = None):
         self.lib = load()

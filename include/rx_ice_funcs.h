@@ -13,7 +13,8 @@
  *    Clusters: one per particle, {p} plus every particle within `cluster_radius` of p in the rest state
  *    (IceObject::SetClusterMoveInfoFromNeight, IceObject.cpp:608-633).
  *
- * 2. The C++-linkage launch functions of Ice_SM.h:23-36 and IceObject.h:388-395, same names and arguments.
+ * 2. The C++-linkage launch functions of Ice_SM.h:23-36, IceObject.h:388-395 and HeatTransfer.cu:19-36, same
+ *    names and arguments.
  *
  * Entry points returning int give 0 on success; CuGetLastErrorString (rx_cu_funcs.h) holds the message.
  */
@@ -75,6 +76,15 @@ void LaunchShapeMatchingGPU(int prtNum, float* prtPos, float* prtVel, float* org
 /* IceObject.h:388-395 (LaunchInterPolationGPU): copies the solid positions/velocities into the fluid arrays
  * (blend coefficient 1, IceObject.cpp:876-897). sldPrtPos/Vel hold 3 floats per particle, sphPrtPos/Vel 4. */
 void LaunchInterPolationGPU(int prtNum, float* sldPrtPos, float* sldPrtVel, float* sphPrtPos, float* sphPrtVel);
+/* HeatTransfer.cu:19-36 / HeatTransfar.h:18-34 (LaunchHeatTransferGPU): one heat-exchange step of prtNum
+ * particles, device arrays of prtNum entries: air exchange of surface particles (surfParticles[i] > 0 =
+ * neighbour count), SPH diffusion over the 30-slot lists of CuSphDetectNeighborParticles (slot 29 = count;
+ * the distance is taken as 1 as in the reference), +300 heat for the particles named in
+ * meltPrtIndx[1..999] (a 1000-entry array, -1 = unused), then the latent/sensible phase update
+ * (phase -2 ice, -1 melting, 1 freezing, 2 water; phaseChangef[i] set to 1 on a completed change). */
+void LaunchHeatTransferGPU(float* heats, float* temps, float* dtemps, const int* surfParticles, const int* neightsParticles,
+                           const float* densty, int* phase, int* phaseChangef, const int* meltPrtIndx, float radius, float airTemp,
+                           float cffCnHt, float cffCnTd, float dt, int prtNum);
 #endif
 
 #endif /* RX_ICE_FUNCS_H_ */

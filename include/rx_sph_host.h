@@ -72,6 +72,8 @@ enum rxArrayType {
 /* PBF solve modes (rxPBDSPH_GPU_SetOption(RX_PBF_OPT_MODE)). */
 enum rxPbfMode {
     RX_PBF_MODE_PLAIN = 0,   /* reference Jacobi: lambda from scratch every iteration                    */
+    RX_PBF_MODE_SCALED = 1,  /* first correction uses WARM_SCALE x the first lambda (no network: the
+                                constant over-relaxation the learned start is measured against)       */
     RX_PBF_MODE_LEARNED = 2  /* first correction uses the network's prediction from the step's features */
 };
 
@@ -79,15 +81,20 @@ enum rxPbfMode {
 enum rxPbfOption {
     RX_PBF_OPT_SYNC_EACH_STEP = 0, /* 1: Update returns after the step completed on the device        */
     RX_PBF_OPT_RECORD_LAMBDA = 1,  /* 1: accumulate sum_k lambda_k (RX_ARRAY_LAMBDA_SUM)              */
-    RX_PBF_OPT_TOL = 2,            /* max density error that ends the learned solve (default 0.01)    */
-    RX_PBF_OPT_ITERATIONS = 3,     /* Jacobi iterations per step (plain solve)                        */
-    RX_PBF_OPT_SKIN = 4,           /* neighbour-list skin as a fraction of h (default 0.1)            */
+    RX_PBF_OPT_TOL = 2,            /* density error that ends an adaptive solve (default 0.01)        */
+    RX_PBF_OPT_ITERATIONS = 3,     /* Jacobi iterations per step (the maximum of an adaptive solve)   */
+    RX_PBF_OPT_SKIN = 4,           /* neighbour-list skin as a fraction of h (default 0.15)           */
     RX_PBF_OPT_PROFILE = 5,        /* 1: time every kernel class with HIP events (rxPBDSPH_GPU_GetProfile) */
     RX_PBF_OPT_RECORD_FEATURES = 6,/* 1: compute the warm-start features every step (RX_ARRAY_FEATURES)   */
     RX_PBF_OPT_MODE = 7,           /* rxPbfMode                                                           */
-    RX_PBF_OPT_ADAPTIVE = 8,       /* 1: skip the remaining corrections once the max density error at the
-                                      start of an iteration is <= TOL                                    */
-    RX_PBF_OPT_WARM_SCALE = 9      /* stability factor on the network's multiplier (learned mode)          */
+    RX_PBF_OPT_ADAPTIVE = 8,       /* 1: skip the remaining corrections once the judged density error at the
+                                      start of an iteration is <= TOL (the loop of rx_sph_pbd_gpu.cpp:862-920) */
+    RX_PBF_OPT_WARM_SCALE = 9,     /* factor on the first multiplier: the network's (learned mode) or the
+                                      first lambda's (scaled mode); default 1                              */
+    RX_PBF_OPT_JUDGE = 10,         /* density error judged against TOL: 0 = max (rho - rho0)+ / rho0,
+                                      1 = its mean over the particles (the measure of
+                                      CuPbdSphCalDensityFluctuation, rx_sph.cu:668-690)                   */
+    RX_PBF_OPT_RESET_RUN = 11      /* clears the run accumulators (stats slots 48..63)                     */
 };
 
 /* Kernel classes of rxPBDSPH_GPU_GetProfile. */
@@ -104,13 +111,19 @@ enum rxProfileClass {
 };
 
 /* Per-step statistics of rxPBDSPH_GPU_GetStats (float slots):
- *   0      max (rho - rho0)+ / rho0 at the step's start positions (= end of the previous step)
- *   1 + k  the same at the start of Jacobi iteration k
- *   17     largest displacement since the neighbour-list build
- *   18     particles whose neighbour list overflowed
- *   19     iterations executed (learned solve)
- *   22     1 when a displacement exceeded what the wide cell stencil covers
- *   23     mean |rho - rho0| / rho0 at the step's start positions                              */
+ *   0       max (rho - rho0)+ / rho0 at the step's start positions (= end of the previous step)
+ *   1 + k   the same at the start of Jacobi iteration k (k < 16)
+ *   17      largest displacement since the neighbour-list build
+ *   18      particles whose neighbour list overflowed
+ *   19      corrections executed in the step
+ *   22      1 when a displacement exceeded what the wide cell stencil covers
+ *   23      mean |rho - rho0| / rho0 at the step's start positions
+ *   24      mean (rho - rho0)+ / rho0 at the step's start positions
+ *   32 + k  the same at the start of Jacobi iteration k (k < 16)
+ * Run accumulators (since SetParticles or RX_PBF_OPT_RESET_RUN), over the steps judged so far (a step is
+ * judged by the prediction pass of the step after it):
+ *   48      max of slot 0        49  max of slot 24      50  corrections executed
+ *   51      steps judged         52  steps whose judged error exceeded TOL                         */
 
 /* Fused PBF solver (rxPBDSPH_GPU, rx_sph.h:1284-1601): Initialize (rx_sph_pbd_gpu.cpp:168-300) and Update
  * (rx_sph_pbd_gpu.cpp:790-957) with min_iter = max_iter Jacobi iterations, CPU-path lambda and collisions,

@@ -783,6 +783,117 @@ void orc_heat_diffuse_f(const float* pos4, const float* dens, const float* temp,
     }
 }
 
+// HeatTransfer.cu:62-75
+static float heat_spline(float d, float r, float w) {
+    float q = d / r;
+    if (q >= 0.0f && q < 1.0f) return w * (1.0f - 1.5f * q * q + 0.75f * q * q * q);
+    if (q >= 1.0f && q < 2.0f) return w * 0.25f * (2.0f - q) * (2.0f - q) * (2.0f - q);
+    return 0.0f;
+}
+
+// LaunchHeatTransferGPU (HeatTransfer.cu:44-57) and its four kernels (:79-204), one particle at a time.
+void orc_heat_transfer_gpu_f(float* heats, float* temps, float* dtemps, const int* surf, const int* neighs, const float* dens,
+                             int* phase, int* changed, const int* melt, float radius, float air, float ht, float td,
+                             float dt, int n) {
+    const float pi = 3.14159265359f, latent = 165.0f;
+    float wspline = 10.0f / (7.0f * pi * (float)std::pow((double)radius, 2.0));
+    for (int i = 0; i < n; ++i) {  // UpdateAirAndParticle (:79-96)
+        if (surf[i] <= 0) continue;
+        float an = 20.0f - (float)surf[i];
+        if (an < 0) an = 0.0f;
+        float area = an / 20.0f;
+        area = std::min(std::max(area, 0.0f), 1.0f);
+        heats[i] += ht * (air - temps[i]) * area;
+    }
+    for (int i = 0; i < n; ++i) {  // UpdateParticleAndParticle (:100-123)
+        float tmp = 0.0f, now = temps[i];
+        int nn = neighs[i * 30 + 29];
+        for (int j = 0; j < nn && j < 30; ++j) {
+            int id = neighs[i * 30 + j];
+            if (id == i) continue;
+            float d = dens[id];
+            if (d < 0.05f) d = 0.05f;
+            tmp += dt * 1.0f * (temps[id] - now) / d * heat_spline(1.0f, radius, wspline);
+        }
+        dtemps[i] = tmp * td;
+    }
+    if (melt)  // UpdateMeltParticle (:127-137): threads 1..999 of the launch
+        for (int k = 1; k < 1000 && k < n; ++k)
+            if (melt[k] >= 0 && melt[k] < n) heats[melt[k]] += 300.0f;
+    for (int i = 0; i < n; ++i) {  // UpdatePhase (:140-204)
+        if (phase[i] == -2 && temps[i] > 250.0f) {
+            phase[i] = -1; temps[i] = 250.0f; heats[i] = 0;
+        } else if (phase[i] == 2 && temps[i] < 250.0f) {
+            phase[i] = 1; temps[i] = 250.0f; heats[i] = latent;
+        }
+        if (phase[i] == -1 || phase[i] == 1) {
+            float c = 2.1f + (2.1f * heats[i] / latent);
+            heats[i] += dtemps[i] * c * 1.0f;
+            if (phase[i] == -1 && heats[i] < -latent / 300.0f) {
+                phase[i] = -2; temps[i] = 249.0f; heats[i] = 0;
+            } else if (phase[i] == 1 && heats[i] > latent) {
+                phase[i] = 2; temps[i] = 251.0f; heats[i] = 0;
+            }
+            if (phase[i] == -1 && heats[i] > latent) {
+                phase[i] = 2; temps[i] = 251.0f; heats[i] = 0; changed[i] = 1;
+            } else if (phase[i] == 1 && heats[i] < 0.0f) {
+                phase[i] = -2; temps[i] = 249.0f; heats[i] = 0; changed[i] = 1;
+            }
+        } else {
+            float c = (temps[i] > 250.0f) ? 4.2f : 2.1f;
+            temps[i] = temps[i] + heats[i] / (c * 1.0f);
+            heats[i] = 0.0f;
+            temps[i] += dtemps[i];
+            if (temps[i] > 500.0f) temps[i] = 500.0f;
+            if (temps[i] < 0.0f) temps[i] = 0.0f;
+        }
+    }
+}
+
+// detectNeighborParticles (rx_sph_kernel.cu:1134-1208) by brute force: the GPU visits the cells of the box
+// [x - h, x + h] in z, y, x order and each cell in sorted (stable) order, i.e. candidates by (hash, index).
+void orc_detect_neighbors_f(const float* p, int n, const OrcParams* P, int* out) {
+    std::vector<uint32_t> hash(n);
+    for (int i = 0; i < n; ++i) hash[i] = hash_of(p[4 * i], p[4 * i + 1], p[4 * i + 2], *P);
+    std::vector<std::pair<uint32_t, int>> c;
+    float h = (float)P->h;
+    for (int i = 0; i < n; ++i) {
+        c.clear();
+        for (int j = 0; j < n; ++j) {
+            float dx = p[4 * i] - p[4 * j], dy = p[4 * i + 1] - p[4 * j + 1], dz = p[4 * i + 2] - p[4 * j + 2];
+            float r = std::sqrt(dx * dx + dy * dy + dz * dz);
+            if (r <= h) c.push_back({hash[j], j});
+        }
+        std::sort(c.begin(), c.end());
+        int cnt = std::min<int>((int)c.size(), 30);
+        for (int k = 0; k < 29; ++k) out[i * 30 + k] = k < cnt ? c[k].second : -1;
+        out[i * 30 + 29] = cnt;  // with 30 hits the 30th neighbour is overwritten by the count
+    }
+}
+
+// CalDistToNormalizedMassCenter + detectSurfaceParticles (rx_sph_kernel.cu:1214-1290)
+void orc_detect_surface_f(const int* nb, int* surf, const float* p, int n, float mass, float radius) {
+    for (int i = 0; i < n; ++i) {
+        int nn = nb[i * 30 + 29];
+        float sx = 0, sy = 0, sz = 0, sm = 0;
+        for (int t = 0; t < nn - 1; ++t) {
+            int j = nb[i * 30 + t];
+            sx += mass * (p[4 * i] - p[4 * j]);
+            sy += mass * (p[4 * i + 1] - p[4 * j + 1]);
+            sz += mass * (p[4 * i + 2] - p[4 * j + 2]);
+            sm += mass;
+        }
+        float d = 1.17549435e-38f;
+        if (sm > 0.0f) {
+            sx /= sm; sy /= sm; sz /= sm;
+            d = std::sqrt(sx * sx + sy * sy + sz * sz);
+        }
+        if (nn <= 3) surf[i] = nn;
+        else if (surf[i] > 0) surf[i] = (d < 0.25f * radius) ? 0 : nn;
+        else surf[i] = (d > 0.35f * radius || d < 0.0f) ? nn : 0;
+    }
+}
+
 void orc_mlp_forward_d(const double* X, int nrows, int nin, const double* W1, const double* b1, int nh1,
                        const double* W2, const double* b2, int nh2, const double* W3, const double* b3, double* Y) {
     std::vector<double> h1(nh1), h2(nh2);

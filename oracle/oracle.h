@@ -104,6 +104,17 @@ void orc_sm_step_d(double* pos4, double* vel4, int n, const int* cl_start, const
 void orc_heat_diffuse_f(const float* pos4, const float* dens, const float* temp, int n, const OrcParams* P,
                         float diffusivity, float dt, float* temp_out);
 
+/* LaunchHeatTransferGPU (HeatTransfer.cu:44-204): air exchange, diffusion over the 30-slot lists (distance
+ * fixed to 1), melt heat for meltPrtIndx[1..min(999, n-1)], phase update; FP32 in the reference's order. */
+void orc_heat_transfer_gpu_f(float* heats, float* temps, float* dtemps, const int* surf, const int* neighs, const float* dens,
+                             int* phase, int* changed, const int* melt, float radius, float air, float ht, float td,
+                             float dt, int n);
+/* detectNeighborParticles (rx_sph_kernel.cu:1166-1208): every j with |x_i - x_j| <= h (FP32, no contraction),
+ * in the order (cell hash of j, j), at most 30; out[i*30 + k], unused slots -1, slot 29 = count. */
+void orc_detect_neighbors_f(const float* pos4, int n, const OrcParams* P, int* out);
+/* detectSurfaceParticles (rx_sph_kernel.cu:1214-1290): surf[i] updated in place from the lists. */
+void orc_detect_surface_f(const int* neighs, int* surf, const float* pos4, int n, float mass, float radius);
+
 /* ---- MLP (learned lambda warm start), FP64 */
 void orc_mlp_forward_d(const double* X, int nrows, int nin, const double* W1, const double* b1, int nh1,
                        const double* W2, const double* b2, int nh2, const double* W3, const double* b3, double* Y);
