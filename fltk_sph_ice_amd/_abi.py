@@ -214,7 +214,7 @@ RX_ARRAY_POSITION, RX_ARRAY_VELOCITY, RX_ARRAY_DENSITY, RX_ARRAY_LAMBDA_SUM, RX_
 RX_PBF_MODE_PLAIN, RX_PBF_MODE_SCALED, RX_PBF_MODE_LEARNED = 0, 1, 2
 (RX_PBF_OPT_SYNC_EACH_STEP, RX_PBF_OPT_RECORD_LAMBDA, RX_PBF_OPT_TOL, RX_PBF_OPT_ITERATIONS, RX_PBF_OPT_SKIN, RX_PBF_OPT_PROFILE,
  RX_PBF_OPT_RECORD_FEATURES, RX_PBF_OPT_MODE, RX_PBF_OPT_ADAPTIVE, RX_PBF_OPT_WARM_SCALE,
- RX_PBF_OPT_JUDGE, RX_PBF_OPT_RESET_RUN) = range(12)
+ RX_PBF_OPT_JUDGE, RX_PBF_OPT_RESET_RUN, RX_PBF_OPT_PROFILE_MASK, RX_PBF_OPT_LIST_TILE) = range(14)
 PROF_CLASSES = ["predict", "grid", "list_lambda", "lambda", "dp", "velocity", "xsph", "features", "mlp"]
 
 _lib = None

@@ -124,23 +124,68 @@ __device__ inline f32x4 forward16(bf16x4 a1, const MlpDev& m, unsigned short* ld
     return p;
 }
 
-// 4 waves per block, 16 rows per wave.
+// Inference in transposed form, no LDS: with the weights as the A operand and the activations as the B
+// operand, H^T = W X^T leaves lane l holding H[row l % 16][units 4 (l / 16) .. + 3] of a 16-unit tile, which is
+// the B fragment of the next layer's 16-deep K step.  Every lane keeps the weight fragments in registers
+// and a wave walks 16-row tiles with a grid stride.
 __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ X, uint n, MlpDev m, float* __restrict__ y,
                                                    float4* __restrict__ xl, float w_scale) {
-    __shared__ __align__(16) unsigned short lds[4][16 * kMlpH];
-    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-    uint r0 = (blockIdx.x * 4 + wave) * 16;
-    bf16x4 a1 = load_x(X, r0, n, m);
-    f32x4 p = forward16(a1, m, lds[wave], nullptr, nullptr, r0, n);
-    if ((l & 15) == 0) {
+    const int l = threadIdx.x & 63, i = l & 15, kb = l >> 4;
+    bf16x4 w1[4], w2[4][4];
+    float b1[4][4], b2[4][4], w3[4][4], mu[4], isd[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        w1[t] = ld_bf4(m.w1 + (16 * t + i) * kMlpIn + 4 * kb);  // A = W1[16t + i][4kb + e]
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) w2[t][ks] = ld_bf4(m.w2 + (16 * t + i) * kMlpH + 16 * ks + 4 * kb);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            uint r = r0 + 4 * (l >> 4) + e;
-            if (r < n) {
-                float t = p[e] * m.out_scale + m.out_bias;
-                if (y) y[r] = t;
-                if (xl) xl[r].w = t * w_scale;
-            }
+            int o = 16 * t + 4 * kb + e;  // unit held by this lane in D
+            b1[t][e] = m.b1[o];
+            b2[t][e] = m.b2[o];
+            w3[t][e] = m.w3[o];
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        mu[e] = m.in_mu[4 * kb + e];
+        isd[e] = m.in_isd[4 * kb + e];
+    }
+    const float b3 = m.b3[0];
+    const uint waves = gridDim.x * (blockDim.x >> 6);
+    for (uint tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); (size_t)tile * 16 < n; tile += waves) {
+        uint r = tile * 16 + i;
+        float v[4] = {0, 0, 0, 0};
+        if (r < n) {
+            float4 x = *(const float4*)(X + (size_t)r * kMlpIn + 4 * kb);
+            v[0] = (x.x - mu[0]) * isd[0];
+            v[1] = (x.y - mu[1]) * isd[1];
+            v[2] = (x.z - mu[2]) * isd[2];
+            v[3] = (x.w - mu[3]) * isd[3];
+        }
+        const bf16x4 xb = pack4(v[0], v[1], v[2], v[3]);  // B = X^T[4kb + e][row i]
+        bf16x4 h1[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            f32x4 d = {0, 0, 0, 0};
+            d = mfma(w1[t], xb, d);  // D[unit 4kb + e][row i] of units 16t..16t+15
+            h1[t] = pack4(silu(d[0] + b1[t][0]), silu(d[1] + b1[t][1]), silu(d[2] + b1[t][2]), silu(d[3] + b1[t][3]));
+        }
+        float p = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            f32x4 acc = {0, 0, 0, 0};
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) acc = mfma(w2[t][ks], h1[ks], acc);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) p += silu(acc[e] + b2[t][e]) * w3[t][e];
+        }
+        p += __shfl_xor(p, 16, 64);
+        p += __shfl_xor(p, 32, 64);
+        if (kb == 0 && r < n) {
+            float out = (p + b3) * m.out_scale + m.out_bias;
+            if (y) y[r] = out;
+            if (xl) xl[r].w = out * w_scale;
         }
     }
 }
@@ -509,11 +554,17 @@ public:
     int m_t = 0;
 };
 
+// 4 waves per block; up to 8 tiles of 16 rows per wave, at most 2048 blocks (8 per CU).
+static unsigned infer_blocks(uint n) {
+    uint tiles = (n + 15) / 16;
+    uint blocks = (tiles + 31) / 32;
+    return blocks < 1 ? 1 : (blocks > 2048 ? 2048 : blocks);
+}
 void rx_mlp_infer_to_w(const float* feat, float4* xl, uint n, const MlpDev& m, float w_scale, hipStream_t s) {
-    if (n) k_mlp_infer<<<(n + 63) / 64, 256, 0, s>>>(feat, n, m, nullptr, xl, w_scale);
+    if (n) k_mlp_infer<<<infer_blocks(n), 256, 0, s>>>(feat, n, m, nullptr, xl, w_scale);
 }
 void rx_mlp_infer(const float* feat, float* y, uint n, const MlpDev& m, hipStream_t s) {
-    if (n) k_mlp_infer<<<(n + 63) / 64, 256, 0, s>>>(feat, n, m, y, nullptr, 1.0f);
+    if (n) k_mlp_infer<<<infer_blocks(n), 256, 0, s>>>(feat, n, m, y, nullptr, 1.0f);
 }
 bool rx_mlp_view(void* obj, MlpDev* out) {
     if (!obj) return false;

@@ -53,6 +53,7 @@ private:
     bool m_adaptive = false;
     float m_warm_scale = 1.0f;
     int m_judge = 0;
+    int m_list_tile = 0;  // LDS tiling variant of the list pass (RX_PBF_OPT_LIST_TILE)
     float m_sd_ref = 1.0f, m_nref = 1.0f;
     MlpDev m_mlp{};
     bool m_mlp_ready = false;
@@ -67,7 +68,6 @@ private:
     float* m_dens = nullptr;
     float* m_lamacc = nullptr;  // sum of lambda over the iterations of the last step
     float* m_ftmp = nullptr;
-    float* m_dstart = nullptr;   // density at the step start
     float* m_feat = nullptr;     // 16 features per particle (learned warm start)
     float4* m_ftmp16 = nullptr;
     uint* m_id = nullptr;       // slot -> original particle index
@@ -77,6 +77,7 @@ private:
     float* m_stats = nullptr;
     // profiling
     bool m_profile = false;
+    unsigned m_profile_mask = ~0u;  // rxProfileClass bits timed while profiling
     struct Mark {
         int cls;
         hipEvent_t a, b;

@@ -102,7 +102,7 @@ __device__ inline void block_max_atomic(float v, float* dst) {
     __syncthreads();
     if (threadIdx.x == 0) {
         float m = wm[0];
-        for (int w = 1; w < kTpb / 64; ++w) m = fmaxf(m, wm[w]);
+        for (uint w = 1; w < (blockDim.x >> 6); ++w) m = fmaxf(m, wm[w]);
         atomic_max_pos(dst, m);
     }
 }
@@ -115,7 +115,7 @@ __device__ inline void block_sum_atomic(float v, float* dst) {
     __syncthreads();
     if (threadIdx.x == 0) {
         float m = 0.0f;
-        for (int w = 0; w < kTpb / 64; ++w) m += ws[w];
+        for (uint w = 0; w < (blockDim.x >> 6); ++w) m += ws[w];
         atomicAdd(dst, m);
     }
 }
@@ -170,7 +170,7 @@ __device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint
 
 // ------------------------------------------------------------------------------------------ predict
 // rxPBDSPH::calDensity + calForceExtAndVisc + integrate (rx_sph_pbd.cpp:599-689, 1409-1446), fused.
-// Reads x, v (w unused); writes x* to xs.  Also judges the previous step: max (rho - rho0)+ / rho0.
+// Reads x, v (w unused); writes (x*, rho_start) to xs.  Also judges the previous step: max (rho - rho0)+ / rho0.
 __global__ __launch_bounds__(kTpb) void k_predict(const float4* __restrict__ x, const float4* __restrict__ v, float4* __restrict__ xs,
                                                   float* __restrict__ dens, float* __restrict__ stats, int mode, PbfK k,
                                                   const uint* __restrict__ list, const uint* __restrict__ cnt,
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(kTpb) void k_predict(const float4* __restrict__ x, 
         float3 a = (k.s.visc * k.s.mass * k.s.lwvisc / d) * fv + k.s.gravity;  // viscosity / rho_i (rx_sph_pbd.cpp:676)
         float3 vn = vi + k.dt * a;
         float3 xn = clamp_wall(p + k.dt * vn, k.s);
-        xs[i] = make_float4(xn.x, xn.y, xn.z, 0.0f);
+        xs[i] = make_float4(xn.x, xn.y, xn.z, d);  // w: density at the step start (features)
         dens[i] = d;
         float e = (d - k.s.rho0) * k.s.inv_rho0;
         err = fmaxf(e, 0.0f);
@@ -226,93 +226,173 @@ __global__ void k_fold_run(float* __restrict__ stats, PbfK k) {
 // Builds the skin list of slot i from the new grid (cell = h + skin) and computes lambda at x*
 // (rx_sph_pbd.cpp:699-822: C = rho/rho0 - 1, sd with the self term).  Writes xl = (x*, lambda).  When feat is
 // set it also writes the 16 features of the learned warm start (12 used, 4 zero padding) for slot i.
-__global__ __launch_bounds__(kTpb) void k_list_lambda(const float4* __restrict__ xs, float4* __restrict__ xl, float4* __restrict__ xbuild,
+// `cands(f)` calls f(j, xs[j]) for every candidate j of the 27 cells around slot i, rows in z, y order and
+// slots ascending within a row.
+template <class Cands>
+__device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, float4* __restrict__ xl, float4* __restrict__ xbuild,
+                                        float* __restrict__ dens, uint* __restrict__ list, uint* __restrict__ cnt,
+                                        float* __restrict__ stats, const PbfK& k, float* __restrict__ feat,
+                                        const float4* __restrict__ v, float& emax, float& esum) {
+    float3 p = f3(p4);
+    float d = 0.0f, sd = 0.0f;
+    float3 gs = f3(0, 0, 0);
+    uint c = 0;
+    // learned-start extras
+    float nn = 0.0f, wsum = 0.0f, wcs = 0.0f, ring = 0.0f, nring = 0.0f, div = 0.0f;
+    const float h2q = 0.25f * k.s.h2;
+    float3 vi = feat ? f3(v[i]) : f3(0, 0, 0);
+    cands([&](uint j, float4 q4) {  // q4 = (x*_j, density at the step start)
+        float3 r = p - f3(q4);
+        float r2 = dot3(r, r);
+        if (r2 <= k.rc2) {
+            if (c < (uint)k.maxn) list[(size_t)c * k.n + i] = j;
+            ++c;
+            if (r2 <= k.s.h2) {
+                float q2 = k.s.h2 - r2;
+                float w = q2 * q2 * q2;
+                d += w;
+                float3 g = f3(0, 0, 0);
+                if (r2 > 0.0f) {
+                    float ri = rinv_of(r2);
+                    float q = k.s.h - r2 * ri;
+                    g = (k.s.gwspiky * k.s.inv_rho0 * q * q * ri) * r;
+                    gs += g;
+                    sd += dot3(g, g);
+                }
+                if (feat) {
+                    float cj = q4.w * k.s.inv_rho0 - 1.0f;
+                    nn += 1.0f;
+                    wsum += w;
+                    wcs += w * cj;
+                    if (r2 > h2q) {
+                        ring += cj;
+                        nring += 1.0f;
+                    }
+                    div += dot3(f3(v[j]) - vi, g);
+                }
+            }
+        }
+    });
+    cnt[i] = c;
+    if (c > (uint)k.maxn) atomicAdd((unsigned int*)&stats[ST_OVERFLOW], 1u);
+    d *= k.s.mass * k.s.wpoly6;
+    float sgs = dot3(gs, gs);
+    sd += sgs;
+    float C = d * k.s.inv_rho0 - 1.0f;
+    float lam = -C / (sd + k.eps);
+    xl[i] = make_float4(p.x, p.y, p.z, lam * k.lam0_scale);
+    xbuild[i] = p4;
+    dens[i] = d;
+    float err = fmaxf(C, 0.0f);
+    emax = fmaxf(emax, err);
+    esum += err;
+    if (feat) {
+        float wall = fminf(fminf(fminf(p.x - k.s.wall_lo.x, k.s.wall_hi.x - p.x), fminf(p.y - k.s.wall_lo.y, k.s.wall_hi.y - p.y)),
+                           fminf(p.z - k.s.wall_lo.z, k.s.wall_hi.z - p.z));
+        float f[12];
+        f[0] = C;
+        f[1] = lam * k.sd_ref;
+        f[2] = sd / k.sd_ref - 1.0f;
+        f[3] = nn * k.inv_nref - 1.0f;
+        f[4] = sqrtf(sgs) * k.inv_gref;
+        f[5] = gs.y * k.inv_gref;
+        f[6] = wsum > 0.0f ? wcs / wsum : 0.0f;       // W-weighted mean of the step-start C
+        f[7] = nring > 0.0f ? ring / nring : 0.0f;     // mean step-start C of the outer ring (h/2 < r <= h)
+        f[8] = p4.w * k.s.inv_rho0 - 1.0f;
+        f[9] = div * k.s.mass * k.dt;
+        f[10] = fminf(fmaxf(wall * k.inv_h, 0.0f), 1.0f) - 0.5f;
+        f[11] = vi.y * k.dt * k.inv_spacing;
+        float4* fo = (float4*)(feat + (size_t)i * 16);
+        fo[0] = make_float4(f[0], f[1], f[2], f[3]);
+        fo[1] = make_float4(f[4], f[5], f[6], f[7]);
+        fo[2] = make_float4(f[8], f[9], f[10], f[11]);
+        fo[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+// One workgroup per segment of kSegW cells along x of one (y, z) row.  The candidates of the whole segment
+// are the slots of 9 rows (y - 1 .. y + 1, z - 1 .. z + 1) over the cells x0 - 1 .. x1 + 1: nine contiguous
+// slot ranges, staged into LDS with coalesced loads.  Every home slot then walks its 27 cells from LDS.  A
+// segment whose neighbourhood exceeds the LDS tile (very dense regions) reads the candidates from global
+// memory instead; both paths visit the same candidates in the same order.
+template <int kSegW>
+inline unsigned list_segments(const FGridK& g) { return (unsigned)(((g.gx + kSegW - 1) / kSegW) * g.gy * g.gz); }
+
+// kSegW cells per segment, kTileCap staged float4 entries, kThreads threads per block (a multiple of 64, <= 256).
+template <int kSegW, int kTileCap, int kThreads>
+__global__ __launch_bounds__(kThreads) void k_list_lambda(const float4* __restrict__ xs, float4* __restrict__ xl, float4* __restrict__ xbuild,
                                                       float* __restrict__ dens, uint* __restrict__ list, uint* __restrict__ cnt,
                                                       float* __restrict__ stats, PbfK k, const uint* __restrict__ bhash,
                                                       const uint* __restrict__ off, float* __restrict__ feat,
-                                                      const float4* __restrict__ v, const float* __restrict__ dstart) {
-    bool valid;
-    uint i = slot_of(k, valid);
-    float err = 0.0f;
-    if (valid) {
-        float4 p4 = xs[i];
-        float3 p = f3(p4);
-        float d = 0.0f, sd = 0.0f;
-        float3 gs = f3(0, 0, 0);
-        uint c = 0;
-        // learned-start extras
-        float nn = 0.0f, wsum = 0.0f, wcs = 0.0f, ring = 0.0f, nring = 0.0f, div = 0.0f;
-        const float h2q = 0.25f * k.s.h2;
-        float3 vi = feat ? f3(v[i]) : f3(0, 0, 0);
-        fg_for_stencil(fg_unhash(bhash[i], k.g), k.g, off, [&](uint j) {
-            float3 r = p - f3(xs[j]);
-            float r2 = dot3(r, r);
-            if (r2 <= k.rc2) {
-                if (c < (uint)k.maxn) list[(size_t)c * k.n + i] = j;
-                ++c;
-                if (r2 <= k.s.h2) {
-                    float q2 = k.s.h2 - r2;
-                    float w = q2 * q2 * q2;
-                    d += w;
-                    float3 g = f3(0, 0, 0);
-                    if (r2 > 0.0f) {
-                        float ri = rinv_of(r2);
-                        float q = k.s.h - r2 * ri;
-                        g = (k.s.gwspiky * k.s.inv_rho0 * q * q * ri) * r;
-                        gs += g;
-                        sd += dot3(g, g);
-                    }
-                    if (feat) {
-                        float cj = dstart[j] * k.s.inv_rho0 - 1.0f;
-                        nn += 1.0f;
-                        wsum += w;
-                        wcs += w * cj;
-                        if (r2 > h2q) {
-                            ring += cj;
-                            nring += 1.0f;
-                        }
-                        div += dot3(f3(v[j]) - vi, g);
-                    }
+                                                      const float4* __restrict__ v) {
+    __shared__ float4 tile[kTileCap > 0 ? kTileCap : 1];
+    __shared__ uint s_lds0[10], s_glob0[9], s_len[9];
+    __shared__ uint s_coff[9][kSegW + 3];
+    const FGridK& g = k.g;
+    const int nsx = (g.gx + kSegW - 1) / kSegW;
+    const uint seg = xcd_block(blockIdx.x, gridDim.x);
+    const int sx = (int)(seg % (uint)nsx);
+    const uint yz = seg / (uint)nsx;
+    const int y = (int)(yz % (uint)g.gy), z = (int)(yz / (uint)g.gy);
+    const int x0 = sx * kSegW, x1 = min(x0 + kSegW, g.gx) - 1;
+    const uint hb = (uint)((z * g.gy + y) * g.gx);
+    const uint hs = off[hb + x0], he = off[hb + x1 + 1];
+    if (hs == he) return;  // the same for every thread of the block
+    const int xl0 = max(x0 - 1, 0), xh0 = min(x1 + 1, g.gx - 1), ncol = xh0 - xl0 + 1;
+    if (threadIdx.x < 9) {
+        const int r = threadIdx.x, zz = z + r / 3 - 1, yy = y + r % 3 - 1;
+        uint g0 = 0, len = 0;
+        if (zz >= 0 && zz < g.gz && yy >= 0 && yy < g.gy) {
+            const uint rb = (uint)((zz * g.gy + yy) * g.gx);
+            g0 = off[rb + xl0];
+            len = off[rb + xh0 + 1] - g0;
+            for (int t = 0; t <= ncol; ++t) s_coff[r][t] = off[rb + xl0 + t] - g0;
+        } else {
+            for (int t = 0; t <= ncol; ++t) s_coff[r][t] = 0;
+        }
+        s_glob0[r] = g0;
+        s_len[r] = len;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint a = 0;
+        for (int r = 0; r < 9; ++r) {
+            s_lds0[r] = a;
+            a += s_len[r];
+        }
+        s_lds0[9] = a;
+    }
+    __syncthreads();
+    const bool tiled = s_lds0[9] <= (uint)kTileCap;
+    if (tiled) {
+        const uint total = s_lds0[9];
+        for (uint t = threadIdx.x; t < total; t += blockDim.x) {
+            int r = 0;
+            while (t >= s_lds0[r + 1]) ++r;
+            tile[t] = xs[s_glob0[r] + (t - s_lds0[r])];
+        }
+        __syncthreads();
+    }
+    float emax = 0.0f, esum = 0.0f;
+    for (uint i = hs + threadIdx.x; i < he; i += blockDim.x) {
+        const int cx = (int)(bhash[i] % (uint)g.gx);
+        const int a = max(cx - 1, 0) - xl0, b = min(cx + 1, g.gx - 1) - xl0 + 1;
+        const float4 p4 = tiled ? tile[s_lds0[4] + (i - s_glob0[4])] : xs[i];
+        auto cands = [&](auto&& f) {
+            for (int r = 0; r < 9; ++r) {
+                const uint cs = s_coff[r][a], ce = s_coff[r][b], gb = s_glob0[r];
+                if (tiled) {
+                    const uint lb = s_lds0[r];
+                    for (uint t = cs; t < ce; ++t) f(gb + t, tile[lb + t]);
+                } else {
+                    for (uint t = cs; t < ce; ++t) f(gb + t, xs[gb + t]);
                 }
             }
-        });
-        cnt[i] = c;
-        if (c > (uint)k.maxn) atomicAdd((unsigned int*)&stats[ST_OVERFLOW], 1u);
-        d *= k.s.mass * k.s.wpoly6;
-        float sgs = dot3(gs, gs);
-        sd += sgs;
-        float C = d * k.s.inv_rho0 - 1.0f;
-        float lam = -C / (sd + k.eps);
-        xl[i] = make_float4(p.x, p.y, p.z, lam * k.lam0_scale);
-        xbuild[i] = p4;
-        dens[i] = d;
-        err = fmaxf(C, 0.0f);
-        if (feat) {
-            float wall = fminf(fminf(fminf(p.x - k.s.wall_lo.x, k.s.wall_hi.x - p.x), fminf(p.y - k.s.wall_lo.y, k.s.wall_hi.y - p.y)),
-                               fminf(p.z - k.s.wall_lo.z, k.s.wall_hi.z - p.z));
-            float f[16];
-            f[0] = C;
-            f[1] = lam * k.sd_ref;
-            f[2] = sd / k.sd_ref - 1.0f;
-            f[3] = nn * k.inv_nref - 1.0f;
-            f[4] = sqrtf(sgs) * k.inv_gref;
-            f[5] = gs.y * k.inv_gref;
-            f[6] = wsum > 0.0f ? wcs / wsum : 0.0f;       // W-weighted mean of the step-start C
-            f[7] = nring > 0.0f ? ring / nring : 0.0f;     // mean step-start C of the outer ring (h/2 < r <= h)
-            f[8] = dstart[i] * k.s.inv_rho0 - 1.0f;
-            f[9] = div * k.s.mass * k.dt;
-            f[10] = fminf(fmaxf(wall * k.inv_h, 0.0f), 1.0f) - 0.5f;
-            f[11] = vi.y * k.dt * k.inv_spacing;
-            f[12] = f[13] = f[14] = f[15] = 0.0f;
-            float4* fo = (float4*)(feat + (size_t)i * 16);
-            fo[0] = make_float4(f[0], f[1], f[2], f[3]);
-            fo[1] = make_float4(f[4], f[5], f[6], f[7]);
-            fo[2] = make_float4(f[8], f[9], f[10], f[11]);
-            fo[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        }
+        };
+        list_lambda_slot(i, p4, cands, xl, xbuild, dens, list, cnt, stats, k, feat, v, emax, esum);
     }
-    block_max_atomic(err, &stats[ST_ERR_ITER + 0]);
-    block_sum_atomic(err, &stats[ST_MPOS_ITER + 0]);
+    block_max_atomic(emax, &stats[ST_ERR_ITER + 0]);
+    block_sum_atomic(esum, &stats[ST_MPOS_ITER + 0]);
 }
 
 // lambda pass (rx_sph_pbd.cpp:699-822) over the list: reads positions xa, writes xl = (x, lambda).
@@ -467,7 +547,7 @@ void rxPBDSPH_GPU::Finalize() {
     if (m_stream) RX_HIP(hipStreamSynchronize(m_stream));
     for (void** p : {(void**)&m_x, (void**)&m_xs, (void**)&m_xl, (void**)&m_xb, (void**)&m_v, (void**)&m_t, (void**)&m_dens,
                      (void**)&m_lamacc, (void**)&m_id, (void**)&m_idt, (void**)&m_list, (void**)&m_cnt, (void**)&m_stats,
-                     (void**)&m_ftmp, (void**)&m_dstart, (void**)&m_feat, (void**)&m_ftmp16})
+                     (void**)&m_ftmp, (void**)&m_feat, (void**)&m_ftmp16})
         if (*p) {
             RX_HIP(hipFree(*p));
             *p = nullptr;
@@ -504,7 +584,7 @@ bool rxPBDSPH_GPU::alloc(uint n) {
               RX_HIP(hipMalloc(&m_dens, f)) && RX_HIP(hipMalloc(&m_lamacc, f)) && RX_HIP(hipMalloc(&m_id, u)) &&
               RX_HIP(hipMalloc(&m_idt, u)) && RX_HIP(hipMalloc(&m_list, u * (size_t)m_maxn)) && RX_HIP(hipMalloc(&m_cnt, u)) &&
               RX_HIP(hipMalloc(&m_stats, sizeof(float) * ST_N)) && RX_HIP(hipMalloc(&m_ftmp, f)) &&
-              RX_HIP(hipMalloc(&m_dstart, f)) && RX_HIP(hipMalloc(&m_feat, v4 * 4)) &&
+              RX_HIP(hipMalloc(&m_feat, v4 * 4)) &&
               RX_HIP(hipMalloc(&m_ftmp16, v4 * 4));
     m_grid.n = n;
     m_grid.reserve(n);
@@ -519,7 +599,7 @@ int rxPBDSPH_GPU::SetParticles(const float* pos4, const float* vel4, int n) {
     if ((uint)n != m_n) {
         for (void** p : {(void**)&m_x, (void**)&m_xs, (void**)&m_xl, (void**)&m_xb, (void**)&m_v, (void**)&m_t, (void**)&m_dens,
                          (void**)&m_lamacc, (void**)&m_id, (void**)&m_idt, (void**)&m_list, (void**)&m_cnt, (void**)&m_stats,
-                         (void**)&m_ftmp, (void**)&m_dstart, (void**)&m_feat, (void**)&m_ftmp16})
+                         (void**)&m_ftmp, (void**)&m_feat, (void**)&m_ftmp16})
             if (*p) {
                 RX_HIP(hipFree(*p));
                 *p = nullptr;
@@ -643,15 +723,33 @@ int rxPBDSPH_GPU::Update(float dt, int step) {
     reorder(m_v, s);
     fg_gather<uint>(m_idt, m_id, m_grid.perm, n, s);
     std::swap(m_id, m_idt);
-    fg_gather<float>(m_dstart, m_dens, m_grid.perm, n, s);
     rx_fill<float>(m_lamacc, 0.0f, n, s);
     end_mark(mk);
 
     // ---- list + lambda_0 (the network's multiplier in learned mode), then up to K corrections
     const bool feats = (m_mode == RX_PBF_MODE_LEARNED) || m_record_features;
     mk = begin_mark(feats ? RX_PROF_FEATURES : RX_PROF_LIST_LAMBDA);
-    k_list_lambda<<<nb, kTpb, 0, s>>>(m_xs, m_xl, m_xb, m_dens, m_list, m_cnt, m_stats, k, m_grid.hash, m_grid.off,
-                                      feats ? m_feat : nullptr, m_v, m_dstart);
+    {
+        float* f = feats ? m_feat : nullptr;
+        const FGridK& gk = m_grid.k;
+        switch (m_list_tile) {
+        case 1:
+            k_list_lambda<16, 2048, 256><<<list_segments<16>(gk), 256, 0, s>>>(m_xs, m_xl, m_xb, m_dens, m_list, m_cnt, m_stats, k,
+                                                                             m_grid.hash, m_grid.off, f, m_v);
+            break;
+        case 2:
+            k_list_lambda<8, 1536, 128><<<list_segments<8>(gk), 128, 0, s>>>(m_xs, m_xl, m_xb, m_dens, m_list, m_cnt, m_stats, k,
+                                                                           m_grid.hash, m_grid.off, f, m_v);
+            break;
+        case 3:  // no staging: every segment reads its candidates from global memory
+            k_list_lambda<16, 0, 256><<<list_segments<16>(gk), 256, 0, s>>>(m_xs, m_xl, m_xb, m_dens, m_list, m_cnt, m_stats, k,
+                                                                          m_grid.hash, m_grid.off, f, m_v);
+            break;
+        default:
+            k_list_lambda<16, 3072, 256><<<list_segments<16>(gk), 256, 0, s>>>(m_xs, m_xl, m_xb, m_dens, m_list, m_cnt, m_stats, k,
+                                                                             m_grid.hash, m_grid.off, f, m_v);
+        }
+    }
     end_mark(mk);
     if (m_mode == RX_PBF_MODE_LEARNED) {
         mk = begin_mark(RX_PROF_MLP);
@@ -758,7 +856,7 @@ int rxPBDSPH_GPU::SetNetwork(void* mlp) {
 }
 
 int rxPBDSPH_GPU::begin_mark(int cls) {
-    if (!m_profile) return -1;
+    if (!m_profile || !((m_profile_mask >> cls) & 1u)) return -1;
     if (m_event_next + 2 > m_event_pool.size()) {
         for (int t = 0; t < 64; ++t) {
             hipEvent_t e;
@@ -869,6 +967,16 @@ int rxPBDSPH_GPU::SetOption(int opt, float value) {
     case RX_PBF_OPT_ITERATIONS: m_env.max_iter = m_env.min_iter = (int)value; return 0;
     case RX_PBF_OPT_PROFILE:
         m_profile = value != 0.0f;
+        return 0;
+    case RX_PBF_OPT_PROFILE_MASK:
+        m_profile_mask = (unsigned)value;
+        return 0;
+    case RX_PBF_OPT_LIST_TILE:
+        if (value < 0.0f || value > 3.0f) {
+            rx_set_error(1, "rxPBDSPH_GPU::SetOption: list tile variant must be 0..3");
+            return 1;
+        }
+        m_list_tile = (int)value;
         return 0;
     case RX_PBF_OPT_SKIN:
         if (value < 0.0f) return 1;

@@ -94,7 +94,11 @@ enum rxPbfOption {
     RX_PBF_OPT_JUDGE = 10,         /* density error judged against TOL: 0 = max (rho - rho0)+ / rho0,
                                       1 = its mean over the particles (the measure of
                                       CuPbdSphCalDensityFluctuation, rx_sph.cu:668-690)                   */
-    RX_PBF_OPT_RESET_RUN = 11      /* clears the run accumulators (stats slots 48..63)                     */
+    RX_PBF_OPT_RESET_RUN = 11,     /* clears the run accumulators (stats slots 48..63)                     */
+    RX_PBF_OPT_PROFILE_MASK = 12,  /* bit c set: time kernel class c (rxProfileClass) while profiling;
+                                      default all                                                          */
+    RX_PBF_OPT_LIST_TILE = 13      /* list pass: 0 = 16-cell segments, 48 KB LDS tile (default); 1 = 32 KB
+                                      tile; 2 = 8-cell segments, 24 KB tile, 128 threads; 3 = no LDS tile  */
 };
 
 /* Kernel classes of rxPBDSPH_GPU_GetProfile. */
