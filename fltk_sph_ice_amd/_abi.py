@@ -168,6 +168,16 @@ SIGNATURES: dict[str, tuple] = {
     "rxPBDSPH_GPU_GetProfile": (i32, [vp, vp, vp, i32]),
     "rxPBDSPH_GPU_SetNetwork": (i32, [vp, vp]),
     "rxPBDSPH_GPU_FeatureScales": (i32, [vp, fp, fp]),
+    "rxPBDSPH_GPU_SetThermal": (i32, [vp, vp, vp, vp]),
+    "rxCoupled_Create": (vp, [vp, vp, vp, vp, i32, i32, vp, vp]),
+    "rxCoupled_Destroy": (None, [vp]),
+    "rxCoupled_Update": (i32, [vp, flt, i32]),
+    "rxCoupled_Sync": (i32, [vp]),
+    "rxCoupled_GetArray": (i32, [vp, i32, vp, i32]),
+    "rxCoupled_GetStats": (i32, [vp, vp, i32]),
+    "rxCoupled_SetOption": (i32, [vp, i32, flt]),
+    "rxCoupled_SetIceTiles": (i32, [vp, i32, vp]),
+    "rxCoupled_GetProfile": (i32, [vp, vp, vp, i32]),
     "rxSPH_GPU_Create": (vp, [C.POINTER(rxSphEnv)]),
     "rxSPH_GPU_Destroy": (None, [vp]),
     "rxSPH_GPU_SetParticles": (i32, [vp, vp, vp, i32]),
@@ -189,6 +199,7 @@ SIGNATURES: dict[str, tuple] = {
     "rxIceSM_SetTiles": (i32, [vp, vp]),
     "rxIceSM_UseGraph": (i32, [vp, i32]),
     "rxIceSM_KernelTimes": (i32, [vp, fp, fp]),
+    "rxIceSM_SetActive": (i32, [vp, vp]),
     # rx_mlp.h
     "rxMLP_NumParams": (i32, []),
     "rxMLP_Create": (vp, [u32]),
@@ -210,12 +221,14 @@ CXX_SYMBOLS = {
                               (None, [vp, vp, vp, vp, vp, vp, vp, vp, vp, flt, flt, flt, flt, flt, i32])),
 }
 
-RX_ARRAY_POSITION, RX_ARRAY_VELOCITY, RX_ARRAY_DENSITY, RX_ARRAY_LAMBDA_SUM, RX_ARRAY_PRESSURE, RX_ARRAY_FEATURES = range(6)
+(RX_ARRAY_POSITION, RX_ARRAY_VELOCITY, RX_ARRAY_DENSITY, RX_ARRAY_LAMBDA_SUM, RX_ARRAY_PRESSURE, RX_ARRAY_FEATURES,
+ RX_ARRAY_TEMPERATURE, RX_ARRAY_HEAT, RX_ARRAY_PHASE) = range(9)
 RX_PBF_MODE_PLAIN, RX_PBF_MODE_SCALED, RX_PBF_MODE_LEARNED = 0, 1, 2
 (RX_PBF_OPT_SYNC_EACH_STEP, RX_PBF_OPT_RECORD_LAMBDA, RX_PBF_OPT_TOL, RX_PBF_OPT_ITERATIONS, RX_PBF_OPT_SKIN, RX_PBF_OPT_PROFILE,
  RX_PBF_OPT_RECORD_FEATURES, RX_PBF_OPT_MODE, RX_PBF_OPT_ADAPTIVE, RX_PBF_OPT_WARM_SCALE,
- RX_PBF_OPT_JUDGE, RX_PBF_OPT_RESET_RUN, RX_PBF_OPT_PROFILE_MASK, RX_PBF_OPT_LIST_TILE) = range(14)
-PROF_CLASSES = ["predict", "grid", "list_lambda", "lambda", "dp", "velocity", "xsph", "features", "mlp"]
+ RX_PBF_OPT_JUDGE, RX_PBF_OPT_RESET_RUN, RX_PBF_OPT_PROFILE_MASK, RX_PBF_OPT_LIST_TILE, RX_PBF_OPT_HEAT_TD,
+ RX_PBF_OPT_HEAT_DT) = range(16)
+PROF_CLASSES = ["predict", "grid", "list_lambda", "lambda", "dp", "velocity", "xsph", "features", "mlp", "heat"]
 
 _lib = None
 

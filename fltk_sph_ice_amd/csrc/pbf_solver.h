@@ -22,6 +22,7 @@ public:
     int SetOption(int opt, float value);
     int Sync();
     int SetNetwork(void* mlp_object);  // weights of an rxMLP object (copied)
+    int SetThermal(const float* temps, const float* heats, const int* phase);
     void FeatureScales(float* sd_ref, float* nref) const {
         if (sd_ref) *sd_ref = m_sd_ref;
         if (nref) *nref = m_nref;
@@ -30,6 +31,19 @@ public:
     // total milliseconds and launch count per kernel class since the last reset.
     int GetProfile(float* ms, int* count, int n);
     hipStream_t stream() const { return m_stream; }
+    // Device view of the cell-ordered state after Update (valid until the next Update): positions, velocities,
+    // slot -> particle ids, list-build positions, thermal state (NULL when not thermal), statistics array and
+    // the slot of the largest-displacement statistic.
+    struct DeviceView {
+        float4* x;
+        float4* v;
+        const uint* id;
+        const float4* xb;
+        const float4* therm;
+        float* stats;
+        int disp2_slot;
+    };
+    DeviceView device_view() const;
     uint size() const { return m_n; }
 
 private:
@@ -68,6 +82,9 @@ private:
     float* m_dens = nullptr;
     float* m_lamacc = nullptr;  // sum of lambda over the iterations of the last step
     float* m_ftmp = nullptr;
+    float4* m_therm = nullptr;   // (temperature, heat, phase, changed) per particle, when thermal
+    float4* m_therm_t = nullptr;
+    float m_heat_td = 0.0f, m_heat_dt = 0.0f;
     float* m_feat = nullptr;     // 16 features per particle (learned warm start)
     float4* m_ftmp16 = nullptr;
     uint* m_id = nullptr;       // slot -> original particle index

@@ -157,12 +157,14 @@ __device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint
     if (ok) {
         const uint* l = list + i;
         uint s = 0;
-        for (; s + 1 < c; s += 2) {  // two independent gathers in flight
-            uint j0 = l[(size_t)s * k.n], j1 = l[(size_t)(s + 1) * k.n];
+        for (; s + 3 < c; s += 4) {  // four independent gathers in flight
+            uint j0 = l[(size_t)s * k.n], j1 = l[(size_t)(s + 1) * k.n], j2 = l[(size_t)(s + 2) * k.n], j3 = l[(size_t)(s + 3) * k.n];
             f(j0);
             f(j1);
+            f(j2);
+            f(j3);
         }
-        if (s < c) f(l[(size_t)s * k.n]);
+        for (; s < c; ++s) f(l[(size_t)s * k.n]);
     } else {
         fg_for_stencil(fg_unhash(bhash[i], k.g), k.g, off, f, 2);
     }
@@ -518,6 +520,80 @@ __global__ __launch_bounds__(kTpb) void k_xsph(const float4* __restrict__ xa, co
     vout[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
 }
 
+// ------------------------------------------------------------------------------------------ heat
+// HeatTransfar::heatParticleAndParticle (HeatTransfar.cpp:429-473): over the neighbours within h at the
+// final positions, dT_i = td sum_{j != i} dt (T_j - T_i) / max(rho_j, 0.1) W_spline(r_ij, h), with the
+// cubic spline of HeatTransfar.cpp:200-236 (sigma = 10 / (7 pi h^2)).  therm = (T, heat, phase, changed).
+__global__ __launch_bounds__(kTpb) void k_heat_diffuse(const float4* __restrict__ x, const float* __restrict__ dens,
+                                                       const float4* __restrict__ therm, float* __restrict__ dtemp,
+                                                       const float* __restrict__ stats, PbfK k, const uint* __restrict__ list,
+                                                       const uint* __restrict__ cnt, const uint* __restrict__ bhash,
+                                                       const uint* __restrict__ off, const float4* __restrict__ xb, float td,
+                                                       float hdt) {
+    bool valid;
+    uint i = slot_of(k, valid);
+    if (!valid) return;
+    float disp2 = stats[ST_DISP2];
+    float3 p = f3(x[i]);
+    float ti = therm[i].x;
+    const float inv_h = k.inv_h, ws = 10.0f / (7.0f * 3.14159265358979f * k.s.h2);
+    float acc = 0.0f;
+    for_neighbors(i, 0, k, list, cnt, bhash, off, disp2, p, xb, [&](uint j) {
+        float3 r = p - f3(x[j]);
+        float r2 = dot3(r, r);
+        if (r2 <= k.s.h2 && j != i) {
+            float q = sqrtf(r2) * inv_h;
+            float w = q < 1.0f ? ws * (1.0f - 1.5f * q * q + 0.75f * q * q * q) : ws * 0.25f * (2.0f - q) * (2.0f - q) * (2.0f - q);
+            float d = fmaxf(dens[j], 0.1f);
+            acc += hdt * (therm[j].x - ti) / d * w;
+        }
+    });
+    dtemp[i] = acc * td;
+}
+
+// HeatTransfar::calcTempAndHeat (HeatTransfar.cpp:280-350): latent / sensible heat and the phase state
+// (-2 ice, -1 melting, 1 freezing, 2 water; 250 is the melting point of the reference's temperature scale).
+__global__ void k_heat_phase(float4* __restrict__ therm, const float* __restrict__ dtemp, uint n) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float latent = 165.0f;
+    float4 th = therm[i];
+    float t = th.x, q = th.y, dT = dtemp[i];
+    int ph = (int)th.z;
+    float chg = th.w;
+    if (ph == -2 && t > 250.0f) {
+        ph = -1; t = 250.0f; q = 0.0f;
+    } else if (ph == 2 && t < 250.0f) {
+        ph = 1; t = 250.0f; q = latent;
+    }
+    if (ph == -1 || ph == 1) {
+        float c = 2.1f + (2.1f * q / latent);
+        q += dT * c * 1.0f;
+        if (ph == -1 && q < -latent / 300.0f) {
+            ph = -2; t = 249.0f; q = 0.0f;
+        } else if (ph == 1 && q > latent) {
+            ph = 2; t = 251.0f; q = 0.0f;
+        }
+        if (ph == -1 && q > latent) {
+            ph = 2; t = 251.0f; q = 0.0f; chg = 1.0f;
+        } else if (ph == 1 && q < 0.0f) {
+            ph = -2; t = 249.0f; q = 0.0f; chg = 1.0f;
+        }
+    } else {
+        float c = (t > 250.0f) ? 4.2f : 2.1f;
+        t = t + q / (c * 1.0f);
+        q = 0.0f;
+        t += dT;
+        t = fminf(fmaxf(t, 0.0f), 500.0f);
+    }
+    therm[i] = make_float4(t, q, (float)ph, chg);
+}
+
+__global__ void k_gather_by_id(float4* __restrict__ dst, const float4* __restrict__ src, const uint* __restrict__ id, uint n) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[id[i]];
+}
+
 __global__ void k_scatter_out(const float4* __restrict__ src, const uint* __restrict__ id, float4* __restrict__ dst, uint n) {
     uint i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[id[i]] = src[i];
@@ -547,7 +623,7 @@ void rxPBDSPH_GPU::Finalize() {
     if (m_stream) RX_HIP(hipStreamSynchronize(m_stream));
     for (void** p : {(void**)&m_x, (void**)&m_xs, (void**)&m_xl, (void**)&m_xb, (void**)&m_v, (void**)&m_t, (void**)&m_dens,
                      (void**)&m_lamacc, (void**)&m_id, (void**)&m_idt, (void**)&m_list, (void**)&m_cnt, (void**)&m_stats,
-                     (void**)&m_ftmp, (void**)&m_feat, (void**)&m_ftmp16})
+                     (void**)&m_ftmp, (void**)&m_feat, (void**)&m_ftmp16, (void**)&m_therm, (void**)&m_therm_t})
         if (*p) {
             RX_HIP(hipFree(*p));
             *p = nullptr;
@@ -599,7 +675,7 @@ int rxPBDSPH_GPU::SetParticles(const float* pos4, const float* vel4, int n) {
     if ((uint)n != m_n) {
         for (void** p : {(void**)&m_x, (void**)&m_xs, (void**)&m_xl, (void**)&m_xb, (void**)&m_v, (void**)&m_t, (void**)&m_dens,
                          (void**)&m_lamacc, (void**)&m_id, (void**)&m_idt, (void**)&m_list, (void**)&m_cnt, (void**)&m_stats,
-                         (void**)&m_ftmp, (void**)&m_feat, (void**)&m_ftmp16})
+                         (void**)&m_ftmp, (void**)&m_feat, (void**)&m_ftmp16, (void**)&m_therm, (void**)&m_therm_t})
             if (*p) {
                 RX_HIP(hipFree(*p));
                 *p = nullptr;
@@ -707,6 +783,10 @@ int rxPBDSPH_GPU::Update(float dt, int step) {
         std::swap(m_id, m_idt);
         fg_gather<float>(m_ftmp, m_lamacc, m_grid.perm, n, s);
         std::swap(m_lamacc, m_ftmp);
+        if (m_therm) {
+            fg_gather<float4>(m_therm_t, m_therm, m_grid.perm, n, s);
+            std::swap(m_therm, m_therm_t);
+        }
         mode = 1;
     }
     int mk = begin_mark(RX_PROF_PREDICT);
@@ -723,6 +803,10 @@ int rxPBDSPH_GPU::Update(float dt, int step) {
     reorder(m_v, s);
     fg_gather<uint>(m_idt, m_id, m_grid.perm, n, s);
     std::swap(m_id, m_idt);
+    if (m_therm) {
+        fg_gather<float4>(m_therm_t, m_therm, m_grid.perm, n, s);
+        std::swap(m_therm, m_therm_t);
+    }
     rx_fill<float>(m_lamacc, 0.0f, n, s);
     end_mark(mk);
 
@@ -779,6 +863,13 @@ int rxPBDSPH_GPU::Update(float dt, int step) {
         std::swap(m_v, m_t);
     }
     std::swap(m_x, m_xs);
+    if (m_therm) {  // heat exchange at the final positions
+        mk = begin_mark(RX_PROF_HEAT);
+        k_heat_diffuse<<<nb, kTpb, 0, s>>>(m_x, m_dens, m_therm, m_ftmp, m_stats, k, m_list, m_cnt, m_grid.hash, m_grid.off, m_xb,
+                                           m_heat_td, m_heat_dt > 0.0f ? m_heat_dt : dt);
+        k_heat_phase<<<rx_blocks(n, 256), 256, 0, s>>>(m_therm, m_ftmp, n);
+        end_mark(mk);
+    }
     m_have_list = true;
     ++m_step;
     if (!RX_HIP(hipGetLastError())) return 1;
@@ -811,6 +902,20 @@ int rxPBDSPH_GPU::GetArray(int type, float* host, int count) {
         k_scatter_out1<<<rx_blocks(n, 256), 256, 0, s>>>(m_lamacc, m_id, (float*)tmp, n);
         RX_HIP(hipMemcpyAsync(host, tmp, sizeof(float) * n, hipMemcpyDeviceToHost, s));
         break;
+    case RX_ARRAY_TEMPERATURE:
+    case RX_ARRAY_HEAT:
+    case RX_ARRAY_PHASE: {
+        if (!m_therm) {
+            rx_set_error(1, "rxPBDSPH_GPU::GetArray: no thermal state (rxPBDSPH_GPU_SetThermal)");
+            return 1;
+        }
+        k_scatter_out<<<rx_blocks(n, 256), 256, 0, s>>>(m_therm, m_id, tmp, n);
+        std::vector<float4> h(n);
+        RX_HIP(hipMemcpyAsync(h.data(), tmp, sizeof(float4) * n, hipMemcpyDeviceToHost, s));
+        if (!RX_HIP(hipStreamSynchronize(s))) return 1;
+        for (uint t = 0; t < n; ++t) host[t] = type == RX_ARRAY_TEMPERATURE ? h[t].x : (type == RX_ARRAY_HEAT ? h[t].y : h[t].z);
+        return 0;
+    }
     case RX_ARRAY_FEATURES:
         if (count < (int)(16 * n)) {
             rx_set_error(1, "rxPBDSPH_GPU::GetArray: features need 16 floats per particle");
@@ -824,6 +929,38 @@ int rxPBDSPH_GPU::GetArray(int type, float* host, int count) {
         return 1;
     }
     return RX_HIP(hipStreamSynchronize(s)) ? 0 : 1;
+}
+
+// Thermal state in original particle order: temperature (reference scale, melting at 250), heat, phase (-2 ice,
+// -1 melting, 1 freezing, 2 water; NULL: -2 below 250, 2 otherwise).  NULL temps removes the thermal state.
+int rxPBDSPH_GPU::SetThermal(const float* temps, const float* heats, const int* phase) {
+    if (!m_n) {
+        rx_set_error(1, "rxPBDSPH_GPU::SetThermal: set the particles first");
+        return 1;
+    }
+    hipStream_t s = m_stream;
+    RX_HIP(hipStreamSynchronize(s));
+    if (!temps) {
+        for (void** p : {(void**)&m_therm, (void**)&m_therm_t})
+            if (*p) {
+                RX_HIP(hipFree(*p));
+                *p = nullptr;
+            }
+        return 0;
+    }
+    if (!m_therm && !(RX_HIP(hipMalloc(&m_therm, sizeof(float4) * m_n)) && RX_HIP(hipMalloc(&m_therm_t, sizeof(float4) * m_n)))) return 1;
+    std::vector<float4> h(m_n);
+    for (uint i = 0; i < m_n; ++i) {
+        int ph = phase ? phase[i] : (temps[i] < 250.0f ? -2 : 2);
+        h[i] = make_float4(temps[i], heats ? heats[i] : 0.0f, (float)ph, 0.0f);
+    }
+    RX_HIP(hipMemcpyAsync(m_t, h.data(), sizeof(float4) * m_n, hipMemcpyHostToDevice, s));
+    k_gather_by_id<<<rx_blocks(m_n, 256), 256, 0, s>>>(m_therm, m_t, m_id, m_n);  // into the current slot order
+    return RX_HIP(hipStreamSynchronize(s)) ? 0 : 1;
+}
+
+rxPBDSPH_GPU::DeviceView rxPBDSPH_GPU::device_view() const {
+    return DeviceView{m_x, m_v, m_id, m_xb, m_therm, m_stats, ST_DISP2};
 }
 
 int rxPBDSPH_GPU::GetStats(float* out, int count) {
@@ -931,6 +1068,10 @@ int rxPBDSPH_GPU_Sync(void* h) { return h ? ((rxPBDSPH_GPU*)h)->Sync() : 1; }
 int rxPBDSPH_GPU_SetOption(void* h, int opt, float value) { return h ? ((rxPBDSPH_GPU*)h)->SetOption(opt, value) : 1; }
 int rxPBDSPH_GPU_GetProfile(void* h, float* ms, int* count, int n) { return h ? ((rxPBDSPH_GPU*)h)->GetProfile(ms, count, n) : 1; }
 int rxPBDSPH_GPU_SetNetwork(void* h, void* mlp) { return h ? ((rxPBDSPH_GPU*)h)->SetNetwork(mlp) : 1; }
+int rxPBDSPH_GPU_SetThermal(void* h, const float* temps, const float* heats, const int* phase) {
+    if (!h) return 1;
+    return ((rxPBDSPH_GPU*)h)->SetThermal(temps, heats, phase);
+}
 int rxPBDSPH_GPU_FeatureScales(void* h, float* sd_ref, float* nref) {
     if (!h) return 1;
     ((rxPBDSPH_GPU*)h)->FeatureScales(sd_ref, nref);
@@ -971,6 +1112,8 @@ int rxPBDSPH_GPU::SetOption(int opt, float value) {
     case RX_PBF_OPT_PROFILE_MASK:
         m_profile_mask = (unsigned)value;
         return 0;
+    case RX_PBF_OPT_HEAT_TD: m_heat_td = value; return 0;
+    case RX_PBF_OPT_HEAT_DT: m_heat_dt = value; return 0;
     case RX_PBF_OPT_LIST_TILE:
         if (value < 0.0f || value > 3.0f) {
             rx_set_error(1, "rxPBDSPH_GPU::SetOption: list tile variant must be 0..3");

@@ -28,6 +28,7 @@ struct SmK {
     float3 lo, hi;
     int volcon;
     uint n;
+    int masked;  // a melt mask is installed (active[] decides which particles and clusters take part)
 };
 
 // ------------------------------------------------------------------------------------------ 3x3 helpers
@@ -116,23 +117,47 @@ __global__ void k_sm_predict(const float4* __restrict__ x, const float4* __restr
     xs[i] = make_float4(c[0], c[1], c[2], 0.0f);
 }
 
-template <int T>
+// 3x3 inverse (cofactors / determinant); false when the determinant is zero.
+__device__ inline bool inv3f(const float a[9], float o[9]) {
+    float d = det3(a);
+    if (d == 0.0f) return false;
+    float id = 1.0f / d;
+    o[0] = (a[4] * a[8] - a[5] * a[7]) * id;
+    o[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+    o[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+    o[3] = (a[5] * a[6] - a[3] * a[8]) * id;
+    o[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+    o[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+    o[6] = (a[3] * a[7] - a[4] * a[6]) * id;
+    o[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+    o[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+    return true;
+}
+
+// One cluster per sub-group of T lanes.  Masked: only active members count; when some members are inactive the
+// rest centre of mass and Aqq^-1 are those of the active members (from the same sums), and below 4 active
+// members the linear term is dropped.
+template <int T, bool Masked>
 __device__ inline void sm_cluster(int c, bool valid, const int* __restrict__ cs, const int* __restrict__ mem, const float4* __restrict__ xs,
-                                  const float4* __restrict__ rest, const float4* __restrict__ pre, float4* __restrict__ tr, const SmK& k) {
+                                  const float4* __restrict__ rest, const float4* __restrict__ pre, float4* __restrict__ tr,
+                                  const int* __restrict__ act, const SmK& k) {
     const int sub = threadIdx.x & (T - 1);
+    constexpr int NA = Masked ? 22 : 15;
     int s = 0, e = 0;
     float3 xref = f3(0, 0, 0), cm0 = f3(0, 0, 0);
+    if (valid && Masked && !act[c]) valid = false;
     if (valid) {
         s = cs[c];
         e = cs[c + 1];
         xref = f3(xs[c]);
         cm0 = f3(pre[3 * c]);
     }
-    float acc[15];
+    float acc[NA];
 #pragma unroll
-    for (int t = 0; t < 15; ++t) acc[t] = 0.0f;
+    for (int t = 0; t < NA; ++t) acc[t] = 0.0f;
     for (int t = s + sub; t < e; t += T) {
         int m = mem[t];
+        if (Masked && !act[m]) continue;
         float3 d = f3(xs[m]) - xref;
         float3 q = f3(rest[m]) - cm0;
         acc[0] += d.x; acc[1] += d.y; acc[2] += d.z;
@@ -140,13 +165,19 @@ __device__ inline void sm_cluster(int c, bool valid, const int* __restrict__ cs,
         acc[6] += d.x * q.x; acc[7] += d.x * q.y; acc[8] += d.x * q.z;
         acc[9] += d.y * q.x; acc[10] += d.y * q.y; acc[11] += d.y * q.z;
         acc[12] += d.z * q.x; acc[13] += d.z * q.y; acc[14] += d.z * q.z;
+        if (Masked) {
+            acc[15] += 1.0f;
+            acc[16 % NA] += q.x * q.x; acc[17 % NA] += q.x * q.y; acc[18 % NA] += q.x * q.z;
+            acc[19 % NA] += q.y * q.y; acc[20 % NA] += q.y * q.z; acc[21 % NA] += q.z * q.z;
+        }
     }
 #pragma unroll
     for (int off = T / 2; off > 0; off >>= 1)
 #pragma unroll
-        for (int t = 0; t < 15; ++t) acc[t] += __shfl_xor(acc[t], off, 64);
+        for (int t = 0; t < NA; ++t) acc[t] += __shfl_xor(acc[t], off, 64);
     if (!valid) return;
-    int kk = e - s;
+    const int kall = e - s;
+    const int kk = Masked ? (int)acc[15 % NA] : kall;
     float RL[9];
     float3 cm;
     if (kk <= 1) {  // rxShapeMatching::shapeMatching returns early (ShapeMatching.cpp:494): goal = x*
@@ -158,7 +189,7 @@ __device__ inline void sm_cluster(int c, bool valid, const int* __restrict__ cs,
         float ik = 1.0f / (float)kk;
         float3 md = f3(acc[0] * ik, acc[1] * ik, acc[2] * ik);
         cm = xref + md;
-        // Apq = sum (x - cm) q^T = sum d q^T - md (sum q)^T
+        // Apq = sum (x - cm) q^T = sum d q^T - md (sum q)^T  (independent of the rest origin of q)
         float apq[9];
         float sq[3] = {acc[3], acc[4], acc[5]}, mdv[3] = {md.x, md.y, md.z};
 #pragma unroll
@@ -167,9 +198,33 @@ __device__ inline void sm_cluster(int c, bool valid, const int* __restrict__ cs,
             for (int cc = 0; cc < 3; ++cc) apq[r * 3 + cc] = acc[6 + r * 3 + cc] - mdv[r] * sq[cc];
         float R[9];
         polar3(apq, R);
-        if (k.beta != 0.0f) {
+        bool linear = k.beta != 0.0f;
+        float ai[9];
+        if (linear) {
             float4 p1 = pre[3 * c + 1], p2 = pre[3 * c + 2];
-            float ai[9] = {pre[3 * c].w, p1.x, p1.y, p1.z, p1.w, p2.x, p2.y, p2.z, p2.w};
+            ai[0] = pre[3 * c].w; ai[1] = p1.x; ai[2] = p1.y; ai[3] = p1.z; ai[4] = p1.w;
+            ai[5] = p2.x; ai[6] = p2.y; ai[7] = p2.z; ai[8] = p2.w;
+        }
+        if (Masked && kk < kall) {
+            // rest centre of mass of the active members and their Aqq = sum q q^T - k delta delta^T
+            float3 dl = f3(sq[0] * ik, sq[1] * ik, sq[2] * ik);
+            cm0 = cm0 + dl;
+            if (kk < 4) {
+                linear = false;
+            } else if (linear) {
+                const float kf = (float)kk;
+                float aq[9] = {acc[16 % NA] - kf * dl.x * dl.x, acc[17 % NA] - kf * dl.x * dl.y, acc[18 % NA] - kf * dl.x * dl.z,
+                               0.0f, acc[19 % NA] - kf * dl.y * dl.y, acc[20 % NA] - kf * dl.y * dl.z,
+                               0.0f, 0.0f, acc[21 % NA] - kf * dl.z * dl.z};
+                aq[3] = aq[1];
+                aq[6] = aq[2];
+                aq[7] = aq[5];
+                if (!inv3f(aq, ai))
+#pragma unroll
+                    for (int t = 0; t < 9; ++t) ai[t] = 0.0f;
+            }
+        }
+        if (linear) {
             float A[9];
 #pragma unroll
             for (int r = 0; r < 3; ++r)
@@ -200,12 +255,21 @@ __device__ inline void sm_cluster(int c, bool valid, const int* __restrict__ cs,
     }
 }
 
+template <int T>
+__device__ inline void sm_cluster_sel(int c, bool valid, const int* __restrict__ cs, const int* __restrict__ mem, const float4* __restrict__ xs,
+                                      const float4* __restrict__ rest, const float4* __restrict__ pre, float4* __restrict__ tr,
+                                      const int* __restrict__ act, const SmK& k) {
+    if (k.masked) sm_cluster<T, true>(c, valid, cs, mem, xs, rest, pre, tr, act, k);
+    else sm_cluster<T, false>(c, valid, cs, mem, xs, rest, pre, tr, act, k);
+}
+
 // Persistent cluster pass: waves loop over work units (bin, first index into the bin-ordered cluster list).
 __global__ __launch_bounds__(256) void k_sm_clusters(const int2* __restrict__ units, int nunits, const int* __restrict__ tiles,
                                                      const int* __restrict__ bin_end, const int* __restrict__ corder,
                                                      const int* __restrict__ cs, const int* __restrict__ mem,
                                                      const float4* __restrict__ xs, const float4* __restrict__ rest,
-                                                     const float4* __restrict__ pre, float4* __restrict__ tr, SmK k) {
+                                                     const float4* __restrict__ pre, float4* __restrict__ tr,
+                                                     const int* __restrict__ act, SmK k) {
     const int wpb = blockDim.x >> 6;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int u = blockIdx.x * wpb + wave; u < nunits; u += gridDim.x * wpb) {
@@ -215,30 +279,39 @@ __global__ __launch_bounds__(256) void k_sm_clusters(const int2* __restrict__ un
         bool valid = idx < bin_end[un.x];
         int c = valid ? corder[idx] : 0;
         switch (T) {
-        case 8: sm_cluster<8>(c, valid, cs, mem, xs, rest, pre, tr, k); break;
-        case 16: sm_cluster<16>(c, valid, cs, mem, xs, rest, pre, tr, k); break;
-        case 32: sm_cluster<32>(c, valid, cs, mem, xs, rest, pre, tr, k); break;
-        default: sm_cluster<64>(c, valid, cs, mem, xs, rest, pre, tr, k); break;
+        case 8: sm_cluster_sel<8>(c, valid, cs, mem, xs, rest, pre, tr, act, k); break;
+        case 16: sm_cluster_sel<16>(c, valid, cs, mem, xs, rest, pre, tr, act, k); break;
+        case 32: sm_cluster_sel<32>(c, valid, cs, mem, xs, rest, pre, tr, act, k); break;
+        default: sm_cluster_sel<64>(c, valid, cs, mem, xs, rest, pre, tr, act, k); break;
         }
     }
 }
 
 // ConvoNormal::StepConvolution (Ice_Convolution_Normal.cpp:32-67) over the clusters containing each particle:
 // x_p = x*_p + alpha (mean_c goal_c(p) - x*_p).
+// With a melt mask, inactive particles keep x* and only active clusters count.
 __global__ void k_sm_average(const int* __restrict__ ps, const int* __restrict__ pc, const float4* __restrict__ tr,
-                             const float4* __restrict__ rest, const float4* __restrict__ xs, float4* __restrict__ xo, SmK k) {
+                             const float4* __restrict__ rest, const float4* __restrict__ xs, float4* __restrict__ xo,
+                             const int* __restrict__ act, SmK k) {
     uint i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= k.n) return;
     int s = ps[i], e = ps[i + 1];
+    float3 x = f3(xs[i]);
+    if (k.masked && !act[i]) {
+        xo[i] = make_float4(x.x, x.y, x.z, 0.0f);
+        return;
+    }
     float3 r = f3(rest[i]);
     float3 acc = f3(0, 0, 0);
+    int cnt = 0;
     for (int t = s; t < e; ++t) {
         int c = pc[t];
+        if (k.masked && !act[c]) continue;
         float4 a = tr[3 * c], b = tr[3 * c + 1], d = tr[3 * c + 2];
         acc += f3(a.x * r.x + a.y * r.y + a.z * r.z + a.w, b.x * r.x + b.y * r.y + b.z * r.z + b.w, d.x * r.x + d.y * r.y + d.z * r.z + d.w);
+        ++cnt;
     }
-    float3 x = f3(xs[i]);
-    if (e > s) x = x + k.alpha * ((1.0f / (float)(e - s)) * acc - x);
+    if (cnt > 0) x = x + k.alpha * ((1.0f / (float)cnt) * acc - x);
     xo[i] = make_float4(x.x, x.y, x.z, 0.0f);
 }
 
@@ -278,7 +351,7 @@ public:
         drop_graph();
         for (void** p : {(void**)&d_x, (void**)&d_v, (void**)&d_xa, (void**)&d_xb, (void**)&d_rest, (void**)&d_pre, (void**)&d_tr,
                          (void**)&d_cs, (void**)&d_mem, (void**)&d_ps, (void**)&d_pc, (void**)&d_units, (void**)&d_tiles,
-                         (void**)&d_binend, (void**)&d_corder})
+                         (void**)&d_binend, (void**)&d_corder, (void**)&d_act})
             if (*p) {
                 RX_HIP(hipFree(*p));
                 *p = nullptr;
@@ -400,7 +473,8 @@ public:
                   RX_HIP(hipMalloc(&d_mem, sizeof(int) * std::max<size_t>(h_mem.size(), 1))) &&
                   RX_HIP(hipMalloc(&d_ps, sizeof(int) * (m_n + 1))) && RX_HIP(hipMalloc(&d_pc, sizeof(int) * std::max<size_t>(h_pc.size(), 1))) &&
                   RX_HIP(hipMalloc(&d_units, sizeof(int2) * (m_n + kNumBins))) && RX_HIP(hipMalloc(&d_tiles, sizeof(int) * kNumBins)) &&
-                  RX_HIP(hipMalloc(&d_binend, sizeof(int) * kNumBins)) && RX_HIP(hipMalloc(&d_corder, sizeof(int) * m_n));
+                  RX_HIP(hipMalloc(&d_binend, sizeof(int) * kNumBins)) && RX_HIP(hipMalloc(&d_corder, sizeof(int) * m_n)) &&
+                  RX_HIP(hipMalloc(&d_act, sizeof(int) * m_n));
         if (!ok) return 1;
         hipStream_t s = m_stream;
         std::vector<float4> pre(3 * (size_t)m_n);
@@ -420,8 +494,33 @@ public:
         RX_HIP(hipMemcpyAsync(d_pc, h_pc.data(), sizeof(int) * h_pc.size(), hipMemcpyHostToDevice, s));
         RX_HIP(hipMemcpyAsync(d_corder, h_corder.data(), sizeof(int) * m_n, hipMemcpyHostToDevice, s));
         RX_HIP(hipMemcpyAsync(d_binend, h_binstart.data() + 1, sizeof(int) * kNumBins, hipMemcpyHostToDevice, s));
+        rx_fill<int>(d_act, 1, m_n, s);
         RX_HIP(hipStreamSynchronize(s));
         return push_tiles();
+    }
+
+    // Melt mask from the host (1 = ice); NULL removes the mask.
+    int set_active(const int* host) {
+        if (host) {
+            RX_HIP(hipMemcpyAsync(d_act, host, sizeof(int) * m_n, hipMemcpyHostToDevice, m_stream));
+        } else {
+            rx_fill<int>(d_act, 1, m_n, m_stream);
+        }
+        m_masked = host != nullptr;
+        drop_graph();  // the step's launch parameters changed
+        return RX_HIP(hipStreamSynchronize(m_stream)) ? 0 : 1;
+    }
+    // Device view for a caller that drives the object from its own kernels (the coupled solver): state arrays
+    // in ice-particle order, the mask, the stream.  Turns the mask on.
+    void device_view(float4** x, float4** v, int** act, hipStream_t* s) {
+        if (!m_masked) {
+            m_masked = true;
+            drop_graph();
+        }
+        *x = d_x;
+        *v = d_v;
+        *act = d_act;
+        *s = m_stream;
     }
 
     int push_tiles() {
@@ -446,6 +545,7 @@ public:
         k.hi = f3(m_env.box_max[0], m_env.box_max[1], m_env.box_max[2]);
         k.volcon = m_env.volume_conservation;
         k.n = m_n;
+        k.masked = m_masked ? 1 : 0;
         return k;
     }
 
@@ -458,7 +558,7 @@ public:
     void launch_clusters(hipStream_t s, const int2* units, int nunits, const float4* xs) {
         SmK k = consts();
         int blocks = std::max(1, std::min((nunits + 3) / 4, m_max_blocks));
-        k_sm_clusters<<<blocks, 256, 0, s>>>(units, nunits, d_tiles, d_binend, d_corder, d_cs, d_mem, xs, d_rest, d_pre, d_tr, k);
+        k_sm_clusters<<<blocks, 256, 0, s>>>(units, nunits, d_tiles, d_binend, d_corder, d_cs, d_mem, xs, d_rest, d_pre, d_tr, d_act, k);
     }
 
     // One step as a fixed launch sequence (no pointer swaps: graph capturable).
@@ -473,7 +573,7 @@ public:
             if (timed) RX_HIP(hipEventRecord(m_ev[0], s));
             launch_clusters(s, d_units, m_nunits, cur);
             if (timed) RX_HIP(hipEventRecord(m_ev[1], s));
-            k_sm_average<<<nb, 256, 0, s>>>(d_ps, d_pc, d_tr, d_rest, cur, nxt, k);
+            k_sm_average<<<nb, 256, 0, s>>>(d_ps, d_pc, d_tr, d_rest, cur, nxt, d_act, k);
             if (timed) RX_HIP(hipEventRecord(m_ev[2], s));
             std::swap(cur, nxt);
             if (timed) {
@@ -610,6 +710,8 @@ public:
     std::vector<int> h_cs, h_mem, h_ps, h_pc, h_corder, h_binstart;
     std::vector<float> h_pre;
     float4 *d_x = nullptr, *d_v = nullptr, *d_xa = nullptr, *d_xb = nullptr, *d_rest = nullptr, *d_pre = nullptr, *d_tr = nullptr;
+    int* d_act = nullptr;  // melt mask: 1 = the particle is ice
+    bool m_masked = false;
     int *d_cs = nullptr, *d_mem = nullptr, *d_ps = nullptr, *d_pc = nullptr, *d_tiles = nullptr, *d_binend = nullptr, *d_corder = nullptr;
     int2* d_units = nullptr;
     std::vector<hipEvent_t> m_ev;
@@ -706,5 +808,13 @@ int rxIceSM_UseGraph(void* h, int on) {
     return 0;
 }
 int rxIceSM_KernelTimes(void* h, float* cl, float* av) { return h ? ((rxIceSM*)h)->kernel_times(cl, av) : 1; }
+int rxIceSM_SetActive(void* h, const int* active) { return h ? ((rxIceSM*)h)->set_active(active) : 1; }
 
 }  // extern "C"
+
+// Internal (not part of the C ABI): device view of an rxIceSM for the coupled solver.
+bool rx_ice_device_view(void* h, float4** x, float4** v, int** act, hipStream_t* s) {
+    if (!h) return false;
+    ((rxIceSM*)h)->device_view(x, v, act, s);
+    return true;
+}

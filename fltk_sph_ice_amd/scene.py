@@ -187,3 +187,63 @@ def pbf_splash(nx: int = 48, ny: int = 32, nz: int = 48, seed: int = 6, box: tup
     sc.block_origin = (0.5 * (box[0] - nx * s), box[1] - ny * s - 0.1, 0.5 * (box[2] - nz * s))
     sc.extra["v0"] = (0.0, -2.0, 0.0)
     return sc
+
+
+@dataclass
+class CoupledScene:
+    """Fluid + melting ice: a PBF fluid block and an ice block (shape-matching clusters) in one box, with a
+    temperature per particle.  Temperatures use the reference's scale (melting point 250 = 0 deg C):
+    the ice starts at 250 (0 C), the fluid at 270 +- U(1) (20 C +- 1)."""
+
+    name: str
+    fluid: Scene             # PBF parameters, box and the fluid block
+    ice_block: tuple         # (nx, ny, nz) at the fluid spacing
+    ice_origin: tuple
+    seed_ice: int
+    seed_temp: int
+    alpha: float = 0.9
+    beta: float = 0.3
+    sm_iterations: int = 2
+    radius_spacings: float = 2.5
+    heat_td: float = 40.0
+
+    @property
+    def n_fluid(self) -> int:
+        return self.fluid.n
+
+    @property
+    def n_ice(self) -> int:
+        return int(np.prod(self.ice_block))
+
+    @property
+    def n(self) -> int:
+        return self.n_fluid + self.n_ice
+
+    def particles(self):
+        """(pos4, vel4, temps, phase): fluid first, then ice."""
+        lib = load()
+        fp, fv = self.fluid.particles()
+        ni = self.n_ice
+        ip = np.zeros((ni, 4), np.float32)
+        org = (C.c_float * 3)(*self.ice_origin)
+        s = self.fluid.spacing
+        lib.rxSceneBox(ip.ctypes.data, None, ni, org, *self.ice_block, s, 0.01 * s, 0.0, self.seed_ice)
+        pos = np.concatenate([fp, ip]).astype(np.float32)
+        vel = np.concatenate([fv, np.zeros_like(ip)]).astype(np.float32)
+        rng = np.random.default_rng(self.seed_temp)
+        temps = np.concatenate([270.0 + rng.uniform(-1.0, 1.0, self.n_fluid), np.full(ni, 250.0)]).astype(np.float32)
+        phase = np.concatenate([np.full(self.n_fluid, 2), np.full(ni, -2)]).astype(np.int32)
+        return pos, vel, temps, phase
+
+
+def coupled_melt(fluid=(128, 128, 96), ice=(64, 128, 64), seed: int = 9) -> CoupledScene:
+    """Fluid + melting ice: 128 x 128 x 96 = 1,572,864 fluid particles (dam break at spacing h/2, h = 0.1,
+    plain-PBF parameters) beside a 64 x 128 x 64 = 524,288-particle ice block, 2,097,152 in all, in a
+    16 x 8 x 8 box.  Clusters from the ice neighbours within 2.5 spacings (uneven sizes up to 81 at the
+    block faces and corners), alpha 0.9, beta 0.3, 2 shape-matching iterations."""
+    s = 0.05
+    box = (2.5 * fluid[0] * s, 1.25 * max(fluid[1], ice[1]) * s, 1.5 * max(fluid[2], ice[2]) * s)
+    fl = pbf_dam_break(*fluid, seed=seed, box=box)
+    fl.name = "coupled_fluid"
+    ice_origin = (fluid[0] * s + 0.3 * box[0], 0.0 + s, 0.5 * (box[2] - ice[2] * s))
+    return CoupledScene("coupled_melt", fl, tuple(ice), ice_origin, seed + 1, seed + 2)

@@ -201,6 +201,12 @@ class IceSM:
         if self.lib.rxIceSM_SetTiles(self.h, t.ctypes.data):
             check(self.lib)
 
+    def set_active(self, active=None):
+        """Melt mask (1 = ice) or None to remove it."""
+        a = None if active is None else np.ascontiguousarray(active, np.int32)
+        if self.lib.rxIceSM_SetActive(self.h, None if a is None else a.ctypes.data):
+            check(self.lib)
+
     def kernel_times(self):
         a, b = C.c_float(), C.c_float()
         if self.lib.rxIceSM_KernelTimes(self.h, C.byref(a), C.byref(b)):
@@ -210,6 +216,89 @@ class IceSM:
     def close(self):
         if self.h:
             self.lib.rxIceSM_Destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Coupled:
+    """rxCoupled: fluid + melting ice (include/rx_sph_host.h) for a CoupledScene."""
+
+    def __init__(self, cs, data=None):
+        self.lib = load()
+        self.scene = cs
+        pos, vel, temps, phase = cs.particles() if data is None else data
+        env = cs.fluid.env()
+        env.max_particles = cs.n
+        ice = _abi.rxIceEnv()
+        self.lib.rxIceEnvDefault(C.byref(ice))
+        sp = cs.fluid.params()
+        ice.alpha, ice.beta, ice.dt = cs.alpha, cs.beta, cs.fluid.dt
+        for k in range(3):
+            ice.gravity[k] = cs.fluid.gravity[k]
+            ice.box_min[k] = sp.BoundaryMin[k] + sp.ParticleRadius
+            ice.box_max[k] = sp.BoundaryMax[k] - sp.ParticleRadius
+        ice.iterations, ice.volume_conservation, ice.restitution = cs.sm_iterations, 1, 0.9
+        ice.cluster_radius = cs.radius_spacings * cs.fluid.spacing
+        self.ice_env = ice
+        pos = np.ascontiguousarray(pos, np.float32)
+        vel = np.ascontiguousarray(vel, np.float32)
+        temps = np.ascontiguousarray(temps, np.float32)
+        phase = np.ascontiguousarray(phase, np.int32)
+        self.h = self.lib.rxCoupled_Create(C.byref(env), C.byref(ice), pos.ctypes.data, vel.ctypes.data, cs.n_fluid, cs.n_ice,
+                                           temps.ctypes.data, phase.ctypes.data)
+        if not self.h:
+            check(self.lib)
+            raise RuntimeError("rxCoupled_Create failed")
+        self.n = cs.n
+        self.option(_abi.RX_PBF_OPT_HEAT_TD, cs.heat_td)
+
+    def option(self, opt: int, value: float):
+        if self.lib.rxCoupled_SetOption(self.h, opt, value):
+            check(self.lib)
+
+    def autotune(self):
+        t = np.zeros(5, np.int32)
+        if self.lib.rxCoupled_SetIceTiles(self.h, 1, t.ctypes.data):
+            check(self.lib)
+        return t
+
+    def step(self, steps: int = 1):
+        if self.lib.rxCoupled_Update(self.h, self.scene.fluid.dt, steps):
+            check(self.lib)
+
+    def sync(self):
+        if self.lib.rxCoupled_Sync(self.h):
+            check(self.lib)
+
+    def array(self, kind: int) -> np.ndarray:
+        vec = kind in (_abi.RX_ARRAY_POSITION, _abi.RX_ARRAY_VELOCITY)
+        out = np.zeros((self.n, 4) if vec else self.n, np.float32)
+        if self.lib.rxCoupled_GetArray(self.h, kind, out.ctypes.data, self.n):
+            check(self.lib)
+        return out
+
+    def profile(self) -> dict:
+        """PBF kernel classes: {class: (total ms, launches)} since the previous call (needs RX_PBF_OPT_PROFILE)."""
+        ms = np.zeros(16, np.float32)
+        cnt = np.zeros(16, np.int32)
+        if self.lib.rxCoupled_GetProfile(self.h, ms.ctypes.data, cnt.ctypes.data, 16):
+            check(self.lib)
+        return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(_abi.PROF_CLASSES) if cnt[i]}
+
+    def stats(self) -> np.ndarray:
+        out = np.zeros(64, np.float32)
+        if self.lib.rxCoupled_GetStats(self.h, out.ctypes.data, 64):
+            check(self.lib)
+        return out
+
+    def close(self):
+        if self.h:
+            self.lib.rxCoupled_Destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -60,6 +60,11 @@ int rxIceSM_GetClusters(void* sm, int* start, int* members);
 int rxIceSM_Autotune(void* sm, int* out_tiles);
 int rxIceSM_SetTiles(void* sm, const int* tiles);
 int rxIceSM_UseGraph(void* sm, int on);
+/* Melt mask (n ints, 1 = ice, host memory; NULL removes it): only clusters whose own particle is ice take
+ * part, over their ice members (rest centre of mass and Aqq of those members; no linear term below 4 of
+ * them); particles that are not ice keep their prediction.  No reference counterpart as a mask: it stands in
+ * for the cluster reconstruction of IceObject::StepMelting (IceObject.cpp:1177-1300). */
+int rxIceSM_SetActive(void* sm, const int* active);
 /* Average milliseconds of the cluster kernel and of the averaging kernel over the last Update (HIP events). */
 int rxIceSM_KernelTimes(void* sm, float* cluster_ms, float* average_ms);
 

@@ -66,7 +66,10 @@ enum rxArrayType {
     RX_ARRAY_DENSITY = 2,    /* float per particle  */
     RX_ARRAY_LAMBDA_SUM = 3, /* float per particle: sum of the multipliers the last step applied (PBF)  */
     RX_ARRAY_PRESSURE = 4,   /* float per particle (WCSPH) */
-    RX_ARRAY_FEATURES = 5    /* 16 floats per particle: warm-start features of the last step (PBF) */
+    RX_ARRAY_FEATURES = 5,   /* 16 floats per particle: warm-start features of the last step (PBF) */
+    RX_ARRAY_TEMPERATURE = 6,/* float per particle (PBF thermal state)                               */
+    RX_ARRAY_HEAT = 7,       /* float per particle: stored latent/sensible heat                      */
+    RX_ARRAY_PHASE = 8       /* float per particle: -2 ice, -1 melting, 1 freezing, 2 water           */
 };
 
 /* PBF solve modes (rxPBDSPH_GPU_SetOption(RX_PBF_OPT_MODE)). */
@@ -97,8 +100,10 @@ enum rxPbfOption {
     RX_PBF_OPT_RESET_RUN = 11,     /* clears the run accumulators (stats slots 48..63)                     */
     RX_PBF_OPT_PROFILE_MASK = 12,  /* bit c set: time kernel class c (rxProfileClass) while profiling;
                                       default all                                                          */
-    RX_PBF_OPT_LIST_TILE = 13      /* list pass: 0 = 16-cell segments, 48 KB LDS tile (default); 1 = 32 KB
+    RX_PBF_OPT_LIST_TILE = 13,     /* list pass: 0 = 16-cell segments, 48 KB LDS tile (default); 1 = 32 KB
                                       tile; 2 = 8-cell segments, 24 KB tile, 128 threads; 3 = no LDS tile  */
+    RX_PBF_OPT_HEAT_TD = 14,       /* thermal diffusion coefficient (HeatTransfar mTD; default 0)         */
+    RX_PBF_OPT_HEAT_DT = 15        /* heat time step (0: the step's dt)                                    */
 };
 
 /* Kernel classes of rxPBDSPH_GPU_GetProfile. */
@@ -111,7 +116,8 @@ enum rxProfileClass {
     RX_PROF_VELOCITY = 5,
     RX_PROF_XSPH = 6,
     RX_PROF_FEATURES = 7,     /* learned solve: feature pass               */
-    RX_PROF_MLP = 8           /* learned solve: network inference          */
+    RX_PROF_MLP = 8,          /* learned solve: network inference          */
+    RX_PROF_HEAT = 9          /* thermal state: diffusion + phase          */
 };
 
 /* Per-step statistics of rxPBDSPH_GPU_GetStats (float slots):
@@ -144,6 +150,12 @@ int rxPBDSPH_GPU_SetOption(void* solver, int option, float value);
 int rxPBDSPH_GPU_GetProfile(void* solver, float* ms, int* count, int n);
 /* Network of the learned warm start (an rxMLP object, see rx_mlp.h); its weights are copied. */
 int rxPBDSPH_GPU_SetNetwork(void* solver, void* mlp);
+/* Thermal state (host arrays in original order; heats and phase may be NULL; temps NULL removes it): after
+ * every step the temperatures diffuse over the neighbours within h at the final positions
+ * (HeatTransfar::heatParticleAndParticle, HeatTransfar.cpp:429-473: dt (T_j - T_i) / max(rho_j, 0.1) times
+ * the cubic spline of radius h, scaled by RX_PBF_OPT_HEAT_TD) and the phase state advances
+ * (HeatTransfar::calcTempAndHeat, HeatTransfar.cpp:280-350; melting point 250 on the reference's scale). */
+int rxPBDSPH_GPU_SetThermal(void* solver, const float* temps, const float* heats, const int* phase);
 /* Feature scales of the scene: lambda denominator and neighbour count of the rest lattice. */
 int rxPBDSPH_GPU_FeatureScales(void* solver, float* sd_ref, float* nref);
 
@@ -157,6 +169,26 @@ int rxSPH_GPU_Update(void* solver, float dt, int steps);
 int rxSPH_GPU_GetArray(void* solver, int type, float* host, int count);
 int rxSPH_GPU_Sync(void* solver);
 int rxSPH_GPU_UseGraph(void* solver, int on);
+
+/* ---------------------------------------------------------------- fluid + melting ice (no single reference
+ * counterpart: the ice-mode frame of rx_fltk_glcanvas.cpp:1134-1145 -- fluid step, ice object step, heat
+ * transfer, ice structure -- as one object).  Particles 0 .. n_fluid-1 are fluid, the next n_ice the ice body
+ * (rest shape = their initial positions, clusters from ice_env->cluster_radius).  temps/phase: thermal state
+ * as rxPBDSPH_GPU_SetThermal (temps required; melting point 250 on the reference's scale).  Per step: PBF step
+ * over all particles with heat exchange, then one shape-matching substep of the particles still frozen
+ * (phase < 0) written back over the fluid result (blend 1).  GetArray/GetStats/SetOption act on the PBF
+ * solver; SetIceTiles(autotune = 1) runs the shape-matching autotuner. */
+struct rxIceEnv;
+void* rxCoupled_Create(const rxSphEnv* fluid_env, const struct rxIceEnv* ice_env, const float* pos4, const float* vel4, int n_fluid,
+                       int n_ice, const float* temps, const int* phase);
+void rxCoupled_Destroy(void* coupled);
+int rxCoupled_Update(void* coupled, float dt, int steps);
+int rxCoupled_Sync(void* coupled);
+int rxCoupled_GetArray(void* coupled, int type, float* host, int count);
+int rxCoupled_GetStats(void* coupled, float* out, int count);
+int rxCoupled_SetOption(void* coupled, int option, float value);
+int rxCoupled_SetIceTiles(void* coupled, int autotune, int* out_tiles);
+int rxCoupled_GetProfile(void* coupled, float* ms, int* count, int n);
 
 #ifdef __cplusplus
 }

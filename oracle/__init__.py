@@ -68,8 +68,10 @@ _SIGS = {
     "orc_polar_decomposition": (None, [P, P, P]),
     "orc_sm_step_f": (None, [P, P, I, P, P, I, P, P]),
     "orc_sm_step_d": (None, [P, P, I, P, P, I, P, P]),
+    "orc_sm_step_masked_f": (None, [P, P, I, P, P, I, P, P, P]),
     "orc_heat_diffuse_f": (None, [P, P, P, I, P, F, F, P]),
     "orc_heat_transfer_gpu_f": (None, [P, P, P, P, P, P, P, P, P, F, F, F, F, F, I]),
+    "orc_heat_cpu_f": (None, [P, P, P, I, P, F, F]),
     "orc_detect_neighbors_f": (None, [P, I, P, P]),
     "orc_detect_surface_f": (None, [P, P, P, I, F, F]),
     "orc_mlp_forward_d": (None, [P, I, I, P, P, I, P, P, I, P, P, P]),

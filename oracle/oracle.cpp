@@ -529,7 +529,11 @@ bool inv3(const double* a, double* o) {
 //   (Ice_Convolution_Normal.cpp:32-67), repeated S.iterations times as Ice_CalcMethod_Iteration::StepObjMove
 //   (Ice_CalcMethod_Iteration.cpp:32-48), and the velocity update of OrientedParticle::UpdatePosAndVel
 //   (OrientedParticle.cpp:377-393).
-template <class T> void sm_step(T* pos, T* vel, int n, const int* cs, const int* mem, int nc, const T* rest, const OrcSmParams& S) {
+// With `active` (may be NULL) a cluster takes part only when its own particle is active, and only its active
+// members count: the rest centre of mass and Aqq are those of the active members; with fewer than 4 active
+// members (Aqq singular in 3-D) the linear term is dropped.  Inactive particles keep their prediction.
+template <class T> void sm_step(T* pos, T* vel, int n, const int* cs, const int* mem, int nc, const T* rest, const OrcSmParams& S,
+                                const int* active = nullptr) {
     double dt = S.dt;
     std::vector<V3> x(n), xs(n);
     for (int i = 0; i < n; ++i) {
@@ -567,30 +571,54 @@ template <class T> void sm_step(T* pos, T* vel, int n, const int* cs, const int*
     }
     std::vector<V3> acc(n);
     std::vector<int> cnt(n);
+    std::vector<int> am;  // active members of the current cluster
     for (int it = 0; it < S.iterations; ++it) {
         std::fill(acc.begin(), acc.end(), V3{0, 0, 0});
         std::fill(cnt.begin(), cnt.end(), 0);
         for (int c = 0; c < nc; ++c) {
-            int k = cs[c + 1] - cs[c];
+            if (active && !active[c]) continue;
+            am.clear();
+            for (int t = cs[c]; t < cs[c + 1]; ++t)
+                if (!active || active[mem[t]]) am.push_back(mem[t]);
+            int k = (int)am.size();
             if (k <= 1) {  // shapeMatching returns early for a single vertex (ShapeMatching.cpp:494)
-                for (int t = cs[c]; t < cs[c + 1]; ++t) { acc[mem[t]] = acc[mem[t]] + xs[mem[t]]; cnt[mem[t]]++; }
+                for (int p : am) { acc[p] = acc[p] + xs[p]; cnt[p]++; }
                 continue;
             }
+            const bool partial = k < cs[c + 1] - cs[c];
+            V3 c0 = cm0[c];
+            double aqqa[9];
+            const double* ai = &aqqi[9 * (size_t)c];
+            bool linear = S.beta != 0.0;
+            if (partial) {  // rest quantities of the active members
+                V3 s0{0, 0, 0};
+                for (int p : am) s0 = s0 + ld(rest, p);
+                c0 = (1.0 / k) * s0;
+                double aqq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                for (int p : am) {
+                    V3 q = ld(rest, p) - c0;
+                    double qq[3] = {q.x, q.y, q.z};
+                    for (int r = 0; r < 3; ++r)
+                        for (int cc = 0; cc < 3; ++cc) aqq[r * 3 + cc] += qq[r] * qq[cc];
+                }
+                if (!inv3(aqq, aqqa)) std::fill(aqqa, aqqa + 9, 0.0);
+                ai = aqqa;
+                if (k < 4) linear = false;
+            }
             V3 cm{0, 0, 0};
-            for (int t = cs[c]; t < cs[c + 1]; ++t) cm = cm + xs[mem[t]];
+            for (int p : am) cm = cm + xs[p];
             cm = (1.0 / k) * cm;
             double apq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-            for (int t = cs[c]; t < cs[c + 1]; ++t) {
-                V3 p = xs[mem[t]] - cm, q = ld(rest, mem[t]) - cm0[c];
+            for (int pi : am) {
+                V3 p = xs[pi] - cm, q = ld(rest, pi) - c0;
                 double pp[3] = {p.x, p.y, p.z}, qq[3] = {q.x, q.y, q.z};
                 for (int r = 0; r < 3; ++r)
                     for (int cc = 0; cc < 3; ++cc) apq[r * 3 + cc] += pp[r] * qq[cc];
             }
             double Rm[9], RL[9];
             polar(apq, Rm, nullptr);
-            if (S.beta != 0.0) {
+            if (linear) {
                 double A[9];
-                const double* ai = &aqqi[9 * (size_t)c];
                 for (int r = 0; r < 3; ++r)
                     for (int cc = 0; cc < 3; ++cc) {
                         double s = 0;
@@ -609,9 +637,8 @@ template <class T> void sm_step(T* pos, T* vel, int n, const int* cs, const int*
             } else {
                 for (int q = 0; q < 9; ++q) RL[q] = Rm[q];
             }
-            for (int t = cs[c]; t < cs[c + 1]; ++t) {
-                int p = mem[t];
-                V3 q = ld(rest, p) - cm0[c];
+            for (int p : am) {
+                V3 q = ld(rest, p) - c0;
                 V3 gp{RL[0] * q.x + RL[1] * q.y + RL[2] * q.z + cm.x, RL[3] * q.x + RL[4] * q.y + RL[5] * q.z + cm.y,
                       RL[6] * q.x + RL[7] * q.y + RL[8] * q.z + cm.z};
                 V3 np = xs[p] + S.alpha * (gp - xs[p]);
@@ -758,6 +785,10 @@ void orc_polar_decomposition(const double* A9, double* R9, double* S9) { polar(A
 void orc_sm_step_f(float* pos4, float* vel4, int n, const int* cs, const int* mem, int nc, const float* rest, const OrcSmParams* S) {
     sm_step(pos4, vel4, n, cs, mem, nc, rest, *S);
 }
+void orc_sm_step_masked_f(float* pos4, float* vel4, int n, const int* cs, const int* mem, int nc, const float* rest,
+                          const OrcSmParams* S, const int* active) {
+    sm_step(pos4, vel4, n, cs, mem, nc, rest, *S, active);
+}
 void orc_sm_step_d(double* pos4, double* vel4, int n, const int* cs, const int* mem, int nc, const double* rest, const OrcSmParams* S) {
     sm_step(pos4, vel4, n, cs, mem, nc, rest, *S);
 }
@@ -781,6 +812,53 @@ void orc_heat_diffuse_f(const float* pos4, const float* dens, const float* temp,
         }
         temp_out[i] = (float)((double)temp[i] + (double)dt * (double)diffusivity * s);
     }
+}
+
+// HeatTransfar::calcTempAndHeat (HeatTransfar.cpp:280-350) for one particle; th = (T, heat, phase, changed).
+static void heat_phase(float* th, float dT) {
+    const float latent = 165.0f;
+    float t = th[0], q = th[1], chg = th[3];
+    int ph = (int)th[2];
+    if (ph == -2 && t > 250.0f) { ph = -1; t = 250.0f; q = 0.0f; }
+    else if (ph == 2 && t < 250.0f) { ph = 1; t = 250.0f; q = latent; }
+    if (ph == -1 || ph == 1) {
+        float c = 2.1f + (2.1f * q / latent);
+        q += dT * c * 1.0f;
+        if (ph == -1 && q < -latent / 300.0f) { ph = -2; t = 249.0f; q = 0.0f; }
+        else if (ph == 1 && q > latent) { ph = 2; t = 251.0f; q = 0.0f; }
+        if (ph == -1 && q > latent) { ph = 2; t = 251.0f; q = 0.0f; chg = 1.0f; }
+        else if (ph == 1 && q < 0.0f) { ph = -2; t = 249.0f; q = 0.0f; chg = 1.0f; }
+    } else {
+        float c = (t > 250.0f) ? 4.2f : 2.1f;
+        t = t + q / (c * 1.0f);
+        q = 0.0f;
+        t += dT;
+        t = std::min(std::max(t, 0.0f), 500.0f);
+    }
+    th[0] = t; th[1] = q; th[2] = (float)ph; th[3] = chg;
+}
+
+// HeatTransfar::heatParticleAndParticle (HeatTransfar.cpp:429-473) over the neighbours within h (true
+// distances), then calcTempAndHeat for every particle.  therm4 = (T, heat, phase, changed) per particle.
+void orc_heat_cpu_f(const float* pos4, const float* dens, float* therm4, int n, const OrcParams* P, float td, float dt) {
+    std::vector<std::vector<Neigh>> nb;
+    neighbour_lists(pos4, n, *P, nb);
+    double h = R<float>(P->h), ws = 10.0 / (7.0 * kPi * h * h);
+    std::vector<float> dT(n);
+    for (int i = 0; i < n; ++i) {
+        double tmp = 0.0;
+        V3 p0 = ld(pos4, i);
+        for (const Neigh& e : nb[i]) {
+            int j = e.idx;
+            if (j == i) continue;
+            double r = norm(p0 - ld(pos4, j)), q = r / h;
+            double w = q < 1.0 ? ws * (1.0 - 1.5 * q * q + 0.75 * q * q * q) : (q < 2.0 ? ws * 0.25 * (2.0 - q) * (2.0 - q) * (2.0 - q) : 0.0);
+            float d = dens[j] < 0.1f ? 0.1f : dens[j];
+            tmp += (double)dt * ((double)therm4[4 * j] - (double)therm4[4 * i]) / d * w;
+        }
+        dT[i] = (float)(tmp * td);
+    }
+    for (int i = 0; i < n; ++i) heat_phase(&therm4[4 * i], dT[i]);
 }
 
 // HeatTransfer.cu:62-75

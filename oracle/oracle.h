@@ -99,6 +99,11 @@ void orc_sm_step_f(float* pos4, float* vel4, int n, const int* cl_start, const i
                    const float* rest_pos4, const OrcSmParams* S);
 void orc_sm_step_d(double* pos4, double* vel4, int n, const int* cl_start, const int* members, int nclusters,
                    const double* rest_pos4, const OrcSmParams* S);
+/* The same step with a melt mask: only clusters whose own particle is active, over their active members (rest
+ * centre of mass and Aqq of the active members; no linear term below 4 of them); inactive particles keep
+ * their prediction. */
+void orc_sm_step_masked_f(float* pos4, float* vel4, int n, const int* cl_start, const int* members, int nclusters,
+                          const float* rest_pos4, const OrcSmParams* S, const int* active);
 
 /* ---- heat (HeatTransfar.cpp / HeatTransfer.cu, SPH diffusion) */
 void orc_heat_diffuse_f(const float* pos4, const float* dens, const float* temp, int n, const OrcParams* P,
@@ -109,6 +114,9 @@ void orc_heat_diffuse_f(const float* pos4, const float* dens, const float* temp,
 void orc_heat_transfer_gpu_f(float* heats, float* temps, float* dtemps, const int* surf, const int* neighs, const float* dens,
                              int* phase, int* changed, const int* melt, float radius, float air, float ht, float td,
                              float dt, int n);
+/* HeatTransfar::heatParticleAndParticle + calcTempAndHeat (HeatTransfar.cpp:280-473) over the neighbours within
+ * h at the given positions; therm4 = (T, heat, phase, changed) per particle, updated in place. */
+void orc_heat_cpu_f(const float* pos4, const float* dens, float* therm4, int n, const OrcParams* P, float td, float dt);
 /* detectNeighborParticles (rx_sph_kernel.cu:1166-1208): every j with |x_i - x_j| <= h (FP32, no contraction),
  * in the order (cell hash of j, j), at most 30; out[i*30 + k], unused slots -1, slot 29 = count. */
 void orc_detect_neighbors_f(const float* pos4, int n, const OrcParams* P, int* out);

@@ -1,4 +1,6 @@
 """Fused PBF solver (rxPBDSPH_GPU) vs the oracle of the CPU path: same step, same inputs."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -79,3 +81,34 @@ def test_list_tile_variants_agree_bitwise(squeeze):
         assert ovf == out[0][3]
     if squeeze < 1.0:
         assert out[0][3] > 0  # lists overflowed too: their passes walked the wide stencil
+
+
+def test_thermal_step_matches_oracle():
+    """Thermal state: after a PBF step the temperatures diffuse at the final positions and the phases advance
+    (HeatTransfar.cpp:280-473 restated by the oracle), from the solver's own final positions and densities."""
+    sc = scene.pbf_dam_break(16, 14, 12, seed=4)
+    pos, vel = sc.particles()
+    n = sc.n
+    rng = np.random.default_rng(8)
+    T = np.where(pos[:, 0] < np.median(pos[:, 0]), 250.0, 270.0).astype(np.float32) + rng.uniform(-1, 1, n).astype(np.float32)
+    T[pos[:, 0] < np.median(pos[:, 0])] = 250.0  # ice exactly at the melting point, water at 20 +- 1 above it
+    phase = np.where(T <= 250.0, -2, 2).astype(np.int32)
+    heat = np.zeros(n, np.float32)
+    s = PBFSolver(sc, pos, vel)
+    assert s.lib.rxPBDSPH_GPU_SetThermal(s.h, T.ctypes.data, heat.ctypes.data, phase.ctypes.data) == 0
+    s.option(_abi.RX_PBF_OPT_HEAT_TD, 40.0)
+    s.step(1)
+    gp = s.positions()
+    gd = s.array(_abi.RX_ARRAY_DENSITY)
+    gT = s.array(_abi.RX_ARRAY_TEMPERATURE)
+    gQ = s.array(_abi.RX_ARRAY_HEAT)
+    gph = s.array(_abi.RX_ARRAY_PHASE)
+    th = np.stack([T, heat, phase.astype(np.float32), np.zeros(n, np.float32)], 1).astype(np.float32).copy()
+    P = oracle.params_from(sc.params(), sc)
+    oracle.load().orc_heat_cpu_f(oracle.ptr(gp), oracle.ptr(gd), oracle.ptr(th), n, C.byref(P), np.float32(40.0), np.float32(sc.dt))
+    assert np.array_equal(gph, th[:, 2])
+    assert np.abs(gT - th[:, 0]).max() <= 1e-5 * 270.0
+    assert np.abs(gQ - th[:, 1]).max() <= 1e-3
+    assert (gph == -1).sum() > 0  # ice next to the water started melting
+    s.step(3)  # the thermal state follows the particles through the rebuilds
+    assert np.isfinite(s.array(_abi.RX_ARRAY_TEMPERATURE)).all()

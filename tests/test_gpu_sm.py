@@ -134,3 +134,32 @@ def test_legacy_launch_shape_matching_disjoint_clusters(lib):
     np.testing.assert_allclose(cur, p[:, :3], atol=2e-6)
     vv = bufs["v"].download(np.float32).reshape(n, 3)
     np.testing.assert_allclose(vv, v[:, :3], atol=5e-4)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_masked_step_parity(graph):
+    """Melt mask: about a third of the particles are no longer ice; clusters use their ice members only."""
+    sc = scene.ice_block(12, 10, 8)
+    sm, pos, vel = _make(sc, graph=graph)
+    cs, mem, _ = sm.clusters()
+    active = (np.random.default_rng(11).random(sc.n) > 0.35).astype(np.int32)
+    sm.set_active(active)
+    S = oracle.OrcSmParams()
+    S.alpha, S.beta, S.dt = sc.alpha, sc.beta, sc.dt
+    for k in range(3):
+        S.gravity[k], S.bmin[k], S.bmax[k] = sc.gravity[k], sc.box_min[k], sc.box_max[k]
+    S.iterations, S.volume_conservation, S.restitution = sc.iterations, 1, 0.9
+    rp, rv, rest = pos.copy(), vel.copy(), pos.copy()
+    for _ in range(3):
+        oracle.load().orc_sm_step_masked_f(oracle.ptr(rp), oracle.ptr(rv), len(rp), oracle.ptr(cs), oracle.ptr(mem), len(cs) - 1,
+                                           oracle.ptr(rest), C.byref(S), oracle.ptr(active))
+    sm.step(3)
+    gp, gv = sm.state()
+    assert np.abs(gp[:, :3] - rp[:, :3]).max() <= 2e-5 * np.abs(rp[:, :3]).max()
+    assert np.abs(gv[:, :3] - rv[:, :3]).max() <= 2e-3 * max(np.abs(rv[:, :3]).max(), 1e-3)
+    sm.set_active(None)  # mask removed: back to the unmasked step
+    sm.set_state(pos, vel)
+    sm.step(1)
+    up, _ = sm.state()
+    rp2, rv2 = _oracle_steps(sc, pos, vel, cs, mem, 1)
+    assert np.abs(up[:, :3] - rp2[:, :3]).max() <= 2e-5 * np.abs(rp2[:, :3]).max()

@@ -157,3 +157,42 @@ def test_detect_surface_rules(orc):
     assert surf[-1] == 2 and surf[-2] == 2
     corner = 0
     assert surf[corner] == nb[corner, 29]
+
+
+def _therm(T, heat=None, phase=None):
+    n = len(T)
+    th = np.zeros((n, 4), np.float32)
+    th[:, 0] = T
+    th[:, 1] = 0 if heat is None else heat
+    th[:, 2] = np.where(np.asarray(T) < 250, -2, 2) if phase is None else phase
+    return th
+
+
+def test_cpu_heat_step_pair_and_conservation(orc):
+    h = 0.1
+    P = _params(h, 0.0, 1.0)
+    # pair at r = 0.5h: dT_0 = td dt (T1 - T0) / rho1 W(0.5h), W = 10/(7 pi h^2)(1 - 1.5/4 + 0.75/8)
+    p = np.zeros((2, 4), np.float32)
+    p[:, :3] = [[0.5, 0.5, 0.5], [0.55, 0.5, 0.5]]
+    dens = np.array([900.0, 1000.0], np.float32)
+    th = _therm([240.0, 260.0], phase=[-2, 2])
+    orc.orc_heat_cpu_f(ptr(p), ptr(dens), ptr(th), 2, C.byref(P), 50.0, 0.01)
+    r = float(np.float32(0.55) - np.float32(0.5)) / h
+    w = 10.0 / (7.0 * math.pi * h * h) * (1 - 1.5 * r * r + 0.75 * r ** 3)
+    d0 = 50.0 * 0.01 * 20.0 / 1000.0 * w
+    d1 = 50.0 * 0.01 * -20.0 / 900.0 * w
+    assert th[0, 0] == pytest.approx(240.0 + d0, rel=1e-6) and th[1, 0] == pytest.approx(260.0 + d1, rel=1e-6)
+    # many particles, uniform density, all sensible: sum of rho-weighted changes vanishes (antisymmetric pairs)
+    rng = np.random.default_rng(3)
+    x = rng.uniform(0.2, 0.5, (600, 3)).astype(np.float32)
+    p = np.zeros((600, 4), np.float32)
+    p[:, :3] = x
+    dens = np.full(600, 1000.0, np.float32)
+    T0 = rng.uniform(100.0, 240.0, 600).astype(np.float32)
+    th = _therm(T0, phase=np.full(600, -2))
+    orc.orc_heat_cpu_f(ptr(p), ptr(dens), ptr(th), 600, C.byref(P), 1.0, 0.01)
+    assert float(np.sum(th[:, 0].astype(np.float64) - T0)) == pytest.approx(0.0, abs=1e-3 * np.abs(th[:, 0] - T0).sum())
+    # a constant field stays constant
+    th = _therm(np.full(600, 120.0, np.float32), phase=np.full(600, -2))
+    orc.orc_heat_cpu_f(ptr(p), ptr(dens), ptr(th), 600, C.byref(P), 1.0, 0.01)
+    assert np.all(th[:, 0] == 120.0)

@@ -129,3 +129,45 @@ def test_beta_one_volume_normalised_linear_map(orc):
     orc.orc_sm_step_d(oracle.ptr(p), oracle.ptr(v), n, oracle.ptr(cs), oracle.ptr(mem), 1, oracle.ptr(rest), C.byref(S))
     expect = (rest[:, :3] - cm0) * (s / np.sqrt(np.prod(s))) + cm0
     np.testing.assert_allclose(p[:, :3], expect, atol=1e-10)
+
+
+def _masked_step(orc, p, v, cs, mem, rest, S, active):
+    p, v = p.astype(np.float32).copy(), v.astype(np.float32).copy()
+    orc.orc_sm_step_masked_f(oracle.ptr(p), oracle.ptr(v), len(p), oracle.ptr(cs), oracle.ptr(mem), len(cs) - 1,
+                             oracle.ptr(np.ascontiguousarray(rest, np.float32)), C.byref(S), oracle.ptr(active))
+    return p, v
+
+
+def test_mask_all_active_equals_unmasked(orc):
+    rng = np.random.default_rng(5)
+    rest = _lattice().astype(np.float32)
+    cs, mem = _clusters(rest, 0.25)
+    pos = rest.copy()
+    pos[:, :3] += rng.normal(0, 0.01, size=(len(rest), 3)).astype(np.float32)
+    vel = np.zeros_like(pos)
+    S = _params(alpha=0.9, beta=0.3, iters=2, g=(0, -9.8, 0))
+    p1, v1 = pos.copy(), vel.copy()
+    orc.orc_sm_step_f(oracle.ptr(p1), oracle.ptr(v1), len(p1), oracle.ptr(cs), oracle.ptr(mem), len(cs) - 1, oracle.ptr(rest), C.byref(S))
+    p2, v2 = _masked_step(orc, pos, vel, cs, mem, rest, S, np.ones(len(rest), np.int32))
+    assert np.array_equal(p1, p2) and np.array_equal(v1, v2)
+
+
+@pytest.mark.parametrize("beta", [0.0, 0.3])
+def test_masked_rigid_motion_is_a_fixed_point(orc, beta):
+    # Clusters that lost members must use the rest centre of mass and Aqq of the members left: a rigid motion of
+    # the rest shape is then still reproduced exactly by every active particle.  With stale (all-member) rest
+    # quantities the goals would be shifted by R (cm0_active - cm0_all).
+    rng = np.random.default_rng(6)
+    rest = _lattice().astype(np.float32)
+    cs, mem = _clusters(rest, 0.25)
+    Q = _rot(rng)
+    t = rng.normal(size=3)
+    pos = rest.copy()
+    pos[:, :3] = (rest[:, :3].astype(np.float64) @ Q.T + t).astype(np.float32)
+    active = (rng.random(len(rest)) > 0.3).astype(np.int32)
+    S = _params(alpha=1.0, beta=beta, iters=2)
+    p, v = _masked_step(orc, pos, np.zeros_like(pos), cs, mem, rest, S, active)
+    a = active == 1
+    np.testing.assert_allclose(p[a, :3], pos[a, :3], atol=2e-5)
+    # inactive particles keep their prediction (zero velocity, no gravity here: unchanged)
+    np.testing.assert_array_equal(p[~a, :3], pos[~a, :3])
