@@ -9,8 +9,8 @@
 //   3. the shape-matching substep (prediction from the post-fluid state, as OrientedPrtObj::UpdateCluster_SM
 //      predicts from the SPH particles, ElasticObject_OP.cpp:572-608; clusters of ice members only);
 //   4. write-back of the still-frozen particles into the PBF state (LaunchInterPolationGPU's blend of 1,
-//      IceObject.cpp:876-897).  The write-back raises the PBF solver's largest-displacement bound so that
-//      its neighbour lists stay exact.
+//      IceObject.cpp:876-897).  The write-back raises the PBF solver's largest-displacement bound, and the
+//      next step starts from a grid built on the written-back positions.
 // The two objects run on their own streams, ordered by events.
 #include <vector>
 
@@ -73,6 +73,9 @@ public:
         m_ni = (uint)ni;
         m_pbf = new rxPBDSPH_GPU();
         if (m_pbf->Initialize(fenv) || m_pbf->SetParticles(pos4, vel4, nf + ni)) return 1;
+        // the ice write-back moves particles by up to about h/2 per step: a wider skin and list keep the
+        // corrections on the list path
+        if (m_pbf->SetOption(RX_PBF_OPT_MAX_NEIGHBORS, 128.0f) || m_pbf->SetOption(RX_PBF_OPT_SKIN, 0.25f)) return 1;
         if (temps && m_pbf->SetThermal(temps, nullptr, phase)) return 1;
         m_thermal = temps != nullptr;
         m_sm = rxIceSM_Create(&ienv, pos4 + 4 * (size_t)nf, ni);  // rest shape: the initial ice positions
@@ -116,6 +119,7 @@ public:
             k_ice_scatter<<<nb, 256, 0, b>>>(d.x, d.v, d.id, d.xb, n, m_nf, m_sx, m_sv, m_act, d.stats + d.disp2_slot);
             RX_HIP(hipEventRecord(m_ev_b, b));
             RX_HIP(hipStreamWaitEvent(a, m_ev_b, 0));
+            m_pbf->invalidate_list();  // the next prediction pass walks a grid built on the written-back state
             if (!RX_HIP(hipGetLastError())) return 1;
         }
         return 0;

@@ -44,6 +44,8 @@ public:
         int disp2_slot;
     };
     DeviceView device_view() const;
+    // The state was moved outside the solver (more than the skin): the next step starts from a fresh grid.
+    void invalidate_list() { m_have_list = false; }
     uint size() const { return m_n; }
 
 private:
@@ -89,7 +91,7 @@ private:
     float4* m_ftmp16 = nullptr;
     uint* m_id = nullptr;       // slot -> original particle index
     uint* m_idt = nullptr;
-    uint* m_list = nullptr;     // [slot][particle] skin neighbour list
+    uint* m_list = nullptr;     // skin neighbour list: uint4 chunks of 4 entries, [chunk][slot]
     uint* m_cnt = nullptr;
     float* m_stats = nullptr;
     // profiling

@@ -29,7 +29,7 @@ struct PbfK {
     float rc2;        // (h + skin)^2, list cutoff
     float half_skin2; // (skin / 2)^2
     float skin;       // list cutoff minus h
-    float wide2;      // largest displacement^2 the 5x5x5 stencil still covers
+    float wide2;      // largest displacement^2 the capped fallback walk still covers
     int maxn;
     float eps;
     float dt, inv_dt;
@@ -48,6 +48,7 @@ struct PbfK {
 namespace {
 
 constexpr int kTpb = 256;
+constexpr int kMaxStencil = 4;  // largest cell radius of a fallback walk
 
 // Per-step statistics slots (float/uint in one array).
 enum Stat {
@@ -136,10 +137,12 @@ __device__ inline float3 clamp_wall(float3 x, const SphK& s) {
 
 // Neighbour candidates of slot i for a pass that needs every j with |x_i - x_j| <= h at the current positions.
 // mode 0: the skin list; mode 1: the 3x3x3 stencil of a grid built on the current positions (cell >= h).
-// A pair within h now was within h + d_i + d_j <= h + d_i + D at the list build (d = displacement since the
-// build, D = its maximum over all particles), so the list of i holds every such j while d_i + D <= skin;
-// otherwise (or when the list overflowed) the pass walks the 5x5x5 stencil of the build-time grid, which
-// covers build distances below 2 cell widths.  `p` is the current position of i, `xb` the build positions.
+// A pair within h now was within L = h + d_i + D of each other at the list build (d = displacement since the
+// build, D = its maximum over all particles), so the list of i holds every such j while d_i + D <= skin.
+// Otherwise (or when the list overflowed) the pass walks the build-time grid over ceil(L / cell) cells in
+// every direction, which holds every such j; the radius is capped at kMaxStencil cells (displacements beyond
+// that raise the stats flag checked by the prediction pass).  `p` is the current position of i, `xb` the
+// build positions.
 template <class F>
 __device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint* __restrict__ list, const uint* __restrict__ cnt,
                                      const uint* __restrict__ bhash, const uint* __restrict__ off, float disp2, float3 p,
@@ -150,23 +153,35 @@ __device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint
     }
     uint c = cnt[i];
     bool ok = c <= (uint)k.maxn;
-    if (ok && disp2 > k.half_skin2) {
+    float di = 0.0f;
+    if (disp2 > k.half_skin2) {
         float3 dd = p - f3(xb[i]);
-        ok = sqrtf(dot3(dd, dd)) + sqrtf(disp2) <= k.skin;
+        di = sqrtf(dot3(dd, dd));
+        ok = ok && di + sqrtf(disp2) <= k.skin;
     }
     if (ok) {
-        const uint* l = list + i;
-        uint s = 0;
-        for (; s + 3 < c; s += 4) {  // four independent gathers in flight
-            uint j0 = l[(size_t)s * k.n], j1 = l[(size_t)(s + 1) * k.n], j2 = l[(size_t)(s + 2) * k.n], j3 = l[(size_t)(s + 3) * k.n];
-            f(j0);
-            f(j1);
-            f(j2);
-            f(j3);
+        // list layout: uint4 chunks of 4 entries, chunk q of slot i at [q * n + i] (a wavefront reads 1 KB
+        // contiguous per chunk); four independent gathers in flight per chunk
+        const uint4* l4 = (const uint4*)list + i;
+        const uint full = c >> 2;
+        for (uint q = 0; q < full; ++q) {
+            uint4 e = l4[(size_t)q * k.n];
+            f(e.x);
+            f(e.y);
+            f(e.z);
+            f(e.w);
         }
-        for (; s < c; ++s) f(l[(size_t)s * k.n]);
+        const uint r = c & 3u;
+        if (r) {
+            uint4 e = l4[(size_t)full * k.n];
+            f(e.x);
+            if (r > 1) f(e.y);
+            if (r > 2) f(e.z);
+        }
     } else {
-        fg_for_stencil(fg_unhash(bhash[i], k.g), k.g, off, f, 2);
+        int rad = (int)ceilf((k.s.h + di + sqrtf(disp2)) * k.g.inv_cell * 1.0001f);
+        rad = min(max(rad, 1), kMaxStencil);
+        fg_for_stencil(fg_unhash(bhash[i], k.g), k.g, off, f, rad);
     }
 }
 
@@ -243,11 +258,19 @@ __device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, float4
     float nn = 0.0f, wsum = 0.0f, wcs = 0.0f, ring = 0.0f, nring = 0.0f, div = 0.0f;
     const float h2q = 0.25f * k.s.h2;
     float3 vi = feat ? f3(v[i]) : f3(0, 0, 0);
+    uint4* l4 = (uint4*)list + i;  // chunk q of slot i at l4[q * n]
+    uint b0 = 0, b1 = 0, b2 = 0, b3 = 0;  // the open chunk, newest entry in b3
     cands([&](uint j, float4 q4) {  // q4 = (x*_j, density at the step start)
         float3 r = p - f3(q4);
         float r2 = dot3(r, r);
         if (r2 <= k.rc2) {
-            if (c < (uint)k.maxn) list[(size_t)c * k.n + i] = j;
+            if (c < (uint)k.maxn) {
+                b0 = b1;
+                b1 = b2;
+                b2 = b3;
+                b3 = j;
+                if ((c & 3u) == 3u) l4[(size_t)(c >> 2) * k.n] = make_uint4(b0, b1, b2, b3);
+            }
             ++c;
             if (r2 <= k.s.h2) {
                 float q2 = k.s.h2 - r2;
@@ -275,6 +298,11 @@ __device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, float4
             }
         }
     });
+    if (c <= (uint)k.maxn && (c & 3u)) {  // the last, partial chunk (entries beyond c are never read)
+        const uint r = c & 3u;
+        uint4 e = r == 1 ? make_uint4(b3, 0, 0, 0) : (r == 2 ? make_uint4(b2, b3, 0, 0) : make_uint4(b1, b2, b3, 0));
+        l4[(size_t)(c >> 2) * k.n] = e;
+    }
     cnt[i] = c;
     if (c > (uint)k.maxn) atomicAdd((unsigned int*)&stats[ST_OVERFLOW], 1u);
     d *= k.s.mass * k.s.wpoly6;
@@ -312,7 +340,26 @@ __device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, float4
     }
 }
 
-// One workgroup per segment of kSegW cells along x of one (y, z) row.  The candidates of the whole segment
+// One thread per slot, in slot order (the default): the 27-cell walk reads candidates from global memory; the
+// state is in cell order, so a wavefront's candidates are few distinct cells and mostly L1/L2 hits.
+__global__ __launch_bounds__(kTpb) void k_list_lambda_slots(const float4* __restrict__ xs, float4* __restrict__ xl,
+                                                            float4* __restrict__ xbuild, float* __restrict__ dens, uint* __restrict__ list,
+                                                            uint* __restrict__ cnt, float* __restrict__ stats, PbfK k,
+                                                            const uint* __restrict__ bhash, const uint* __restrict__ off,
+                                                            float* __restrict__ feat, const float4* __restrict__ v) {
+    bool valid;
+    uint i = slot_of(k, valid);
+    float emax = 0.0f, esum = 0.0f;
+    if (valid) {
+        auto cands = [&](auto&& f) { fg_for_stencil(fg_unhash(bhash[i], k.g), k.g, off, [&](uint j) { f(j, xs[j]); }); };
+        list_lambda_slot(i, xs[i], cands, xl, xbuild, dens, list, cnt, stats, k, feat, v, emax, esum);
+    }
+    block_max_atomic(emax, &stats[ST_ERR_ITER + 0]);
+    block_sum_atomic(esum, &stats[ST_MPOS_ITER + 0]);
+}
+
+// Alternative with LDS staging (RX_PBF_OPT_LIST_TILE 1..4).  One workgroup per segment of kSegW cells along x
+// of one (y, z) row.  The candidates of the whole segment
 // are the slots of 9 rows (y - 1 .. y + 1, z - 1 .. z + 1) over the cells x0 - 1 .. x1 + 1: nine contiguous
 // slot ranges, staged into LDS with coalesced loads.  Every home slot then walks its 27 cells from LDS.  A
 // segment whose neighbourhood exceeds the LDS tile (very dense regions) reads the candidates from global
@@ -729,7 +776,7 @@ PbfK rxPBDSPH_GPU::consts(float dt) const {
     k.rc2 = rc * rc;
     k.half_skin2 = 0.25f * m_skin * m_skin;
     k.skin = m_skin;
-    float wide = 0.5f * (2.0f * m_grid.k.cell - m_env.h);  // 5x5x5 stencil reach minus h, shared by both particles
+    float wide = 0.5f * ((float)kMaxStencil * m_grid.k.cell - m_env.h);  // largest fallback reach minus h, per particle
     k.wide2 = wide * wide;
     k.maxn = m_maxn;
     k.eps = m_env.epsilon;
@@ -818,20 +865,23 @@ int rxPBDSPH_GPU::Update(float dt, int step) {
         const FGridK& gk = m_grid.k;
         switch (m_list_tile) {
         case 1:
-            k_list_lambda<16, 2048, 256><<<list_segments<16>(gk), 256, 0, s>>>(m_xs, m_xl, m_xb, m_dens, m_list, m_cnt, m_stats, k,
+            k_list_lambda<16, 3072, 256><<<list_segments<16>(gk), 256, 0, s>>>(m_xs, m_xl, m_xb, m_dens, m_list, m_cnt, m_stats, k,
                                                                              m_grid.hash, m_grid.off, f, m_v);
             break;
         case 2:
+            k_list_lambda<16, 2048, 256><<<list_segments<16>(gk), 256, 0, s>>>(m_xs, m_xl, m_xb, m_dens, m_list, m_cnt, m_stats, k,
+                                                                             m_grid.hash, m_grid.off, f, m_v);
+            break;
+        case 3:
             k_list_lambda<8, 1536, 128><<<list_segments<8>(gk), 128, 0, s>>>(m_xs, m_xl, m_xb, m_dens, m_list, m_cnt, m_stats, k,
                                                                            m_grid.hash, m_grid.off, f, m_v);
             break;
-        case 3:  // no staging: every segment reads its candidates from global memory
+        case 4:  // segments without staging: every segment reads its candidates from global memory
             k_list_lambda<16, 0, 256><<<list_segments<16>(gk), 256, 0, s>>>(m_xs, m_xl, m_xb, m_dens, m_list, m_cnt, m_stats, k,
                                                                           m_grid.hash, m_grid.off, f, m_v);
             break;
         default:
-            k_list_lambda<16, 3072, 256><<<list_segments<16>(gk), 256, 0, s>>>(m_xs, m_xl, m_xb, m_dens, m_list, m_cnt, m_stats, k,
-                                                                             m_grid.hash, m_grid.off, f, m_v);
+            k_list_lambda_slots<<<nb, kTpb, 0, s>>>(m_xs, m_xl, m_xb, m_dens, m_list, m_cnt, m_stats, k, m_grid.hash, m_grid.off, f, m_v);
         }
     }
     end_mark(mk);
@@ -971,6 +1021,7 @@ int rxPBDSPH_GPU::GetStats(float* out, int count) {
     // uint counters are returned as floats
     h[ST_OVERFLOW] = (float)__builtin_bit_cast(uint, h[ST_OVERFLOW]);
     h[ST_ITERS] = (float)__builtin_bit_cast(uint, h[ST_ITERS]);
+    h[ST_BAD] = (float)__builtin_bit_cast(uint, h[ST_BAD]);
     h[ST_MEAN_ABS_START] = h[ST_MEAN_ABS_START] / (float)m_n;
     h[ST_MPOS_START] = h[ST_MPOS_START] / (float)m_n;
     for (int t = 0; t < 16; ++t) h[ST_MPOS_ITER + t] = h[ST_MPOS_ITER + t] / (float)m_n;
@@ -1113,10 +1164,26 @@ int rxPBDSPH_GPU::SetOption(int opt, float value) {
         m_profile_mask = (unsigned)value;
         return 0;
     case RX_PBF_OPT_HEAT_TD: m_heat_td = value; return 0;
+    case RX_PBF_OPT_MAX_NEIGHBORS: {
+        int m = (int)value;
+        if (m < 8 || m > 512 || (m & 3)) {
+            rx_set_error(1, "rxPBDSPH_GPU::SetOption: list capacity must be a multiple of 4 in 8..512");
+            return 1;
+        }
+        if (m != m_maxn && m_list) {
+            RX_HIP(hipStreamSynchronize(m_stream));
+            RX_HIP(hipFree(m_list));
+            m_list = nullptr;
+            if (!RX_HIP(hipMalloc(&m_list, sizeof(uint) * (size_t)m_n * m))) return 1;
+        }
+        m_maxn = m;
+        m_have_list = false;
+        return 0;
+    }
     case RX_PBF_OPT_HEAT_DT: m_heat_dt = value; return 0;
     case RX_PBF_OPT_LIST_TILE:
-        if (value < 0.0f || value > 3.0f) {
-            rx_set_error(1, "rxPBDSPH_GPU::SetOption: list tile variant must be 0..3");
+        if (value < 0.0f || value > 4.0f) {
+            rx_set_error(1, "rxPBDSPH_GPU::SetOption: list tile variant must be 0..4");
             return 1;
         }
         m_list_tile = (int)value;

@@ -100,10 +100,12 @@ enum rxPbfOption {
     RX_PBF_OPT_RESET_RUN = 11,     /* clears the run accumulators (stats slots 48..63)                     */
     RX_PBF_OPT_PROFILE_MASK = 12,  /* bit c set: time kernel class c (rxProfileClass) while profiling;
                                       default all                                                          */
-    RX_PBF_OPT_LIST_TILE = 13,     /* list pass: 0 = 16-cell segments, 48 KB LDS tile (default); 1 = 32 KB
-                                      tile; 2 = 8-cell segments, 24 KB tile, 128 threads; 3 = no LDS tile  */
+    RX_PBF_OPT_LIST_TILE = 13,     /* list pass: 0 = one thread per slot, global reads (default); LDS-staged
+                                      row segments: 1 = 16 cells, 48 KB tile; 2 = 16 cells, 32 KB; 3 = 8
+                                      cells, 24 KB, 128 threads; 4 = 16-cell segments without the tile     */
     RX_PBF_OPT_HEAT_TD = 14,       /* thermal diffusion coefficient (HeatTransfar mTD; default 0)         */
-    RX_PBF_OPT_HEAT_DT = 15        /* heat time step (0: the step's dt)                                    */
+    RX_PBF_OPT_HEAT_DT = 15,       /* heat time step (0: the step's dt)                                    */
+    RX_PBF_OPT_MAX_NEIGHBORS = 16  /* neighbour-list capacity per particle, a multiple of 4 (default 96)    */
 };
 
 /* Kernel classes of rxPBDSPH_GPU_GetProfile. */
@@ -126,7 +128,8 @@ enum rxProfileClass {
  *   17      largest displacement since the neighbour-list build
  *   18      particles whose neighbour list overflowed
  *   19      corrections executed in the step
- *   22      1 when a displacement exceeded what the wide cell stencil covers
+ *   22      1 when a displacement since the list build exceeded what the capped fallback walk covers
+ *           (4 cells: the passes may then miss pairs)
  *   23      mean |rho - rho0| / rho0 at the step's start positions
  *   24      mean (rho - rho0)+ / rho0 at the step's start positions
  *   32 + k  the same at the start of Jacobi iteration k (k < 16)

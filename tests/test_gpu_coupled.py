@@ -35,6 +35,9 @@ def test_first_step_is_fluid_step_then_masked_shape_matching():
     h = lib.rxPBDSPH_GPU_Create(C.byref(env))
     assert h
     assert lib.rxPBDSPH_GPU_SetParticles(h, pos.ctypes.data, vel.ctypes.data, cs.n) == 0
+    # the coupled object's list settings (same grid, same summation order)
+    assert lib.rxPBDSPH_GPU_SetOption(h, _abi.RX_PBF_OPT_MAX_NEIGHBORS, np.float32(128)) == 0
+    assert lib.rxPBDSPH_GPU_SetOption(h, _abi.RX_PBF_OPT_SKIN, np.float32(0.25)) == 0
     assert lib.rxPBDSPH_GPU_SetThermal(h, temps.ctypes.data, None, phase.ctypes.data) == 0
     assert lib.rxPBDSPH_GPU_SetOption(h, _abi.RX_PBF_OPT_HEAT_TD, np.float32(cs.heat_td)) == 0
     assert lib.rxPBDSPH_GPU_Update(h, np.float32(fl.dt), 1) == 0
@@ -74,6 +77,9 @@ def test_first_step_is_fluid_step_then_masked_shape_matching():
 
 def test_coupled_run_melts_ice_at_the_contact():
     cs = _small()
+    sp = cs.fluid.spacing
+    # ice 1.5 spacings from the water (within h): heat crosses the interface from the first step
+    cs.ice_origin = (cs.fluid.block[0] * sp + 1.5 * sp, cs.ice_origin[1], cs.ice_origin[2])
     c = Coupled(cs)
     c.autotune()
     c.step(40)
