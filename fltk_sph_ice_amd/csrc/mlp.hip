@@ -33,9 +33,11 @@ __device__ inline unsigned short f2bf(float f) {  // round to nearest even
     return (unsigned short)(u >> 16);
 }
 __device__ inline float bf2f(unsigned short b) { return __uint_as_float((uint)b << 16); }
-__device__ inline float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU with the hardware exp2 and reciprocal (one v_exp_f32, one v_rcp_f32): the activations are rounded to
+// bf16 for the next layer, far coarser than the few-ulp error of the fast division.
+__device__ inline float silu(float z) { return __fdividef(z, 1.0f + __expf(-z)); }
 __device__ inline float dsilu(float z) {
-    float s = 1.0f / (1.0f + __expf(-z));
+    float s = __fdividef(1.0f, 1.0f + __expf(-z));
     return s * (1.0f + z * (1.0f - s));
 }
 

@@ -143,12 +143,111 @@ __device__ inline float3 clamp_wall(float3 x, const SphK& s) {
 // every direction, which holds every such j; the radius is capped at kMaxStencil cells (displacements beyond
 // that raise the stats flag checked by the prediction pass).  `p` is the current position of i, `xb` the
 // build positions.
-template <class F>
+// Walks the cells within `rad` of cell c (rows of contiguous slots), fetching four candidates ahead of their use:
+// the loads of a group are issued together, so a wavefront keeps several gathers in flight.
+template <class Fetch, class Body>
+__device__ inline void stencil_batched(int3 c, const FGridK& g, const uint* __restrict__ off, Fetch&& fetch, Body&& body, int rad) {
+    int x0 = max(c.x - rad, 0), x1 = min(c.x + rad, g.gx - 1);
+    int y0 = max(c.y - rad, 0), y1 = min(c.y + rad, g.gy - 1);
+    int z0 = max(c.z - rad, 0), z1 = min(c.z + rad, g.gz - 1);
+    for (int z = z0; z <= z1; ++z)
+        for (int y = y0; y <= y1; ++y) {
+            uint hb = (uint)((z * g.gy + y) * g.gx);
+            uint j = off[hb + x0], e = off[hb + x1 + 1];
+            for (; j + 3 < e; j += 4) {
+                auto a = fetch(j);
+                auto b = fetch(j + 1);
+                auto cc = fetch(j + 2);
+                auto d = fetch(j + 3);
+                body(j, a);
+                body(j + 1, b);
+                body(j + 2, cc);
+                body(j + 3, d);
+            }
+            for (; j < e; ++j) body(j, fetch(j));
+        }
+}
+
+// Neighbour candidates of slot i for a pass that needs every j with |x_i - x_j| <= h at the current positions;
+// `fetch(j)` loads what the pass needs of j, `body(j, data)` consumes it.
+// mode 0: the skin list; mode 1: the 3x3x3 stencil of a grid built on the current positions (cell >= h).
+// A pair within h now was within L = h + d_i + D of each other at the list build (d = displacement since the
+// build, D = its maximum over all particles), so the list of i holds every such j while d_i + D <= skin.
+// Otherwise (or when the list overflowed) the pass walks the build-time grid over ceil(L / cell) cells in
+// every direction, which holds every such j; the radius is capped at kMaxStencil cells (displacements beyond
+// that raise the stats flag checked by the prediction pass).  `p` is the current position of i, `xb` the
+// build positions.
+// The list walk: groups of G chunks (4 G neighbours); the next group's indices are loaded while the current
+// group's neighbours are fetched, and all fetches of a group go out before any of them is used.
+template <int G, class Fetch, class Body>
+__device__ inline void list_walk(const uint4* __restrict__ l4, uint c, uint n, Fetch&& fetch, Body&& body) {
+    using D = decltype(fetch(0u));
+    const uint full = c >> 2;
+    uint q = 0;
+    if (full) {
+        uint4 e[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) e[g] = l4[(size_t)min((uint)g, full - 1) * n];
+        for (; q + G <= full; q += G) {
+            uint4 en[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) en[g] = l4[(size_t)min(q + G + g, full - 1) * n];
+            D d[4 * G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                d[4 * g] = fetch(e[g].x);
+                d[4 * g + 1] = fetch(e[g].y);
+                d[4 * g + 2] = fetch(e[g].z);
+                d[4 * g + 3] = fetch(e[g].w);
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                body(e[g].x, d[4 * g]);
+                body(e[g].y, d[4 * g + 1]);
+                body(e[g].z, d[4 * g + 2]);
+                body(e[g].w, d[4 * g + 3]);
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g) e[g] = en[g];
+        }
+#pragma unroll
+        for (int g = 0; g + 1 < G; ++g) {  // fewer than G whole chunks left: e[0 ..] hold them
+            if (q + g < full) {
+                D a = fetch(e[g].x), b = fetch(e[g].y), cc = fetch(e[g].z), dd = fetch(e[g].w);
+                body(e[g].x, a);
+                body(e[g].y, b);
+                body(e[g].z, cc);
+                body(e[g].w, dd);
+            }
+        }
+    }
+    const uint r = c & 3u;
+    if (r) {
+        uint4 t = l4[(size_t)full * n];
+        D a = fetch(t.x);
+        if (r > 1) {
+            D b = fetch(t.y);
+            if (r > 2) {
+                D cc = fetch(t.z);
+                body(t.x, a);
+                body(t.y, b);
+                body(t.z, cc);
+            } else {
+                body(t.x, a);
+                body(t.y, b);
+            }
+        } else {
+            body(t.x, a);
+        }
+    }
+}
+
+template <int G = 1, class Fetch, class Body>
 __device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint* __restrict__ list, const uint* __restrict__ cnt,
                                      const uint* __restrict__ bhash, const uint* __restrict__ off, float disp2, float3 p,
-                                     const float4* __restrict__ xb, F&& f) {
+                                     const float4* __restrict__ xb, Fetch&& fetch, Body&& body) {
     if (mode == 1) {
-        fg_for_stencil(fg_unhash(bhash[i], k.g), k.g, off, f, 1);
+        stencil_batched(fg_unhash(bhash[i], k.g), k.g, off, fetch, body, 1);
         return;
     }
     uint c = cnt[i];
@@ -161,29 +260,18 @@ __device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint
     }
     if (ok) {
         // list layout: uint4 chunks of 4 entries, chunk q of slot i at [q * n + i] (a wavefront reads 1 KB
-        // contiguous per chunk); four independent gathers in flight per chunk
-        const uint4* l4 = (const uint4*)list + i;
-        const uint full = c >> 2;
-        for (uint q = 0; q < full; ++q) {
-            uint4 e = l4[(size_t)q * k.n];
-            f(e.x);
-            f(e.y);
-            f(e.z);
-            f(e.w);
-        }
-        const uint r = c & 3u;
-        if (r) {
-            uint4 e = l4[(size_t)full * k.n];
-            f(e.x);
-            if (r > 1) f(e.y);
-            if (r > 2) f(e.z);
-        }
+        // contiguous per chunk)
+        list_walk<G>((const uint4*)list + i, c, k.n, fetch, body);
     } else {
         int rad = (int)ceilf((k.s.h + di + sqrtf(disp2)) * k.g.inv_cell * 1.0001f);
         rad = min(max(rad, 1), kMaxStencil);
-        fg_for_stencil(fg_unhash(bhash[i], k.g), k.g, off, f, rad);
+        stencil_batched(fg_unhash(bhash[i], k.g), k.g, off, fetch, body, rad);
     }
 }
+
+struct XV {  // position and velocity of a neighbour
+    float4 x, v;
+};
 
 // ------------------------------------------------------------------------------------------ predict
 // rxPBDSPH::calDensity + calForceExtAndVisc + integrate (rx_sph_pbd.cpp:599-689, 1409-1446), fused.
@@ -202,16 +290,17 @@ __global__ __launch_bounds__(kTpb) void k_predict(const float4* __restrict__ x, 
         float3 p = f3(x[i]), vi = f3(v[i]);
         float d = 0.0f;
         float3 fv = f3(0, 0, 0);
-        for_neighbors(i, mode, k, list, cnt, bhash, off, disp2, p, xb, [&](uint j) {
-            float4 pj = x[j];
-            float3 r = p - f3(pj);
-            float r2 = dot3(r, r);
-            if (r2 <= k.s.h2) {
-                float q = k.s.h2 - r2;
-                d += q * q * q;
-                if (r2 > 0.0f) fv += (k.s.h - r2 * rinv_of(r2)) * (f3(v[j]) - vi);
-            }
-        });
+        for_neighbors<2>(
+            i, mode, k, list, cnt, bhash, off, disp2, p, xb, [&](uint j) { return XV{x[j], v[j]}; },
+            [&](uint, const XV& q) {
+                float3 r = p - f3(q.x);
+                float r2 = dot3(r, r);
+                if (r2 <= k.s.h2) {
+                    float qq = k.s.h2 - r2;
+                    d += qq * qq * qq;
+                    if (r2 > 0.0f) fv += (k.s.h - r2 * rinv_of(r2)) * (f3(q.v) - vi);
+                }
+            });
         d *= k.s.mass * k.s.wpoly6;
         float3 a = (k.s.visc * k.s.mass * k.s.lwvisc / d) * fv + k.s.gravity;  // viscosity / rho_i (rx_sph_pbd.cpp:676)
         float3 vn = vi + k.dt * a;
@@ -246,7 +335,8 @@ __global__ void k_fold_run(float* __restrict__ stats, PbfK k) {
 // `cands(f)` calls f(j, xs[j]) for every candidate j of the 27 cells around slot i, rows in z, y order and
 // slots ascending within a row.
 template <class Cands>
-__device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, float4* __restrict__ xl, float4* __restrict__ xbuild,
+__device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, const float4* __restrict__ xs, float4* __restrict__ xl,
+                                        float4* __restrict__ xbuild,
                                         float* __restrict__ dens, uint* __restrict__ list, uint* __restrict__ cnt,
                                         float* __restrict__ stats, const PbfK& k, float* __restrict__ feat,
                                         const float4* __restrict__ v, float& emax, float& esum) {
@@ -257,7 +347,7 @@ __device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, float4
     // learned-start extras
     float nn = 0.0f, wsum = 0.0f, wcs = 0.0f, ring = 0.0f, nring = 0.0f, div = 0.0f;
     const float h2q = 0.25f * k.s.h2;
-    float3 vi = feat ? f3(v[i]) : f3(0, 0, 0);
+    const float3 vi = feat ? f3(v[i]) : f3(0, 0, 0);
     uint4* l4 = (uint4*)list + i;  // chunk q of slot i at l4[q * n]
     uint b0 = 0, b1 = 0, b2 = 0, b3 = 0;  // the open chunk, newest entry in b3
     cands([&](uint j, float4 q4) {  // q4 = (x*_j, density at the step start)
@@ -351,8 +441,10 @@ __global__ __launch_bounds__(kTpb) void k_list_lambda_slots(const float4* __rest
     uint i = slot_of(k, valid);
     float emax = 0.0f, esum = 0.0f;
     if (valid) {
-        auto cands = [&](auto&& f) { fg_for_stencil(fg_unhash(bhash[i], k.g), k.g, off, [&](uint j) { f(j, xs[j]); }); };
-        list_lambda_slot(i, xs[i], cands, xl, xbuild, dens, list, cnt, stats, k, feat, v, emax, esum);
+        auto cands = [&](auto&& f) {
+            stencil_batched(fg_unhash(bhash[i], k.g), k.g, off, [&](uint j) { return xs[j]; }, [&](uint j, const float4& q) { f(j, q); }, 1);
+        };
+        list_lambda_slot(i, xs[i], cands, xs, xl, xbuild, dens, list, cnt, stats, k, feat, v, emax, esum);
     }
     block_max_atomic(emax, &stats[ST_ERR_ITER + 0]);
     block_sum_atomic(esum, &stats[ST_MPOS_ITER + 0]);
@@ -438,7 +530,7 @@ __global__ __launch_bounds__(kThreads) void k_list_lambda(const float4* __restri
                 }
             }
         };
-        list_lambda_slot(i, p4, cands, xl, xbuild, dens, list, cnt, stats, k, feat, v, emax, esum);
+        list_lambda_slot(i, p4, cands, xs, xl, xbuild, dens, list, cnt, stats, k, feat, v, emax, esum);
     }
     block_max_atomic(emax, &stats[ST_ERR_ITER + 0]);
     block_sum_atomic(esum, &stats[ST_MPOS_ITER + 0]);
@@ -459,21 +551,23 @@ __global__ __launch_bounds__(kTpb) void k_lambda(const float4* __restrict__ xa, 
         float3 p = f3(xa[i]);
         float d = 0.0f, sd = 0.0f;
         float3 gs = f3(0, 0, 0);
-        for_neighbors(i, 0, k, list, cnt, bhash, off, disp2, p, xb, [&](uint j) {
-            float3 r = p - f3(xa[j]);
-            float r2 = dot3(r, r);
-            if (r2 <= k.s.h2) {
-                float q2 = k.s.h2 - r2;
-                d += q2 * q2 * q2;
-                if (r2 > 0.0f) {
-                    float ri = rinv_of(r2);
-                    float q = k.s.h - r2 * ri;
-                    float3 g = (k.s.gwspiky * k.s.inv_rho0 * q * q * ri) * r;
-                    gs += g;
-                    sd += dot3(g, g);
+        for_neighbors<4>(
+            i, 0, k, list, cnt, bhash, off, disp2, p, xb, [&](uint j) { return xa[j]; },
+            [&](uint, const float4& qj) {
+                float3 r = p - f3(qj);
+                float r2 = dot3(r, r);
+                if (r2 <= k.s.h2) {
+                    float q2 = k.s.h2 - r2;
+                    d += q2 * q2 * q2;
+                    if (r2 > 0.0f) {
+                        float ri = rinv_of(r2);
+                        float q = k.s.h - r2 * ri;
+                        float3 g = (k.s.gwspiky * k.s.inv_rho0 * q * q * ri) * r;
+                        gs += g;
+                        sd += dot3(g, g);
+                    }
                 }
-            }
-        });
+            });
         d *= k.s.mass * k.s.wpoly6;
         sd += dot3(gs, gs);
         float C = d * k.s.inv_rho0 - 1.0f;
@@ -508,22 +602,23 @@ __global__ __launch_bounds__(kTpb) void k_dp(const float4* __restrict__ xl, floa
         float li = p4.w;
         float dt2 = k.dt * k.dt;
         float3 acc = f3(0, 0, 0);
-        for_neighbors(i, 0, k, list, cnt, bhash, off, disp2, p, xbuild, [&](uint j) {
-            float4 q4 = xl[j];
-            float3 r = p - f3(q4);
-            float r2 = dot3(r, r);
-            if (r2 <= k.s.h2 && r2 > 0.0f) {
-                float ri = rinv_of(r2);
-                float sc = 0.0f;
-                if (k.s.ap) {
-                    float q2 = k.s.h2 - r2;
-                    float ww = k.s.wpoly6 * q2 * q2 * q2 * k.s.inv_ap_wq;
-                    sc = -k.s.ap_k * (k.s.ap_n == 4.0f ? pow4(ww) : powf(ww, k.s.ap_n)) * dt2;
+        for_neighbors<4>(
+            i, 0, k, list, cnt, bhash, off, disp2, p, xbuild, [&](uint j) { return xl[j]; },
+            [&](uint, const float4& q4) {
+                float3 r = p - f3(q4);
+                float r2 = dot3(r, r);
+                if (r2 <= k.s.h2 && r2 > 0.0f) {
+                    float ri = rinv_of(r2);
+                    float sc = 0.0f;
+                    if (k.s.ap) {
+                        float q2 = k.s.h2 - r2;
+                        float ww = k.s.wpoly6 * q2 * q2 * q2 * k.s.inv_ap_wq;
+                        sc = -k.s.ap_k * (k.s.ap_n == 4.0f ? pow4(ww) : powf(ww, k.s.ap_n)) * dt2;
+                    }
+                    float q = k.s.h - r2 * ri;
+                    acc += ((li + q4.w + sc) * (k.s.gwspiky * k.s.inv_rho0) * q * q * ri) * r;
                 }
-                float q = k.s.h - r2 * ri;
-                acc += ((li + q4.w + sc) * (k.s.gwspiky * k.s.inv_rho0) * q * q * ri) * r;
-            }
-        });
+            });
         float3 xn = clamp_wall(p + acc, k.s);
         xa[i] = make_float4(xn.x, xn.y, xn.z, 0.0f);
         lamacc[i] += li;  // the multiplier this step has applied so far
@@ -554,15 +649,16 @@ __global__ __launch_bounds__(kTpb) void k_xsph(const float4* __restrict__ xa, co
     float3 p = f3(xa[i]);
     float3 vi = f3(v[i]);
     float3 acc = f3(0, 0, 0);
-    for_neighbors(i, 0, k, list, cnt, bhash, off, disp2, p, xb, [&](uint j) {
-        float3 r = p - f3(xa[j]);
-        float r2 = dot3(r, r);
-        if (r2 <= k.s.h2) {
-            float4 vj = v[j];
-            float q = k.s.h2 - r2;
-            acc += (q * q * q / vj.w) * (f3(vj) - vi);
-        }
-    });
+    for_neighbors<2>(
+        i, 0, k, list, cnt, bhash, off, disp2, p, xb, [&](uint j) { return XV{xa[j], v[j]}; },
+        [&](uint, const XV& q4) {
+            float3 r = p - f3(q4.x);
+            float r2 = dot3(r, r);
+            if (r2 <= k.s.h2) {
+                float q = k.s.h2 - r2;
+                acc += (q * q * q / q4.v.w) * (f3(q4.v) - vi);
+            }
+        });
     float3 vn = vi + (k.xsph_c * k.s.mass * k.s.wpoly6) * acc;
     vout[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
 }
@@ -585,16 +681,18 @@ __global__ __launch_bounds__(kTpb) void k_heat_diffuse(const float4* __restrict_
     float ti = therm[i].x;
     const float inv_h = k.inv_h, ws = 10.0f / (7.0f * 3.14159265358979f * k.s.h2);
     float acc = 0.0f;
-    for_neighbors(i, 0, k, list, cnt, bhash, off, disp2, p, xb, [&](uint j) {
-        float3 r = p - f3(x[j]);
-        float r2 = dot3(r, r);
-        if (r2 <= k.s.h2 && j != i) {
-            float q = sqrtf(r2) * inv_h;
-            float w = q < 1.0f ? ws * (1.0f - 1.5f * q * q + 0.75f * q * q * q) : ws * 0.25f * (2.0f - q) * (2.0f - q) * (2.0f - q);
-            float d = fmaxf(dens[j], 0.1f);
-            acc += hdt * (therm[j].x - ti) / d * w;
-        }
-    });
+    for_neighbors(
+        i, 0, k, list, cnt, bhash, off, disp2, p, xb, [&](uint j) { return x[j]; },
+        [&](uint j, const float4& qj) {
+            float3 r = p - f3(qj);
+            float r2 = dot3(r, r);
+            if (r2 <= k.s.h2 && j != i) {
+                float q = sqrtf(r2) * inv_h;
+                float w = q < 1.0f ? ws * (1.0f - 1.5f * q * q + 0.75f * q * q * q) : ws * 0.25f * (2.0f - q) * (2.0f - q) * (2.0f - q);
+                float d = fmaxf(dens[j], 0.1f);
+                acc += hdt * (therm[j].x - ti) / d * w;
+            }
+        });
     dtemp[i] = acc * td;
 }
 
