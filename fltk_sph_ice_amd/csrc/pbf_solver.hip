@@ -345,9 +345,8 @@ __device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, const 
     float3 gs = f3(0, 0, 0);
     uint c = 0;
     // learned-start extras
-    float nn = 0.0f, wsum = 0.0f, wcs = 0.0f, ring = 0.0f, nring = 0.0f, div = 0.0f;
+    float nn = 0.0f, wsum = 0.0f, wcs = 0.0f, ring = 0.0f, nring = 0.0f;
     const float h2q = 0.25f * k.s.h2;
-    const float3 vi = feat ? f3(v[i]) : f3(0, 0, 0);
     uint4* l4 = (uint4*)list + i;  // chunk q of slot i at l4[q * n]
     uint b0 = 0, b1 = 0, b2 = 0, b3 = 0;  // the open chunk, newest entry in b3
     cands([&](uint j, float4 q4) {  // q4 = (x*_j, density at the step start)
@@ -383,7 +382,6 @@ __device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, const 
                         ring += cj;
                         nring += 1.0f;
                     }
-                    div += dot3(f3(v[j]) - vi, g);
                 }
             }
         }
@@ -419,9 +417,9 @@ __device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, const 
         f[6] = wsum > 0.0f ? wcs / wsum : 0.0f;       // W-weighted mean of the step-start C
         f[7] = nring > 0.0f ? ring / nring : 0.0f;     // mean step-start C of the outer ring (h/2 < r <= h)
         f[8] = p4.w * k.s.inv_rho0 - 1.0f;
-        f[9] = div * k.s.mass * k.dt;
+        f[9] = (p4.w - d) / d;  // div v dt from the continuity equation: -(rho* - rho_start) / rho*
         f[10] = fminf(fmaxf(wall * k.inv_h, 0.0f), 1.0f) - 0.5f;
-        f[11] = vi.y * k.dt * k.inv_spacing;
+        f[11] = v[i].y * k.dt * k.inv_spacing;
         float4* fo = (float4*)(feat + (size_t)i * 16);
         fo[0] = make_float4(f[0], f[1], f[2], f[3]);
         fo[1] = make_float4(f[4], f[5], f[6], f[7]);
