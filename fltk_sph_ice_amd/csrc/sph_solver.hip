@@ -25,11 +25,43 @@ struct SphStepK {
     float dt;
 };
 
+// kLanes lanes per particle: every row of the 27-cell walk is split among the lanes of the particle (strided),
+// each lane fetching its candidates two at a time, and the partial sums are combined with xor shuffles.  At
+// 32k particles this gives kLanes times the work-items of one thread per particle (all CUs busy) and many
+// gathers in flight per particle (8 lanes measured 0.074 ms per step against 0.084 for 4 and 0.21 for 1).
+constexpr int kLanes = 8;
+
 __device__ inline uint slot_xcd(uint n, bool& valid) {
     uint b = xcd_block(blockIdx.x, gridDim.x);
-    uint i = b * blockDim.x + threadIdx.x;
+    uint i = (b * blockDim.x + threadIdx.x) / kLanes;
     valid = i < n;
     return valid ? i : 0;
+}
+
+template <class Fetch, class Body>
+__device__ inline void stencil_split(int3 c, const FGridK& g, const uint* __restrict__ off, int sub, Fetch&& fetch, Body&& body) {
+    int x0 = max(c.x - 1, 0), x1 = min(c.x + 1, g.gx - 1);
+    int y0 = max(c.y - 1, 0), y1 = min(c.y + 1, g.gy - 1);
+    int z0 = max(c.z - 1, 0), z1 = min(c.z + 1, g.gz - 1);
+    for (int z = z0; z <= z1; ++z)
+        for (int y = y0; y <= y1; ++y) {
+            uint hb = (uint)((z * g.gy + y) * g.gx);
+            uint j = off[hb + x0] + (uint)sub, e = off[hb + x1 + 1];
+            for (; j + kLanes < e; j += 2 * kLanes) {
+                auto a = fetch(j);
+                auto b = fetch(j + kLanes);
+                body(j, a);
+                body(j + kLanes, b);
+            }
+            if (j < e) body(j, fetch(j));
+        }
+}
+
+template <class T>
+__device__ inline T lanes_sum(T v) {
+#pragma unroll
+    for (int d = 1; d < kLanes; d <<= 1) v += __shfl_xor(v, d, 64);
+    return v;
 }
 
 // rxSPH::calDensity + pressure (rx_sph_cpu.cpp:495-571)
@@ -37,21 +69,32 @@ __global__ __launch_bounds__(kTpb) void k_density(const float4* __restrict__ x, 
                                                   const uint* __restrict__ off, SphStepK k) {
     bool valid;
     uint i = slot_xcd(k.n, valid);
-    if (!valid) return;
-    float3 p = f3(x[i]);
+    const int sub = threadIdx.x & (kLanes - 1);
     float d = 0.0f;
-    fg_for_stencil(fg_unhash(bhash[i], k.g), k.g, off, [&](uint j) {
-        float3 r = p - f3(x[j]);
-        float r2 = dot3(r, r);
-        if (r2 <= k.s.h2) {
-            float q = k.s.h2 - r2;
-            d += q * q * q;
-        }
-    });
+    if (valid) {
+        float3 p = f3(x[i]);
+        stencil_split(
+            fg_unhash(bhash[i], k.g), k.g, off, sub, [&](uint j) { return x[j]; },
+            [&](uint, const float4& q) {
+                float3 r = p - f3(q);
+                float r2 = dot3(r, r);
+                if (r2 <= k.s.h2) {
+                    float qq = k.s.h2 - r2;
+                    d += qq * qq * qq;
+                }
+            });
+    }
+    d = lanes_sum(d);  // every lane of the particle takes part (valid is uniform over them)
+    if (!valid || sub) return;
     d *= k.s.mass * k.s.wpoly6;
     float pr = k.s.gas_k * (d - k.s.rho0);
     dp[i] = make_float2(d, pr / (d * d));
 }
+
+struct XVD {  // neighbour position, velocity and (rho, p / rho^2)
+    float4 x, v;
+    float2 d;
+};
 
 // rxSPH::calForce + integrate (rx_sph_cpu.cpp:574-614, 1040-1077); writes the new state.
 __global__ __launch_bounds__(kTpb) void k_force_integrate(const float4* __restrict__ x, const float4* __restrict__ v,
@@ -60,21 +103,29 @@ __global__ __launch_bounds__(kTpb) void k_force_integrate(const float4* __restri
                                                           float4* __restrict__ vo, SphStepK k) {
     bool valid;
     uint i = slot_xcd(k.n, valid);
-    if (!valid) return;
-    float3 p = f3(x[i]), vi = f3(v[i]);
-    float prsi = dp[i].y;
+    const int sub = threadIdx.x & (kLanes - 1);
     float3 fp = f3(0, 0, 0), fv = f3(0, 0, 0);
-    fg_for_stencil(fg_unhash(bhash[i], k.g), k.g, off, [&](uint j) {
-        float3 r = p - f3(x[j]);
-        float r2 = dot3(r, r);
-        if (r2 <= k.s.h2 && r2 > 0.0f) {
-            float rr = sqrtf(r2);
-            float2 dj = dp[j];
-            float q = k.s.h - rr;
-            fp += ((prsi + dj.y) * q * q / rr) * r;
-            fv += (q / dj.x) * (f3(v[j]) - vi);
-        }
-    });
+    float3 p = f3(0, 0, 0), vi = f3(0, 0, 0);
+    if (valid) {
+        p = f3(x[i]);
+        vi = f3(v[i]);
+        const float prsi = dp[i].y;
+        stencil_split(
+            fg_unhash(bhash[i], k.g), k.g, off, sub, [&](uint j) { return XVD{x[j], v[j], dp[j]}; },
+            [&](uint, const XVD& q) {
+                float3 r = p - f3(q.x);
+                float r2 = dot3(r, r);
+                if (r2 <= k.s.h2 && r2 > 0.0f) {
+                    float rr = sqrtf(r2);
+                    float qq = k.s.h - rr;
+                    fp += ((prsi + q.d.y) * qq * qq / rr) * r;
+                    fv += (qq / q.d.x) * (f3(q.v) - vi);
+                }
+            });
+    }
+    fp = f3(lanes_sum(fp.x), lanes_sum(fp.y), lanes_sum(fp.z));
+    fv = f3(lanes_sum(fv.x), lanes_sum(fv.y), lanes_sum(fv.z));
+    if (!valid || sub) return;
     float3 a = (-k.s.mass * k.s.gwspiky) * fp + (k.s.visc * k.s.mass * k.s.lwvisc) * fv + k.s.gravity;
     float3 vn = vi + k.dt * a;
     float3 xn = p + k.dt * vn;
@@ -171,7 +222,7 @@ public:
         fg_gather<float4>(m_vw, m_v, m_grid.perm, m_n, s);
         fg_gather<uint>(m_idw, m_id, m_grid.perm, m_n, s);
         RX_HIP(hipMemcpyAsync(m_id, m_idw, sizeof(uint) * m_n, hipMemcpyDeviceToDevice, s));
-        unsigned nb = rx_blocks(m_n, kTpb);
+        unsigned nb = rx_blocks(m_n * kLanes, kTpb);
         k_density<<<nb, kTpb, 0, s>>>(m_xw, m_dp, m_grid.hash, m_grid.off, k);
         k_force_integrate<<<nb, kTpb, 0, s>>>(m_xw, m_vw, m_dp, m_grid.hash, m_grid.off, m_x, m_v, k);
     }
