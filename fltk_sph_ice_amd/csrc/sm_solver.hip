@@ -155,11 +155,10 @@ __device__ inline void sm_cluster(int c, bool valid, const int* __restrict__ cs,
     float acc[NA];
 #pragma unroll
     for (int t = 0; t < NA; ++t) acc[t] = 0.0f;
-    for (int t = s + sub; t < e; t += T) {
-        int m = mem[t];
-        if (Masked && !act[m]) continue;
-        float3 d = f3(xs[m]) - xref;
-        float3 q = f3(rest[m]) - cm0;
+    auto member = [&](bool on, const float4& xm, const float4& rm) {
+        if (Masked && !on) return;
+        float3 d = f3(xm) - xref;
+        float3 q = f3(rm) - cm0;
         acc[0] += d.x; acc[1] += d.y; acc[2] += d.z;
         acc[3] += q.x; acc[4] += q.y; acc[5] += q.z;
         acc[6] += d.x * q.x; acc[7] += d.x * q.y; acc[8] += d.x * q.z;
@@ -170,6 +169,18 @@ __device__ inline void sm_cluster(int c, bool valid, const int* __restrict__ cs,
             acc[16 % NA] += q.x * q.x; acc[17 % NA] += q.x * q.y; acc[18 % NA] += q.x * q.z;
             acc[19 % NA] += q.y * q.y; acc[20 % NA] += q.y * q.z; acc[21 % NA] += q.z * q.z;
         }
+    };
+    int t = s + sub;
+    for (; t + T < e; t += 2 * T) {  // two members in flight per lane
+        const int m0 = mem[t], m1 = mem[t + T];
+        const float4 x0 = xs[m0], r0 = rest[m0], x1 = xs[m1], r1 = rest[m1];
+        const bool a0 = Masked ? act[m0] != 0 : true, a1 = Masked ? act[m1] != 0 : true;
+        member(a0, x0, r0);
+        member(a1, x1, r1);
+    }
+    if (t < e) {
+        const int m = mem[t];
+        member(Masked ? act[m] != 0 : true, xs[m], rest[m]);
     }
 #pragma unroll
     for (int off = T / 2; off > 0; off >>= 1)
@@ -304,12 +315,26 @@ __global__ void k_sm_average(const int* __restrict__ ps, const int* __restrict__
     float3 r = f3(rest[i]);
     float3 acc = f3(0, 0, 0);
     int cnt = 0;
-    for (int t = s; t < e; ++t) {
-        int c = pc[t];
-        if (k.masked && !act[c]) continue;
-        float4 a = tr[3 * c], b = tr[3 * c + 1], d = tr[3 * c + 2];
+    auto add = [&](int c, const float4& a, const float4& b, const float4& d) {
+        if (k.masked && !act[c]) return;
         acc += f3(a.x * r.x + a.y * r.y + a.z * r.z + a.w, b.x * r.x + b.y * r.y + b.z * r.z + b.w, d.x * r.x + d.y * r.y + d.z * r.z + d.w);
         ++cnt;
+    };
+    int t = s;
+    for (; t + 3 < e; t += 4) {  // four clusters' transforms in flight before any is used
+        const int c0 = pc[t], c1 = pc[t + 1], c2 = pc[t + 2], c3 = pc[t + 3];
+        const float4 a0 = tr[3 * c0], b0 = tr[3 * c0 + 1], d0 = tr[3 * c0 + 2];
+        const float4 a1 = tr[3 * c1], b1 = tr[3 * c1 + 1], d1 = tr[3 * c1 + 2];
+        const float4 a2 = tr[3 * c2], b2 = tr[3 * c2 + 1], d2 = tr[3 * c2 + 2];
+        const float4 a3 = tr[3 * c3], b3 = tr[3 * c3 + 1], d3 = tr[3 * c3 + 2];
+        add(c0, a0, b0, d0);
+        add(c1, a1, b1, d1);
+        add(c2, a2, b2, d2);
+        add(c3, a3, b3, d3);
+    }
+    for (; t < e; ++t) {
+        const int c = pc[t];
+        add(c, tr[3 * c], tr[3 * c + 1], tr[3 * c + 2]);
     }
     if (cnt > 0) x = x + k.alpha * ((1.0f / (float)cnt) * acc - x);
     xo[i] = make_float4(x.x, x.y, x.z, 0.0f);
