@@ -151,6 +151,7 @@ SIGNATURES: dict[str, tuple] = {
     "CuGetLastErrorCode": (i32, []),
     "CuGetLastErrorString": (C.c_char_p, []),
     "CuClearLastError": (None, []),
+    "CuGetStream": (C.c_void_p, []),
     "CuPbdSphDensityErrorStats": (None, [vp, flt, u32, fp, fp]),
     # rx_sph_host.h
     "rxSphEnvDefault": (None, [C.POINTER(rxSphEnv)]),

@@ -65,43 +65,67 @@ def sph_step(scene, pos4: np.ndarray, vel4: np.ndarray, steps: int = 1):
             dens.download(np.float32))
 
 
-def pbf_step(scene, pos4: np.ndarray, vel4: np.ndarray, steps: int = 1):
+class PbfAbi:
     """PBF steps through the ABI in the order of rxPBDSPH_GPU::Update with a fixed iteration count and the
-    XSPH pass of the workload; returns (pos4, vel4, dens, max_err per iteration of the last step)."""
-    lib = load()
-    sp = scene.params()
-    n = pos4.shape[0]
-    lib.CuSetParameters(C.byref(sp))
-    g = GridBuffers(n, sp.NumCells, lib)
-    dpos, dvel = DevBuf.from_array(pos4), DevBuf.from_array(vel4)
-    dens, frc = DevBuf(4 * n), DevBuf(16 * n)
-    ppos, pvel = DevBuf(16 * n), DevBuf(16 * n)
-    scl, dp = DevBuf(4 * n), DevBuf(16 * n)
-    errs = []
-    mx, mean = C.c_float(), C.c_float()
-    for _ in range(steps):
-        errs = []
+    XSPH pass of the workload.  Buffers are allocated once; `step()` issues the calls of one step."""
+
+    def __init__(self, scene, pos4: np.ndarray, vel4: np.ndarray):
+        self.lib = lib = load()
+        self.scene = scene
+        self.sp = sp = scene.params()
+        self.n = n = pos4.shape[0]
+        lib.CuSetParameters(C.byref(sp))
+        self.g = GridBuffers(n, sp.NumCells, lib)
+        self.dpos, self.dvel = DevBuf.from_array(pos4), DevBuf.from_array(vel4)
+        self.dens, self.frc = DevBuf(4 * n), DevBuf(16 * n)
+        self.ppos, self.pvel = DevBuf(16 * n), DevBuf(16 * n)
+        self.scl, self.dp = DevBuf(4 * n), DevBuf(16 * n)
+        self.errs: list[float] = []
+
+    def step(self, stats: bool = True, timer=None):
+        """One step.  With `stats` the maximum density error of every iteration is read back to the host as the
+        reference loop does (CuPbdSphDensityErrorStats synchronises); without it the iterations run unjudged.
+        `timer`, when given, is called as timer(name, fn, *args) for the lambda and position-correction calls
+        (to bracket them with events on the library stream)."""
+        run = timer or (lambda _name, fn, *a: fn(*a))
+        lib, sc, g, n = self.lib, self.scene, self.g, self.n
+        dpos, dvel, dens, frc = self.dpos, self.dvel, self.dens, self.frc
+        ppos, pvel, scl, dp = self.ppos, self.pvel, self.scl, self.dp
+        mx, mean = C.c_float(), C.c_float()
+        self.errs = []
         c = g.set_object_to_cell(dpos.ptr, dvel.ptr)
         lib.CuPbdSphDensity(dens.ptr, c)
-        lib.CuPbdSphExternalForces(dens.ptr, frc.ptr, c, scene.dt)
-        lib.CuPbdSphIntegrate(dpos.ptr, dvel.ptr, frc.ptr, None, ppos.ptr, pvel.ptr, scene.dt, n)
-        for _it in range(scene.iterations):
+        lib.CuPbdSphExternalForces(dens.ptr, frc.ptr, c, sc.dt)
+        lib.CuPbdSphIntegrate(dpos.ptr, dvel.ptr, frc.ptr, None, ppos.ptr, pvel.ptr, sc.dt, n)
+        for _it in range(sc.iterations):
             c = g.set_object_to_cell(ppos.ptr, pvel.ptr)
-            lib.CuPbdSphScalingFactor(ppos.ptr, dens.ptr, scl.ptr, scene.epsilon, c)
-            lib.CuPbdSphDensityErrorStats(dens.ptr, sp.Density, n, C.byref(mx), C.byref(mean))
-            errs.append(mx.value)
-            lib.CuPbdSphPositionCorrection(ppos.ptr, scl.ptr, dp.ptr, c)
+            run("lambda", lib.CuPbdSphScalingFactor, ppos.ptr, dens.ptr, scl.ptr, sc.epsilon, c)
+            if stats:
+                lib.CuPbdSphDensityErrorStats(dens.ptr, self.sp.Density, n, C.byref(mx), C.byref(mean))
+                self.errs.append(mx.value)
+            run("dp", lib.CuPbdSphPositionCorrection, ppos.ptr, scl.ptr, dp.ptr, c)
             lib.CuPbdSphCorrectPosition(ppos.ptr, dp.ptr, n)
             pvel.fill(0)
             frc.fill(0)
-            lib.CuPbdSphIntegrate(ppos.ptr, pvel.ptr, frc.ptr, None, ppos.ptr, pvel.ptr, scene.dt, n)
-        lib.CuPbdSphUpdateVelocity(ppos.ptr, dpos.ptr, pvel.ptr, scene.dt, n)
+            lib.CuPbdSphIntegrate(ppos.ptr, pvel.ptr, frc.ptr, None, ppos.ptr, pvel.ptr, sc.dt, n)
+        lib.CuPbdSphUpdateVelocity(ppos.ptr, dpos.ptr, pvel.ptr, sc.dt, n)
         lib.CuCopyArrayD2D(dpos.ptr, ppos.ptr, 16 * n)
-        if scene.xsph_c != 0.0:
+        if sc.xsph_c != 0.0:
             c = g.set_object_to_cell(dpos.ptr, pvel.ptr)
-            lib.CuXSphViscosity(dpos.ptr, pvel.ptr, dvel.ptr, dens.ptr, scene.xsph_c, c)
+            lib.CuXSphViscosity(dpos.ptr, pvel.ptr, dvel.ptr, dens.ptr, sc.xsph_c, c)
         else:
             lib.CuCopyArrayD2D(dvel.ptr, pvel.ptr, 16 * n)
         check(lib)
-    return (dpos.download(np.float32).reshape(n, 4), dvel.download(np.float32).reshape(n, 4),
-            dens.download(np.float32), errs)
+
+    def arrays(self):
+        n = self.n
+        return (self.dpos.download(np.float32).reshape(n, 4), self.dvel.download(np.float32).reshape(n, 4),
+                self.dens.download(np.float32))
+
+
+def pbf_step(scene, pos4: np.ndarray, vel4: np.ndarray, steps: int = 1):
+    """PBF steps through the ABI (PbfAbi); returns (pos4, vel4, dens, max_err per iteration of the last step)."""
+    s = PbfAbi(scene, pos4, vel4)
+    for _ in range(steps):
+        s.step()
+    return (*s.arrays(), s.errs)

@@ -314,6 +314,8 @@ extern "C" {
 
 int CuGetLastErrorCode(void) { return g_err.code; }
 const char* CuGetLastErrorString(void) { return g_err.msg; }
+void* CuGetStream(void) { return (void*)rx_stream(); }
+
 void CuClearLastError(void) {
     g_err.code = 0;
     g_err.msg[0] = 0;

@@ -15,19 +15,48 @@ namespace {
 constexpr int kTpb = 256;
 
 // Visits the slots of all particles in the cells overlapped by [p - h, p + h] (the cell range of
-// rx_sph_kernel.cu:266-278), row by row.
-template <class F>
-__device__ inline void for_each_candidate(float3 p, const SphK& k, const uint* __restrict__ cs, const uint* __restrict__ ce, F&& f) {
+// rx_sph_kernel.cu:266-278), row by row and in slot order.  Candidates go in batches of four: `load(j)` for all
+// four (the position, and the original index where the pass needs it), then `more(j, a)` for all four (the
+// loads that only pairs within range need, issued under that test), then `body(j, a, b)` in slot order.  Every
+// load of a stage is in flight before the first of them is used, instead of one candidate at a time.
+struct Nothing {};
+__device__ inline Nothing no_more(uint, const float4&) { return {}; }
+
+template <class Load, class More, class Body>
+__device__ inline void for_each_candidate(float3 p, const SphK& k, const uint* __restrict__ cs, const uint* __restrict__ ce,
+                                          Load&& load, More&& more, Body&& body) {
     int3 g0 = grid_pos(f3(p.x - k.h, p.y - k.h, p.z - k.h), k);
     int3 g1 = grid_pos(f3(p.x + k.h, p.y + k.h, p.z + k.h), k);
     int nx = g1.x - g0.x + 1;
     for (int z = g0.z; z <= g1.z; ++z)
         for (int y = g0.y; y <= g1.y; ++y) {
-            uint s, e;
-            row_range(cs, ce, grid_hash(make_int3(g0.x, y, z), k), nx, s, e);
-            for (uint j = s; j < e; ++j) f(j);
+            uint j, e;
+            row_range(cs, ce, grid_hash(make_int3(g0.x, y, z), k), nx, j, e);
+            for (; j + 3 < e; j += 4) {
+                auto a0 = load(j);
+                auto a1 = load(j + 1);
+                auto a2 = load(j + 2);
+                auto a3 = load(j + 3);
+                auto b0 = more(j, a0);
+                auto b1 = more(j + 1, a1);
+                auto b2 = more(j + 2, a2);
+                auto b3 = more(j + 3, a3);
+                body(j, a0, b0);
+                body(j + 1, a1, b1);
+                body(j + 2, a2, b2);
+                body(j + 3, a3, b3);
+            }
+            for (; j < e; ++j) {
+                auto a = load(j);
+                body(j, a, more(j, a));
+            }
         }
 }
+
+struct PosIdx {  // sorted position of a candidate and its original index
+    float4 x;
+    uint o;
+};
 
 __device__ inline float pow_n(float x, float n) { return (n == 4.0f) ? (x * x) * (x * x) : powf(x, n); }
 
@@ -38,14 +67,16 @@ __global__ __launch_bounds__(kTpb) void k_sph_density(float* __restrict__ dens, 
     if (i >= c.uNumParticles) return;
     float3 p = f3(c.dSortedPos[i]);
     float d = 0.0f;
-    for_each_candidate(p, k, c.dCellStart, c.dCellEnd, [&](uint j) {
-        float3 r = p - f3(c.dSortedPos[j]);
-        float r2 = dot3(r, r);
-        if (r2 <= k.h2) {
-            float q = k.h2 - r2;
-            d += k.mass * k.wpoly6 * q * q * q;
-        }
-    });
+    for_each_candidate(
+        p, k, c.dCellStart, c.dCellEnd, [&](uint j) { return c.dSortedPos[j]; }, no_more,
+        [&](uint, const float4& x, Nothing) {
+            float3 r = p - f3(x);
+            float r2 = dot3(r, r);
+            if (r2 <= k.h2) {
+                float q = k.h2 - r2;
+                d += k.mass * k.wpoly6 * q * q * q;
+            }
+        });
     uint o = c.dSortedIndex[i];
     dens[o] = d;
     if (write_pres) pres[o] = k.gas_k * (d - k.rho0);
@@ -62,20 +93,34 @@ __global__ __launch_bounds__(kTpb) void k_sph_force(const float* __restrict__ de
     float d0 = dens[o];
     float prsi = pres[o] / (d0 * d0);
     float3 fp = f3(0, 0, 0), fv = f3(0, 0, 0);
-    for_each_candidate(p0, k, c.dCellStart, c.dCellEnd, [&](uint j) {
-        if (j == i) return;
-        float3 rij = p0 - f3(c.dSortedPos[j]);
-        float r2 = dot3(rij, rij);
-        if (r2 <= k.h2 && r2 > 0.0f) {
-            float r = sqrtf(r2);
-            uint oj = c.dSortedIndex[j];
-            float dj = dens[oj];
-            float prsj = pres[oj] / (dj * dj);
+    struct Pair {
+        bool in;
+        float4 v;
+        float d, pr;
+    };
+    for_each_candidate(
+        p0, k, c.dCellStart, c.dCellEnd, [&](uint j) { return PosIdx{c.dSortedPos[j], c.dSortedIndex[j]}; },
+        [&](uint j, const PosIdx& a) {
+            Pair b{};
+            float3 rij = p0 - f3(a.x);
+            float r2 = dot3(rij, rij);
+            b.in = j != i && r2 <= k.h2 && r2 > 0.0f;
+            if (b.in) {
+                b.v = c.dSortedVel[j];
+                b.d = dens[a.o];
+                b.pr = pres[a.o];
+            }
+            return b;
+        },
+        [&](uint, const PosIdx& a, const Pair& b) {
+            if (!b.in) return;
+            float3 rij = p0 - f3(a.x);
+            float r = sqrtf(dot3(rij, rij));
+            float prsj = b.pr / (b.d * b.d);
             float q = k.h - r;
             fp += (k.mass * (prsi + prsj) * k.gwspiky * q * q / r) * rij;
-            fv += (k.mass / dj * k.lwvisc * q) * (f3(c.dSortedVel[j]) - v0);
-        }
-    });
+            fv += (k.mass / b.d * k.lwvisc * q) * (f3(b.v) - v0);
+        });
     float3 a = (-1.0f) * fp + k.visc * fv + k.gravity;
     frc[o] = make_float4(a.x, a.y, a.z, 0.0f);
 }
@@ -86,17 +131,26 @@ __global__ __launch_bounds__(kTpb) void k_sph_normal(float4* __restrict__ nrm, c
     if (i >= c.uNumParticles) return;
     float3 p0 = f3(c.dSortedPos[i]);
     float3 n = f3(0, 0, 0);
-    for_each_candidate(p0, k, c.dCellStart, c.dCellEnd, [&](uint j) {
-        if (j == i) return;
-        float3 rij = p0 - f3(c.dSortedPos[j]);
-        float r2 = dot3(rij, rij);
-        float r = sqrtf(r2);
-        if (r <= k.h && r > 0.0001f) {
-            float d1 = dens[c.dSortedIndex[j]];
-            float q = k.h2 - r2;
-            n += (k.mass / d1 * k.gwpoly6 * q * q) * rij;
-        }
-    });
+    struct Pair {
+        bool in;
+        float d;
+    };
+    for_each_candidate(
+        p0, k, c.dCellStart, c.dCellEnd, [&](uint j) { return PosIdx{c.dSortedPos[j], c.dSortedIndex[j]}; },
+        [&](uint j, const PosIdx& a) {
+            Pair b{};
+            float3 rij = p0 - f3(a.x);
+            float r = sqrtf(dot3(rij, rij));
+            b.in = j != i && r <= k.h && r > 0.0001f;
+            if (b.in) b.d = dens[a.o];
+            return b;
+        },
+        [&](uint, const PosIdx& a, const Pair& b) {
+            if (!b.in) return;
+            float3 rij = p0 - f3(a.x);
+            float q = k.h2 - dot3(rij, rij);
+            n += (k.mass / b.d * k.gwpoly6 * q * q) * rij;
+        });
     float l = sqrtf(dot3(n, n));
     if (l > 0.0f) n = (1.0f / l) * n;
     nrm[c.dSortedIndex[i]] = make_float4(n.x, n.y, n.z, 0.0f);
@@ -217,15 +271,26 @@ __global__ __launch_bounds__(kTpb) void k_pbf_ext_force(const float* __restrict_
     uint o = c.dSortedIndex[i];
     float d0 = dens[o];
     float3 fv = f3(0, 0, 0);
-    for_each_candidate(p0, k, c.dCellStart, c.dCellEnd, [&](uint j) {
-        if (j == i) return;
-        float3 rij = p0 - f3(c.dSortedPos[j]);
-        float r2 = dot3(rij, rij);
-        if (r2 <= k.h2 && r2 > 0.0f) {
-            float r = sqrtf(r2);
-            fv += (k.lwvisc * (k.h - r)) * (f3(c.dSortedVel[j]) - v0);
-        }
-    });
+    struct Pair {
+        bool in;
+        float4 v;
+    };
+    for_each_candidate(
+        p0, k, c.dCellStart, c.dCellEnd, [&](uint j) { return c.dSortedPos[j]; },
+        [&](uint j, const float4& x) {
+            Pair b{};
+            float3 rij = p0 - f3(x);
+            float r2 = dot3(rij, rij);
+            b.in = j != i && r2 <= k.h2 && r2 > 0.0f;
+            if (b.in) b.v = c.dSortedVel[j];
+            return b;
+        },
+        [&](uint, const float4& x, const Pair& b) {
+            if (!b.in) return;
+            float3 rij = p0 - f3(x);
+            float r = sqrtf(dot3(rij, rij));
+            fv += (k.lwvisc * (k.h - r)) * (f3(b.v) - v0);
+        });
     // rx_sph_pbd.cpp:676: viscosity uses the particle's own density
     float3 a = (k.visc * k.mass / d0) * fv + k.gravity;
     frc[o] = make_float4(a.x, a.y, a.z, 0.0f);
@@ -239,21 +304,23 @@ __global__ __launch_bounds__(kTpb) void k_pbf_lambda(float* __restrict__ dens, f
     float3 p = f3(c.dSortedPos[i]);
     float d = 0.0f, sd = 0.0f;
     float3 gs = f3(0, 0, 0);
-    for_each_candidate(p, k, c.dCellStart, c.dCellEnd, [&](uint j) {
-        float3 rij = p - f3(c.dSortedPos[j]);
-        float r2 = dot3(rij, rij);
-        if (r2 <= k.h2) {
-            float q2 = k.h2 - r2;
-            d += k.mass * k.wpoly6 * q2 * q2 * q2;
-            if (r2 > 0.0f) {
-                float r = sqrtf(r2);
-                float q = k.h - r;
-                float3 g = (k.gwspiky * q * q / r * k.inv_rho0) * rij;
-                gs += g;
-                sd += dot3(g, g);
+    for_each_candidate(
+        p, k, c.dCellStart, c.dCellEnd, [&](uint j) { return c.dSortedPos[j]; }, no_more,
+        [&](uint, const float4& x, Nothing) {
+            float3 rij = p - f3(x);
+            float r2 = dot3(rij, rij);
+            if (r2 <= k.h2) {
+                float q2 = k.h2 - r2;
+                d += k.mass * k.wpoly6 * q2 * q2 * q2;
+                if (r2 > 0.0f) {
+                    float r = sqrtf(r2);
+                    float q = k.h - r;
+                    float3 g = (k.gwspiky * q * q / r * k.inv_rho0) * rij;
+                    gs += g;
+                    sd += dot3(g, g);
+                }
             }
-        }
-    });
+        });
     sd += dot3(gs, gs);
     float C = d * k.inv_rho0 - 1.0f;
     uint o = c.dSortedIndex[i];
@@ -269,10 +336,24 @@ __global__ __launch_bounds__(kTpb) void k_pbf_dp(const float* __restrict__ scl, 
     float si = scl[c.dSortedIndex[i]];
     float dt2 = k.dt * k.dt;
     float3 acc = f3(0, 0, 0);
-    for_each_candidate(p, k, c.dCellStart, c.dCellEnd, [&](uint j) {
-        float3 rij = p - f3(c.dSortedPos[j]);
-        float r2 = dot3(rij, rij);
-        if (r2 <= k.h2 && r2 > 0.0f) {
+    struct Pair {
+        bool in;
+        float s;
+    };
+    for_each_candidate(
+        p, k, c.dCellStart, c.dCellEnd, [&](uint j) { return PosIdx{c.dSortedPos[j], c.dSortedIndex[j]}; },
+        [&](uint, const PosIdx& a) {
+            Pair b{};
+            float3 rij = p - f3(a.x);
+            float r2 = dot3(rij, rij);
+            b.in = r2 <= k.h2 && r2 > 0.0f;
+            if (b.in) b.s = scl[a.o];
+            return b;
+        },
+        [&](uint, const PosIdx& a, const Pair& b) {
+            if (!b.in) return;
+            float3 rij = p - f3(a.x);
+            float r2 = dot3(rij, rij);
             float r = sqrtf(r2);
             float sc = 0.0f;
             if (k.ap) {
@@ -281,10 +362,8 @@ __global__ __launch_bounds__(kTpb) void k_pbf_dp(const float* __restrict__ scl, 
                 sc = -k.ap_k * pow_n(ww, k.ap_n) * dt2;
             }
             float q = k.h - r;
-            float sj = scl[c.dSortedIndex[j]];
-            acc += ((si + sj + sc) * k.gwspiky * q * q / r * k.inv_rho0) * rij;
-        }
-    });
+            acc += ((si + b.s + sc) * k.gwspiky * q * q / r * k.inv_rho0) * rij;
+        });
     dp[c.dSortedIndex[i]] = make_float4(acc.x, acc.y, acc.z, 0.0f);
 }
 
@@ -321,15 +400,29 @@ __global__ __launch_bounds__(kTpb) void k_xsph(const float4* __restrict__ vel, f
     uint o = c.dSortedIndex[i];
     float3 v0 = f3(vel[o]);
     float3 acc = f3(0, 0, 0);
-    for_each_candidate(p, k, c.dCellStart, c.dCellEnd, [&](uint j) {
-        float3 rij = p - f3(c.dSortedPos[j]);
-        float r2 = dot3(rij, rij);
-        if (r2 <= k.h2) {
-            uint oj = c.dSortedIndex[j];
-            float q = k.h2 - r2;
-            acc += (k.mass / dens[oj] * k.wpoly6 * q * q * q) * (f3(vel[oj]) - v0);
-        }
-    });
+    struct Pair {
+        bool in;
+        float d;
+        float4 v;
+    };
+    for_each_candidate(
+        p, k, c.dCellStart, c.dCellEnd, [&](uint j) { return PosIdx{c.dSortedPos[j], c.dSortedIndex[j]}; },
+        [&](uint, const PosIdx& a) {
+            Pair b{};
+            float3 rij = p - f3(a.x);
+            b.in = dot3(rij, rij) <= k.h2;
+            if (b.in) {
+                b.d = dens[a.o];
+                b.v = vel[a.o];
+            }
+            return b;
+        },
+        [&](uint, const PosIdx& a, const Pair& b) {
+            if (!b.in) return;
+            float3 rij = p - f3(a.x);
+            float q = k.h2 - dot3(rij, rij);
+            acc += (k.mass / b.d * k.wpoly6 * q * q * q) * (f3(b.v) - v0);
+        });
     float3 v = v0 + cx * acc;
     nvel[o] = make_float4(v.x, v.y, v.z, 0.0f);
 }

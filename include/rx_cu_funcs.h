@@ -122,6 +122,9 @@ void CuPbdSphNormal(float* dNrms, float* dDens, rxParticleCell cell);           
 int CuGetLastErrorCode(void);
 const char* CuGetLastErrorString(void);
 void CuClearLastError(void);
+/* The HIP stream (hipStream_t, non-blocking) on which every entry point of this header runs its work, so that
+ * a caller can order or time its own work against it (events, stream waits). */
+void* CuGetStream(void);
 /* Max over particles of max(rho - rho0, 0)/rho0 and mean of |rho - rho0|/rho0, reduced on the device. */
 void CuPbdSphDensityErrorStats(float* dDens, float rest_dens, uint nprts, float* host_max_pos, float* host_mean_abs);
 
