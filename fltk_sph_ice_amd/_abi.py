@@ -133,6 +133,12 @@ SIGNATURES: dict[str, tuple] = {
     "CuSphIntegrate": (None, [vp, vp, vp, vp, vp, flt, u32]),
     "CuSphCheckDelete": (None, [vp, vp, vp, fp, fp, fp, u32]),
     "CuSphCheckDeleteX": (None, [vp, vp, vp, flt, fp, u32]),
+    "CuSphBoundaryVolume": (None, [vp, flt, rxParticleCell]),
+    "CuSphBoundaryDensity": (None, [vp, vp, vp, vp, rxParticleCell, u32]),
+    "CuSphBoundaryForces": (None, [vp, vp, vp, vp, vp, rxParticleCell, u32]),
+    "CuPbdSphBoundaryDensity": (None, [vp, vp, vp, rxParticleCell, u32]),
+    "CuPbdSphScalingFactorWithBoundary": (None, [vp, vp, vp, flt, rxParticleCell, vp, vp, rxParticleCell]),
+    "CuPbdSphPositionCorrectionWithBoundary": (None, [vp, vp, vp, rxParticleCell, vp, vp, rxParticleCell]),
     "CuSphDetectNeighborParticles": (None, [vp, rxParticleCell, flt, i32]),
     "CuSphDetectSurfaceParticles": (None, [vp, vp, rxParticleCell, vp, flt]),
     "CuPbdSphDensity": (None, [vp, rxParticleCell]),

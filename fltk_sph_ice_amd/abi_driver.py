@@ -10,7 +10,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._abi import check, load, rxParticleCell
+from ._abi import check, float3, load, rxParticleCell, uint3
 from .device import DevBuf
 
 
@@ -44,6 +44,43 @@ class GridBuffers:
         lib.CuReorderDataAndFindCellStart(c, dpos, dvel)
         check(lib)
         return c
+
+
+def set_boundary_grid(sp, box_min, box_max, h: float) -> None:
+    """Boundary-particle grid geometry of rxSPH_GPU::InitBoundary (rx_sph_gpu.cpp:495-511): the box expanded by
+    4 particle radii, cell = h (the build's cell rule), GridSizeB = int(extent / h) + 1 per axis."""
+    r = np.float32(sp.ParticleRadius)
+    cells = 1
+    for k in range(3):
+        lo = np.float32(box_min[k]) - np.float32(4.0) * r
+        hi = np.float32(box_max[k]) + np.float32(4.0) * r
+        n = int((hi - lo) / np.float32(h)) + 1
+        sp.WorldOriginB[k], sp.WorldMaxB[k], sp.CellWidthB[k] = float(lo), float(hi), float(np.float32(h))
+        sp.GridSizeB[k] = n
+        cells *= n
+    sp.NumCellsB = cells
+
+
+class BoundaryGrid:
+    """Boundary particles through the ABI in the order of rxSPH_GPU::InitBoundary (rx_sph_gpu.cpp:482-572): hash
+    (CuCalcHashB), sort, reorder + cell bounds (CuReorderDataAndFindCellStartB), then the volumes
+    (CuSphBoundaryVolume).  `sp` must carry the boundary grid (set_boundary_grid) and be the parameters set
+    with CuSetParameters."""
+
+    def __init__(self, sp, bpos4: np.ndarray, lib=None):
+        self.lib = lib = lib or load()
+        self.n = n = bpos4.shape[0]
+        self.g = GridBuffers(n, sp.NumCellsB, lib)
+        self.dpos = DevBuf.from_array(np.ascontiguousarray(bpos4, np.float32))
+        self.vol = DevBuf(4 * n)
+        self.scl = DevBuf(4 * n)
+        lib.CuCalcHashB(self.g.hash.ptr, self.g.index.ptr, self.dpos.ptr, float3(*sp.WorldOriginB), float3(*sp.CellWidthB),
+                        uint3(*sp.GridSizeB), n)
+        lib.CuSort(self.g.hash.ptr, self.g.index.ptr, n)
+        self.cell = self.g.cell()
+        lib.CuReorderDataAndFindCellStartB(self.cell, self.dpos.ptr)
+        lib.CuSphBoundaryVolume(self.vol.ptr, sp.Mass, self.cell)
+        check(lib)
 
 
 def sph_step(scene, pos4: np.ndarray, vel4: np.ndarray, steps: int = 1):

@@ -421,7 +421,8 @@ void CuSort(unsigned int* dHash, uint* dIndex, uint numParticles) {
         return;
     }
     hipStream_t s = rx_stream();
-    uint n = numParticles, nc = g_params.NumCells;
+    // keys are cell hashes of the fluid grid or of the boundary-particle grid (CuCalcHashB)
+    uint n = numParticles, nc = std::max(g_params.NumCells, g_params.NumCellsB);
     size_t nb = (size_t)nc + 1;
     uint* cnt = (uint*)rx_scratch(0, sizeof(uint) * (nb + 1));
     uint* aux = (uint*)rx_scratch(1, sizeof(uint) * (size_t)n * 7 + 64);

@@ -505,9 +505,300 @@ __global__ __launch_bounds__(256) void k_err_final(const float* __restrict__ pma
     }
 }
 
+
+// ------------------------------------------------------------------------------------------ boundary particles
+// Boundary particles (Akinci et al. 2012, "Versatile rigid-fluid coupling for incompressible SPH") sit in a
+// second sorted grid whose geometry is rxSimParams::WorldOriginB / CellWidthB / GridSizeB
+// (calcGridPosB/calcGridHashB, rx_cu_common.cu:155-178).  Each carries a volume V_b; it contributes
+// rho0 V_b W to a fluid density and acts on the fluid as a particle of mass rho0 V_b.
+SphK bgrid_consts(const rxSimParams& p) {
+    SphK k = rx_make_consts(p);
+    k.origin = p.WorldOriginB;
+    k.cellw = p.CellWidthB;
+    k.gx = (int)p.GridSizeB.x;
+    k.gy = (int)p.GridSizeB.y;
+    k.gz = (int)p.GridSizeB.z;
+    k.ncells = p.NumCellsB;
+    return k;
+}
+
+// rx_sph_kernel.cu:500-564 (sphCalBoundaryVolume): V_b = m / sum_{b'} m W(r_bb') over boundary particles,
+// itself included (rx_sph_cpu.cpp:667-693).
+__global__ __launch_bounds__(kTpb) void k_boundary_volume(float* __restrict__ vol, rxParticleCell b, SphK kb) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.uNumParticles) return;
+    float3 p = f3(b.dSortedPos[i]);
+    float mw = 0.0f;
+    for_each_candidate(
+        p, kb, b.dCellStart, b.dCellEnd, [&](uint j) { return b.dSortedPos[j]; }, no_more,
+        [&](uint, const float4& x, Nothing) {
+            float3 r = p - f3(x);
+            float r2 = dot3(r, r);
+            if (r2 <= kb.h2) {
+                float q = kb.h2 - r2;
+                mw += kb.mass * kb.wpoly6 * q * q * q;
+            }
+        });
+    vol[b.dSortedIndex[i]] = kb.mass / mw;
+}
+
+// Boundary term of a fluid density at p: sum_b rho0 V_b W(r).
+__device__ inline float boundary_density(float3 p, const float* __restrict__ vol, const rxParticleCell& b, const SphK& kb) {
+    struct Pair {
+        bool in;
+        float v;
+    };
+    float d = 0.0f;
+    for_each_candidate(
+        p, kb, b.dCellStart, b.dCellEnd, [&](uint j) { return PosIdx{b.dSortedPos[j], b.dSortedIndex[j]}; },
+        [&](uint, const PosIdx& a) {
+            Pair q{};
+            float3 r = p - f3(a.x);
+            q.in = dot3(r, r) <= kb.h2;
+            if (q.in) q.v = vol[a.o];
+            return q;
+        },
+        [&](uint, const PosIdx& a, const Pair& q) {
+            if (!q.in) return;
+            float3 r = p - f3(a.x);
+            float w = kb.h2 - dot3(r, r);
+            d += kb.rho0 * q.v * kb.wpoly6 * w * w * w;
+        });
+    return d;
+}
+
+// rx_sph_kernel.cu:575-650 (sphCalBoundaryDensity): rho_i += boundary term, p_i = k (rho_i - rho0); original order.
+__global__ __launch_bounds__(kTpb) void k_sph_boundary_density(float* __restrict__ dens, float* __restrict__ pres,
+                                                               const float4* __restrict__ pos, const float* __restrict__ vol,
+                                                               rxParticleCell b, SphK kb, uint n, int write_pres) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float d = boundary_density(f3(pos[i]), vol, b, kb) + dens[i];
+    dens[i] = d;
+    if (write_pres) pres[i] = kb.gas_k * (d - kb.rho0);
+}
+
+// rx_sph_kernel.cu:662-727 (sphCalBoundaryForce): a_i += sum_b -rho0 V_b (p_i / rho_i^2) grad W_spiky(r_ib),
+// for 1e-4 < r <= h.  This is the GPU form: an acceleration like the rest of dFrc.  The CPU loop
+// (rx_sph_cpu.cpp:617-651) divides the same sum by the particle mass, which is not an acceleration.
+__global__ __launch_bounds__(kTpb) void k_sph_boundary_force(const float* __restrict__ dens, const float* __restrict__ pres,
+                                                             const float4* __restrict__ pos, const float* __restrict__ vol,
+                                                             float4* __restrict__ frc, rxParticleCell b, SphK kb, uint n) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float3 p = f3(pos[i]);
+    float d0 = dens[i];
+    float prsi = pres[i] / (d0 * d0);
+    struct Pair {
+        bool in;
+        float v;
+    };
+    float3 f = f3(0, 0, 0);
+    for_each_candidate(
+        p, kb, b.dCellStart, b.dCellEnd, [&](uint j) { return PosIdx{b.dSortedPos[j], b.dSortedIndex[j]}; },
+        [&](uint, const PosIdx& a) {
+            Pair q{};
+            float3 rij = p - f3(a.x);
+            float r = sqrtf(dot3(rij, rij));
+            q.in = r <= kb.h && r > 0.0001f;
+            if (q.in) q.v = vol[a.o];
+            return q;
+        },
+        [&](uint, const PosIdx& a, const Pair& q) {
+            if (!q.in) return;
+            float3 rij = p - f3(a.x);
+            float r = sqrtf(dot3(rij, rij));
+            float w = kb.h - r;
+            f += (-kb.rho0 * q.v * prsi * kb.gwspiky * w * w / r) * rij;
+        });
+    float4 a = frc[i];
+    frc[i] = make_float4(a.x + f.x, a.y + f.y, a.z + f.z, a.w);
+}
+
+// Boundary terms of a PBF multiplier at p: the density sum (as above) and sum_b |(rho0 V_b / m) grad W / rho0|^2
+// (rx_pbdsph_kernel.cu:533-560, rx_sph_pbd.cpp:790-812).
+__device__ inline void boundary_lambda_terms(float3 p, const float* __restrict__ vol, const rxParticleCell& b, const SphK& kb,
+                                             float& d, float& sd) {
+    struct Pair {
+        bool in;
+        float v;
+    };
+    for_each_candidate(
+        p, kb, b.dCellStart, b.dCellEnd, [&](uint j) { return PosIdx{b.dSortedPos[j], b.dSortedIndex[j]}; },
+        [&](uint, const PosIdx& a) {
+            Pair q{};
+            float3 r = p - f3(a.x);
+            q.in = dot3(r, r) <= kb.h2;
+            if (q.in) q.v = vol[a.o];
+            return q;
+        },
+        [&](uint, const PosIdx& a, const Pair& q) {
+            if (!q.in) return;
+            float3 rij = p - f3(a.x);
+            float r2 = dot3(rij, rij);
+            float w = kb.h2 - r2;
+            d += kb.rho0 * q.v * kb.wpoly6 * w * w * w;
+            if (r2 > 0.0f) {
+                float r = sqrtf(r2);
+                float t = kb.h - r;
+                float3 g = (kb.rho0 * q.v / kb.mass * kb.gwspiky * t * t / r * kb.inv_rho0) * rij;
+                sd += dot3(g, g);
+            }
+        });
+}
+
+// rx_pbdsph_kernel.cu:493-520 (pbdsphCalBoundaryDensity): rho_i += boundary term; original order.
+// (k_sph_boundary_density with write_pres = 0.)
+
+// rx_pbdsph_kernel.cu:577-657 (pbdsphCalScalingFactorWithBoundary) with the fluid terms of k_pbf_lambda (self
+// term included, rx_sph_pbd.cpp:699-822): rho_i and sd_i gain the boundary terms.
+__global__ __launch_bounds__(kTpb) void k_pbf_lambda_wb(float* __restrict__ dens, float* __restrict__ scl, float eps, rxParticleCell c,
+                                                        SphK k, const float* __restrict__ vol, rxParticleCell b, SphK kb) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.uNumParticles) return;
+    float3 p = f3(c.dSortedPos[i]);
+    float d = 0.0f, sd = 0.0f;
+    float3 gs = f3(0, 0, 0);
+    for_each_candidate(
+        p, k, c.dCellStart, c.dCellEnd, [&](uint j) { return c.dSortedPos[j]; }, no_more,
+        [&](uint, const float4& x, Nothing) {
+            float3 rij = p - f3(x);
+            float r2 = dot3(rij, rij);
+            if (r2 <= k.h2) {
+                float q2 = k.h2 - r2;
+                d += k.mass * k.wpoly6 * q2 * q2 * q2;
+                if (r2 > 0.0f) {
+                    float r = sqrtf(r2);
+                    float q = k.h - r;
+                    float3 g = (k.gwspiky * q * q / r * k.inv_rho0) * rij;
+                    gs += g;
+                    sd += dot3(g, g);
+                }
+            }
+        });
+    sd += dot3(gs, gs);
+    boundary_lambda_terms(p, vol, b, kb, d, sd);
+    float C = d * k.inv_rho0 - 1.0f;
+    uint o = c.dSortedIndex[i];
+    scl[o] = -C / (sd + eps);
+    dens[o] = d;
+}
+
+// rx_pbdsph_kernel.cu:666-745 (pbdsphCalBoundaryScalingFactor) with the CPU rule of rx_sph_pbd.cpp:825-906:
+// lambda_b = -C_b / (sd_b + eps), C_b = (sum_fluid m W + sum_b' rho0 V_b' W) / rho0 - 1,
+// sd_b = sum_fluid |grad W / rho0|^2 + sum_b' |(rho0 V_b' / m) grad W / rho0|^2 (r > 0).  Every fluid particle
+// within h counts (the GPU kernel skips the fluid slot whose number equals the boundary slot's).
+__global__ __launch_bounds__(kTpb) void k_pbf_boundary_lambda(float* __restrict__ sclb, float eps, rxParticleCell c, SphK k,
+                                                              const float* __restrict__ vol, rxParticleCell b, SphK kb) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= b.uNumParticles) return;
+    float3 p = f3(b.dSortedPos[i]);
+    float d = 0.0f, sd = 0.0f;
+    for_each_candidate(
+        p, k, c.dCellStart, c.dCellEnd, [&](uint j) { return c.dSortedPos[j]; }, no_more,
+        [&](uint, const float4& x, Nothing) {
+            float3 rij = p - f3(x);
+            float r2 = dot3(rij, rij);
+            if (r2 <= k.h2) {
+                float q2 = k.h2 - r2;
+                d += k.mass * k.wpoly6 * q2 * q2 * q2;
+                if (r2 > 0.0f) {
+                    float r = sqrtf(r2);
+                    float q = k.h - r;
+                    float3 g = (k.gwspiky * q * q / r * k.inv_rho0) * rij;
+                    sd += dot3(g, g);
+                }
+            }
+        });
+    boundary_lambda_terms(p, vol, b, kb, d, sd);
+    float C = d * k.inv_rho0 - 1.0f;
+    sclb[b.dSortedIndex[i]] = -C / (sd + eps);
+}
+
+// rx_pbdsph_kernel.cu:760-860 (pbdsphPositionCorrectionWithBoundary), rx_sph_pbd.cpp:978-1018: the fluid terms of
+// k_pbf_dp plus sum_b (lambda_i + lambda_b + s_corr) grad W_spiky / rho0 with s_corr's W scaled by rho0 V_b / m.
+__global__ __launch_bounds__(kTpb) void k_pbf_dp_wb(const float* __restrict__ scl, float4* __restrict__ dp, rxParticleCell c, SphK k,
+                                                    const float* __restrict__ vol, const float* __restrict__ sclb,
+                                                    rxParticleCell b, SphK kb) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.uNumParticles) return;
+    float3 p = f3(c.dSortedPos[i]);
+    float si = scl[c.dSortedIndex[i]];
+    float dt2 = k.dt * k.dt;
+    float3 acc = f3(0, 0, 0);
+    struct Pair {
+        bool in;
+        float s, v;
+    };
+    for_each_candidate(
+        p, k, c.dCellStart, c.dCellEnd, [&](uint j) { return PosIdx{c.dSortedPos[j], c.dSortedIndex[j]}; },
+        [&](uint, const PosIdx& a) {
+            Pair q{};
+            float3 rij = p - f3(a.x);
+            float r2 = dot3(rij, rij);
+            q.in = r2 <= k.h2 && r2 > 0.0f;
+            if (q.in) q.s = scl[a.o];
+            return q;
+        },
+        [&](uint, const PosIdx& a, const Pair& q) {
+            if (!q.in) return;
+            float3 rij = p - f3(a.x);
+            float r2 = dot3(rij, rij);
+            float r = sqrtf(r2);
+            float sc = 0.0f;
+            if (k.ap) {
+                float q2 = k.h2 - r2;
+                float ww = k.wpoly6 * q2 * q2 * q2 * k.inv_ap_wq;
+                sc = -k.ap_k * pow_n(ww, k.ap_n) * dt2;
+            }
+            float t = k.h - r;
+            acc += ((si + q.s + sc) * k.gwspiky * t * t / r * k.inv_rho0) * rij;
+        });
+    for_each_candidate(
+        p, kb, b.dCellStart, b.dCellEnd, [&](uint j) { return PosIdx{b.dSortedPos[j], b.dSortedIndex[j]}; },
+        [&](uint, const PosIdx& a) {
+            Pair q{};
+            float3 rij = p - f3(a.x);
+            float r2 = dot3(rij, rij);
+            q.in = r2 <= kb.h2 && r2 > 0.0f;
+            if (q.in) {
+                q.s = sclb[a.o];
+                q.v = vol[a.o];
+            }
+            return q;
+        },
+        [&](uint, const PosIdx& a, const Pair& q) {
+            if (!q.in) return;
+            float3 rij = p - f3(a.x);
+            float r2 = dot3(rij, rij);
+            float r = sqrtf(r2);
+            float sc = 0.0f;
+            if (kb.ap) {
+                float q2 = kb.h2 - r2;
+                float ww = kb.rho0 * q.v / kb.mass * kb.wpoly6 * q2 * q2 * q2 * kb.inv_ap_wq;
+                sc = -kb.ap_k * pow_n(ww, kb.ap_n) * dt2;
+            }
+            float t = kb.h - r;
+            acc += ((si + q.s + sc) * kb.gwspiky * t * t / r * kb.inv_rho0) * rij;
+        });
+    dp[c.dSortedIndex[i]] = make_float4(acc.x, acc.y, acc.z, 0.0f);
+}
+
 bool need_params(const char* fn) {
     if (!rx_params_set()) {
         rx_set_error(3, "%s: CuSetParameters must be called first", fn);
+        return false;
+    }
+    return true;
+}
+
+// The boundary grid of `b` must match rxSimParams::GridSizeB (its cell arrays are indexed by that geometry).
+bool boundary_grid_ok(const rxParticleCell& b, const char* fn) {
+    const rxSimParams& p = rx_params();
+    size_t cells = (size_t)p.GridSizeB.x * p.GridSizeB.y * p.GridSizeB.z;
+    if (cells == 0 || cells != b.uNumCells || !b.dCellStart || !b.dCellEnd || (b.uNumParticles && (!b.dSortedPos || !b.dSortedIndex))) {
+        rx_set_error(2, "%s: boundary grid (GridSizeB %ux%ux%u) does not match the cell data (%u cells)", fn, p.GridSizeB.x,
+                     p.GridSizeB.y, p.GridSizeB.z, b.uNumCells);
         return false;
     }
     return true;
@@ -672,6 +963,62 @@ void CuPbdSphDensityErrorStats(float* dDens, float rest_dens, uint nprts, float*
     RX_HIP(hipStreamSynchronize(s));
     if (host_max_pos) *host_max_pos = h[0];
     if (host_mean_abs) *host_mean_abs = h[1];
+}
+
+
+// ---------------------------------------------------------------- boundary particles (rx_cu_funcs.cuh:81-83, 131-135)
+// rx_sph.cu:501-512.  As in the reference the volume uses rxSimParams::Mass; the mass argument is not read.
+void CuSphBoundaryVolume(float* dVolB, float mass, rxParticleCell cell) {
+    if (!need_params("CuSphBoundaryVolume") || !boundary_grid_ok(cell, "CuSphBoundaryVolume") || !cell.uNumParticles) return;
+    k_boundary_volume<<<rx_blocks(cell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>(dVolB, cell, bgrid_consts(rx_params()));
+    finish("CuSphBoundaryVolume");
+}
+
+// rx_sph.cu:520-532
+void CuSphBoundaryDensity(float* dDens, float* dPres, float* dPos, float* dVolB, rxParticleCell bcell, uint pnum) {
+    if (!need_params("CuSphBoundaryDensity") || !boundary_grid_ok(bcell, "CuSphBoundaryDensity") || !pnum) return;
+    k_sph_boundary_density<<<rx_blocks(pnum, kTpb), kTpb, 0, rx_stream()>>>(dDens, dPres, (const float4*)dPos, dVolB, bcell,
+                                                                            bgrid_consts(rx_params()), pnum, 1);
+    finish("CuSphBoundaryDensity");
+}
+
+// rx_sph.cu:541-553
+void CuSphBoundaryForces(float* dDens, float* dPres, float* dPos, float* dVolB, float* dFrc, rxParticleCell bcell, uint pnum) {
+    if (!need_params("CuSphBoundaryForces") || !boundary_grid_ok(bcell, "CuSphBoundaryForces") || !pnum) return;
+    k_sph_boundary_force<<<rx_blocks(pnum, kTpb), kTpb, 0, rx_stream()>>>(dDens, dPres, (const float4*)dPos, dVolB, (float4*)dFrc,
+                                                                          bcell, bgrid_consts(rx_params()), pnum);
+    finish("CuSphBoundaryForces");
+}
+
+// rx_sph.cu:754-766
+void CuPbdSphBoundaryDensity(float* dDens, float* dPos, float* dVolB, rxParticleCell bcell, uint pnum) {
+    if (!need_params("CuPbdSphBoundaryDensity") || !boundary_grid_ok(bcell, "CuPbdSphBoundaryDensity") || !pnum) return;
+    k_sph_boundary_density<<<rx_blocks(pnum, kTpb), kTpb, 0, rx_stream()>>>(dDens, nullptr, (const float4*)dPos, dVolB, bcell,
+                                                                            bgrid_consts(rx_params()), pnum, 0);
+    finish("CuPbdSphBoundaryDensity");
+}
+
+// rx_sph.cu:779-815: the fluid multipliers and densities, then the boundary particles' multipliers (dSclB).
+void CuPbdSphScalingFactorWithBoundary(float* dPos, float* dDens, float* dScl, float eps, rxParticleCell cell, float* dVolB,
+                                       float* dSclB, rxParticleCell bcell) {
+    if (!need_params("CuPbdSphScalingFactorWithBoundary") || !boundary_grid_ok(bcell, "CuPbdSphScalingFactorWithBoundary")) return;
+    SphK k = rx_make_consts(rx_params()), kb = bgrid_consts(rx_params());
+    if (cell.uNumParticles)
+        k_pbf_lambda_wb<<<rx_blocks(cell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>(dDens, dScl, eps, cell, k, dVolB, bcell, kb);
+    if (bcell.uNumParticles)
+        k_pbf_boundary_lambda<<<rx_blocks(bcell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>(dSclB, eps, cell, k, dVolB, bcell, kb);
+    finish("CuPbdSphScalingFactorWithBoundary");
+}
+
+// rx_sph.cu:827-850
+void CuPbdSphPositionCorrectionWithBoundary(float* dPos, float* dScl, float* dDp, rxParticleCell cell, float* dVolB, float* dSclB,
+                                            rxParticleCell bcell) {
+    if (!need_params("CuPbdSphPositionCorrectionWithBoundary") || !boundary_grid_ok(bcell, "CuPbdSphPositionCorrectionWithBoundary") ||
+        !cell.uNumParticles)
+        return;
+    k_pbf_dp_wb<<<rx_blocks(cell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>(dScl, (float4*)dDp, cell, rx_make_consts(rx_params()),
+                                                                                dVolB, dSclB, bcell, bgrid_consts(rx_params()));
+    finish("CuPbdSphPositionCorrectionWithBoundary");
 }
 
 }  // extern "C"

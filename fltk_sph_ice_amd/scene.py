@@ -247,3 +247,23 @@ def coupled_melt(fluid=(128, 128, 96), ice=(64, 128, 64), seed: int = 9) -> Coup
     fl.name = "coupled_fluid"
     ice_origin = (fluid[0] * s + 0.3 * box[0], 0.0 + s, 0.5 * (box[2] - ice[2] * s))
     return CoupledScene("coupled_melt", fl, tuple(ice), ice_origin, seed + 1, seed + 2)
+
+
+# --------------------------------------------------------------------------------------------- boundary particles
+def box_surface_particles(box_min, box_max, spacing: float) -> np.ndarray:
+    """Boundary particles on the six faces of the box, a square lattice of the given spacing per face with the
+    shared edges and corners listed once; float32 (n, 4).  (The reference samples the container surface with
+    marching cubes and a relaxation, rxSolid::GenerateParticlesOnSurf, rx_sph_solid.cpp:148-216; the ABI
+    takes any positions, so a lattice stands in for that host step.)"""
+    lo = np.asarray(box_min, np.float64)
+    hi = np.asarray(box_max, np.float64)
+    cnt = np.maximum(np.round((hi - lo) / spacing).astype(int), 1)
+    axes = [lo[k] + (hi[k] - lo[k]) * np.arange(cnt[k] + 1) / cnt[k] for k in range(3)]
+    g = np.stack(np.meshgrid(*axes, indexing="ij"), -1).reshape(-1, 3)
+    on = np.zeros(len(g), bool)
+    for k in range(3):
+        on |= np.isclose(g[:, k], lo[k]) | np.isclose(g[:, k], hi[k])
+    out = np.zeros((int(on.sum()), 4), np.float32)
+    out[:, :3] = g[on]
+    return out
+

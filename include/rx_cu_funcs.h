@@ -48,8 +48,9 @@ void CuUnmapGLBufferObject(cudaGraphicsResource* resource);                     
 /* Exclusive prefix sum of numElements uints (rx_cu_funcs.cuh:45). */
 void CuScan(unsigned int* output, unsigned int* input, unsigned int numElements);
 /* Ascending key/value sort of (dHash, dIndex), in place, STABLE: equal keys keep their input order
- * (rx_cu_funcs.cuh:46).  Keys must be cell hashes below the NumCells of the last CuSetParameters, or the
- * 0x7FFFFFFF marker that CuCalcHash writes for attr == -1 particles; any other key sets an error. */
+ * (rx_cu_funcs.cuh:46).  Keys must be cell hashes below max(NumCells, NumCellsB) of the last CuSetParameters
+ * (fluid or boundary-particle grid), or the 0x7FFFFFFF marker that CuCalcHash writes for attr == -1
+ * particles; any other key sets an error. */
 void CuSort(unsigned int* dHash, uint* dIndex, uint numParticles);
 /* Exclusive prefix sum of numElements floats (rx_cu_funcs.cuh:48). */
 void CuScanf(float* output, float* input, unsigned int numElements);
@@ -88,6 +89,18 @@ void CuSphIntegrate(float* pos, float* vel, float* frc, float* dens, int* attr, 
 void CuSphCheckDelete(float* pos, float* vel, int* attr, float minp[3], float maxp[3], float far_point[3], uint nprts);
 void CuSphCheckDeleteX(float* pos, float* vel, int* attr, float xmax, float farpoint[3], uint nprts);
 
+/* Boundary particles (rx_cu_funcs.cuh:81-83): a second sorted grid (bcell) over rxSimParams::WorldOriginB /
+ * CellWidthB / GridSizeB, built with CuCalcHashB + CuSort + CuReorderDataAndFindCellStartB.  Each reports an
+ * error and does nothing when GridSizeB does not match bcell.uNumCells.
+ * V_b = Mass / sum_{b'} Mass W_poly6(r_bb') (self included), written in original boundary order; the mass
+ * argument is not read (rxSimParams::Mass is used, as in rx_sph.cu:501-512) (:81). */
+void CuSphBoundaryVolume(float* dVolB, float mass, rxParticleCell cell);
+/* rho_i += sum_b rho0 V_b W_poly6(r_ib) and p_i = k (rho_i - rho0); dPos, dDens, dPres in original fluid order (:82). */
+void CuSphBoundaryDensity(float* dDens, float* dPres, float* dPos, float* dVolB, rxParticleCell bcell, uint pnum);
+/* dFrc_i += sum_b -rho0 V_b (p_i / rho_i^2) grad W_spiky(r_ib) for 1e-4 < r <= h (an acceleration, as the GPU
+ * kernel computes it) (:83). */
+void CuSphBoundaryForces(float* dDens, float* dPres, float* dPos, float* dVolB, float* dFrc, rxParticleCell bcell, uint pnum);
+
 /* Neighbour list of the ice/heat path (rx_cu_funcs.cuh:98): neights[i*30 + k] = neighbour k of particle i
  * (original index, r <= h as in the reference; radius is unused), unused slots -1, slot 29 = count. */
 void CuSphDetectNeighborParticles(int* neights, rxParticleCell celldata, float radius, int prtNum);
@@ -116,6 +129,17 @@ void CuPbdSphIntegrate(float* pos, float* vel, float* acc, int* attr,
 void CuPbdSphCheckDelete(float* pos, float* vel, int* attr, float minp[3], float maxp[3], float far_point[3], uint nprts); /* :123 */
 void CuPbdSphCheckDeleteX(float* pos, float* vel, int* attr, float xmax, float farpoint[3], uint nprts);                   /* :124 */
 void CuPbdSphNormal(float* dNrms, float* dDens, rxParticleCell cell);                                                     /* :128 */
+/* rho_i += sum_b rho0 V_b W_poly6(r_ib); original fluid order (:131). */
+void CuPbdSphBoundaryDensity(float* dDens, float* dPos, float* dVolB, rxParticleCell bcell, uint pnum);
+/* Fluid: lambda_i and rho_i of CuPbdSphScalingFactor with the boundary terms added to rho_i and to the
+ * denominator (sum_b |(rho0 V_b / Mass) grad W_spiky / rho0|^2).  Boundary: lambda_b from the fluid and boundary
+ * particles within h of b, written to dSclB in original boundary order (:132-133). */
+void CuPbdSphScalingFactorWithBoundary(float* dPos, float* dDens, float* dScl, float eps, rxParticleCell cell,
+                                       float* dVolB, float* dSclB, rxParticleCell bcell);
+/* dp_i of CuPbdSphPositionCorrection plus sum_b (lambda_i + lambda_b + s_corr) grad W_spiky / rho0, s_corr from
+ * (rho0 V_b / Mass) W_poly6 (:134-135). */
+void CuPbdSphPositionCorrectionWithBoundary(float* dPos, float* dScl, float* dDp, rxParticleCell cell,
+                                            float* dVolB, float* dSclB, rxParticleCell bcell);
 
 /* ================================================================ additive entry points (no reference counterpart) */
 /* Last error of this thread: 0 = none.  Cleared by CuClearLastError. */

@@ -58,6 +58,11 @@ _SIGS = {
     "orc_sph_step_f": (None, [P, P, I, P, F, P, P, P]),
     "orc_sph_step_d": (None, [P, P, I, P, D, P]),
     "orc_pbf_density_f": (None, [P, I, P, P]),
+    "orc_boundary_volume_f": (None, [P, I, P, P]),
+    "orc_boundary_density_f": (None, [P, I, P, P, I, P, P, P]),
+    "orc_boundary_force_f": (None, [P, I, P, P, I, P, P, P, P]),
+    "orc_pbf_scaling_factor_wb_f": (None, [P, I, P, P, I, P, P, P, P]),
+    "orc_pbf_position_correction_wb_f": (None, [P, P, I, P, P, P, I, P, F, P]),
     "orc_pbf_ext_force_f": (None, [P, P, P, I, P, P]),
     "orc_pbf_scaling_factor_f": (None, [P, I, P, P, P]),
     "orc_pbf_position_correction_f": (None, [P, P, I, P, F, P]),

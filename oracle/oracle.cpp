@@ -292,11 +292,78 @@ template <class T> void pbf_ext_force(const T* p4, const T* v4, const T* dens, i
     }
 }
 
+// ------------------------------------------------------------------------------------------ boundary particles
+// Boundary particles of the RX_USE_BOUNDARY_PARTICLE configuration: positions b4 (nb of them) and volumes vol.
+template <class T> struct Boundary {
+    const T* b4 = nullptr;
+    const T* vol = nullptr;
+    int nb = 0;
+};
+
+// GetNearestNeighborsB (rx_sph_cpu.cpp:1352-1362): the boundary particles with Dist2 <= h^2 (Dist2 held as
+// RXREAL), found by a scan of all of them in index order.
+template <class T> void boundary_neighbours(V3 pos, const T* b4, int nb, const OrcParams& P, std::vector<int>& out) {
+    out.clear();
+    double h = R<T>(P.h), h2 = R<T>(h * h);
+    for (int j = 0; j < nb; ++j) {
+        V3 d = pos - ld(b4, j);
+        if (R<T>(dot(d, d)) <= h2) out.push_back(j);
+    }
+}
+
+// rxSPH::calBoundaryVolumes (rx_sph_cpu.cpp:667-693): V_b = m / sum_{b'} m W(r), itself included.
+template <class T> void boundary_volumes(const T* b4, int nb, const OrcParams& P, T* vol) {
+    double h = R<T>(P.h), m = R<T>(P.mass), aw = R<T>(coef(0, h, 1));
+    std::vector<int> nbr;
+    for (int i = 0; i < nb; ++i) {
+        V3 pos0 = ld(b4, i);
+        boundary_neighbours(pos0, b4, nb, P, nbr);
+        T mw = 0;
+        for (int j : nbr) mw = (T)((double)mw + m * W_poly6(norm(ld(b4, j) - pos0), h, aw));
+        vol[i] = (T)(m / (double)mw);
+    }
+}
+
+// Boundary density term of rxSPH::calDensity / rxPBDSPH::calScalingFactor (rx_sph_cpu.cpp:527-548,
+// rx_sph_pbd.cpp:729-748): sum_b rho0 V_b W(r).
+template <class T> T boundary_density(V3 pos0, const Boundary<T>& B, const OrcParams& P, std::vector<int>& nbr) {
+    double h = R<T>(P.h), aw = R<T>(coef(0, h, 1)), r0 = R<T>(P.rest_dens);
+    boundary_neighbours(pos0, B.b4, B.nb, P, nbr);
+    T brho = 0;
+    for (int j : nbr) brho = (T)((double)brho + r0 * (double)B.vol[j] * W_poly6(norm(ld(B.b4, j) - pos0), h, aw));
+    return brho;
+}
+
+// Boundary pressure term as the GPU kernel computes it (rx_sph_kernel.cu:662-727): an acceleration,
+// sum_b -rho0 V_b (p_i / rho_i^2) grad W_spiky for 1e-4 < r <= h.  (The CPU loop, rx_sph_cpu.cpp:617-651,
+// divides this sum by the mass.)
+template <class T> void boundary_force(const T* p4, int n, const Boundary<T>& B, const OrcParams& P, const T* dens, const T* pres,
+                                       T* acc) {
+    double h = R<T>(P.h), ag = R<T>(coef(1, h, 2)), r0 = R<T>(P.rest_dens);
+    std::vector<int> nbr;
+    for (int i = 0; i < n; ++i) {
+        V3 pos0 = ld(p4, i);
+        double prsi = R<T>((double)pres[i] / ((double)dens[i] * (double)dens[i]));
+        boundary_neighbours(pos0, B.b4, B.nb, P, nbr);
+        V3 bp{0, 0, 0};
+        for (int j : nbr) {
+            V3 rij = pos0 - ld(B.b4, j);
+            double r = norm(rij);
+            if (r <= 0.0001) continue;
+            bp = bp + (-r0 * (double)B.vol[j] * prsi * GW_spiky_s(r, h, ag)) * rij;
+        }
+        for (int k = 0; k < 3; ++k) acc[4 * i + k] = (T)((double)acc[4 * i + k] + (k == 0 ? bp.x : k == 1 ? bp.y : bp.z));
+    }
+}
+
 // rxPBDSPH::calScalingFactor (rx_sph_pbd.cpp:699-822), fluid-only branch.
+// With boundary particles (RX_USE_BOUNDARY_PARTICLE) the density and the denominator gain the boundary terms
+// (rx_sph_pbd.cpp:729-748, 790-812).
 template <class T> void pbf_lambda(const T* p4, int n, const std::vector<std::vector<Neigh>>& nb, const OrcParams& P,
-                                   T* dens, T* lambda) {
+                                   T* dens, T* lambda, const Boundary<T>* B = nullptr) {
     double h = R<T>(P.h), m = R<T>(P.mass), aw = R<T>(coef(0, h, 1)), ag = R<T>(coef(1, h, 2));
     double r0 = R<T>(P.rest_dens), eps = R<T>(P.eps);
+    std::vector<int> bn;
     for (int i = 0; i < n; ++i) {
         V3 pos0 = ld(p4, i);
         T d = 0;
@@ -304,6 +371,7 @@ template <class T> void pbf_lambda(const T* p4, int n, const std::vector<std::ve
             double r = norm(pos0 - ld(p4, e.idx));
             d = (T)((double)d + m * W_poly6(r, h, aw));
         }
+        if (B) d = (T)((double)d + (double)boundary_density(pos0, *B, P, bn));
         dens[i] = d;
         T C = (T)((double)d / r0 - 1.0);
         T sd = 0;
@@ -323,17 +391,61 @@ template <class T> void pbf_lambda(const T* p4, int n, const std::vector<std::ve
             }
             sd = (T)((double)sd + dot(dp, dp));
         }
+        if (B) {
+            for (int j : bn) {
+                V3 rij = pos0 - ld(B->b4, j);
+                double r = norm(rij);
+                V3 dpb = ((r0 * (double)B->vol[j] / m) * GW_spiky_s(r, h, ag) / r0) * rij;
+                sd = (T)((double)sd + dot(dpb, dpb));
+            }
+        }
         lambda[i] = (T)(-(double)C / ((double)sd + eps));
+    }
+}
+
+// rxPBDSPH::calScalingFactor, boundary-particle loop (rx_sph_pbd.cpp:825-906): lambda_b from the fluid and
+// boundary particles within h of b.
+template <class T> void pbf_boundary_lambda(const T* p4, int n, const Boundary<T>& B, const OrcParams& P, T* lambda_b) {
+    double h = R<T>(P.h), m = R<T>(P.mass), aw = R<T>(coef(0, h, 1)), ag = R<T>(coef(1, h, 2));
+    double r0 = R<T>(P.rest_dens), eps = R<T>(P.eps), h2 = R<T>(h * h);
+    std::vector<int> fn, bn;
+    for (int i = 0; i < B.nb; ++i) {
+        V3 pos0 = ld(B.b4, i);
+        fn.clear();
+        for (int j = 0; j < n; ++j) {  // GetNearestNeighbors: Dist2 <= h^2
+            V3 d = pos0 - ld(p4, j);
+            if (R<T>(dot(d, d)) <= h2) fn.push_back(j);
+        }
+        boundary_neighbours(pos0, B.b4, B.nb, P, bn);
+        T brho = 0;
+        for (int j : fn) brho = (T)((double)brho + m * W_poly6(norm(ld(p4, j) - pos0), h, aw));
+        for (int j : bn) brho = (T)((double)brho + r0 * (double)B.vol[j] * W_poly6(norm(ld(B.b4, j) - pos0), h, aw));
+        T C = (T)((double)brho / r0 - 1.0);
+        T sd = 0;
+        for (int k : fn) {
+            V3 rik = pos0 - ld(p4, k);
+            V3 dp = (GW_spiky_s(norm(rik), h, ag) / r0) * rik;
+            sd = (T)((double)sd + dot(dp, dp));
+        }
+        for (int j : bn) {
+            V3 rij = pos0 - ld(B.b4, j);
+            V3 dpb = ((r0 * (double)B.vol[j] / m) * GW_spiky_s(norm(rij), h, ag) / r0) * rij;
+            sd = (T)((double)sd + dot(dpb, dpb));
+        }
+        lambda_b[i] = (T)(-(double)C / ((double)sd + eps));
     }
 }
 
 // rxPBDSPH::calPositionCorrection (rx_sph_pbd.cpp:925-1026), fluid-only branch; s_corr carries dt^2 as in
 // the reference (rx_sph_pbd.cpp:970).
+// With boundary particles: plus sum_b (lambda_i + lambda_b + s_corr) grad W / rho0, s_corr's W scaled by
+// rho0 V_b / m (rx_sph_pbd.cpp:978-1018).
 template <class T> void pbf_dp(const T* p4, const T* lambda, int n, const std::vector<std::vector<Neigh>>& nb,
-                               const OrcParams& P, double dt, T* dp4) {
+                               const OrcParams& P, double dt, T* dp4, const Boundary<T>* B = nullptr, const T* lambda_b = nullptr) {
     double h = R<T>(P.h), aw = R<T>(coef(0, h, 1)), ag = R<T>(coef(1, h, 2)), r0 = R<T>(P.rest_dens);
     double k = R<T>(P.ap_k), nn = R<T>(P.ap_n), dq = R<T>(P.ap_q * h);
     double wq = R<T>(W_poly6(dq, h, aw));
+    std::vector<int> bn;
     for (int i = 0; i < n; ++i) {
         V3 pos0 = ld(p4, i);
         V3 dpij{0, 0, 0};
@@ -348,6 +460,23 @@ template <class T> void pbf_dp(const T* p4, const T* lambda, int n, const std::v
                 scorr = R<T>(-k * std::pow(ww, nn) * dt * dt);
             }
             dpij = dpij + (((double)lambda[i] + (double)lambda[j] + scorr) * GW_spiky_s(r, h, ag) / r0) * rij;
+        }
+        if (B) {
+            double m = R<T>(P.mass);
+            boundary_neighbours(pos0, B->b4, B->nb, P, bn);
+            V3 dpb{0, 0, 0};
+            for (int j : bn) {
+                V3 rij = pos0 - ld(B->b4, j);
+                double r = norm(rij);
+                if (r > h) continue;
+                double scorr = 0.0;
+                if (P.use_ap) {
+                    double ww = R<T>((r0 * (double)B->vol[j] / m) * W_poly6(r, h, aw) / wq);
+                    scorr = R<T>(-k * std::pow(ww, nn) * dt * dt);
+                }
+                dpb = dpb + (((double)lambda[i] + (double)lambda_b[j] + scorr) * GW_spiky_s(r, h, ag) / r0) * rij;
+            }
+            dpij = dpij + dpb;
         }
         st(dp4, i, dpij);
         dp4[4 * i + 3] = 0;
@@ -771,6 +900,36 @@ void orc_pbf_xsph_f(const float* pos4, const float* vel4, const float* dens, int
     std::vector<std::vector<Neigh>> nb;
     neighbour_lists(pos4, n, *P, nb);
     pbf_xsph(pos4, vel4, dens, n, nb, *P, (double)c, out_vel4);
+}
+void orc_boundary_volume_f(const float* b4, int nb, const OrcParams* P, float* vol) { boundary_volumes(b4, nb, *P, vol); }
+void orc_boundary_density_f(const float* pos4, int n, const float* b4, const float* vol, int nb, const OrcParams* P, float* dens,
+                            float* pres) {
+    Boundary<float> B{b4, vol, nb};
+    std::vector<int> bn;
+    double k = R<float>(P->gas_k), r0 = R<float>(P->rest_dens);
+    for (int i = 0; i < n; ++i) {
+        dens[i] = (float)((double)dens[i] + (double)boundary_density(ld(pos4, i), B, *P, bn));
+        if (pres) pres[i] = (float)(k * ((double)dens[i] - r0));
+    }
+}
+void orc_boundary_force_f(const float* pos4, int n, const float* b4, const float* vol, int nb, const OrcParams* P,
+                          const float* dens, const float* pres, float* acc4) {
+    boundary_force(pos4, n, Boundary<float>{b4, vol, nb}, *P, dens, pres, acc4);
+}
+void orc_pbf_scaling_factor_wb_f(const float* pos4, int n, const float* b4, const float* vol, int nb, const OrcParams* P,
+                                 float* dens, float* lambda, float* lambda_b) {
+    std::vector<std::vector<Neigh>> nbl;
+    neighbour_lists(pos4, n, *P, nbl);
+    Boundary<float> B{b4, vol, nb};
+    pbf_lambda(pos4, n, nbl, *P, dens, lambda, &B);
+    if (lambda_b) pbf_boundary_lambda(pos4, n, B, *P, lambda_b);
+}
+void orc_pbf_position_correction_wb_f(const float* pos4, const float* lambda, int n, const float* b4, const float* vol,
+                                      const float* lambda_b, int nb, const OrcParams* P, float dt, float* dp4) {
+    std::vector<std::vector<Neigh>> nbl;
+    neighbour_lists(pos4, n, *P, nbl);
+    Boundary<float> B{b4, vol, nb};
+    pbf_dp(pos4, lambda, n, nbl, *P, (double)dt, dp4, &B, lambda_b);
 }
 void orc_pbf_step_f(float* pos4, float* vel4, int n, const OrcParams* P, float* dens, float* lambda_sum, OrcPbfStats* st) {
     pbf_step(pos4, vel4, n, *P, dens, lambda_sum, st);

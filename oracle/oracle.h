@@ -77,6 +77,18 @@ void orc_pbf_ext_force_f(const float* pos4, const float* vel4, const float* dens
 void orc_pbf_scaling_factor_f(const float* pos4, int n, const OrcParams* P, float* dens, float* lambda);
 void orc_pbf_position_correction_f(const float* pos4, const float* lambda, int n, const OrcParams* P, float dt, float* dp4);
 void orc_pbf_xsph_f(const float* pos4, const float* vel4, const float* dens, int n, const OrcParams* P, float c, float* out_vel4);
+/* Boundary particles (RX_USE_BOUNDARY_PARTICLE): b4 positions, vol volumes, nb of them.
+ * rx_sph_cpu.cpp:667-693 (volumes), 527-548 (density term), rx_sph_kernel.cu:662-727 (pressure term, GPU form),
+ * rx_sph_pbd.cpp:699-914 (multipliers with boundary, boundary multipliers), 925-1026 (position correction). */
+void orc_boundary_volume_f(const float* b4, int nb, const OrcParams* P, float* vol);
+void orc_boundary_density_f(const float* pos4, int n, const float* b4, const float* vol, int nb, const OrcParams* P, float* dens,
+                            float* pres);
+void orc_boundary_force_f(const float* pos4, int n, const float* b4, const float* vol, int nb, const OrcParams* P,
+                          const float* dens, const float* pres, float* acc4);
+void orc_pbf_scaling_factor_wb_f(const float* pos4, int n, const float* b4, const float* vol, int nb, const OrcParams* P,
+                                 float* dens, float* lambda, float* lambda_b);
+void orc_pbf_position_correction_wb_f(const float* pos4, const float* lambda, int n, const float* b4, const float* vol,
+                                      const float* lambda_b, int nb, const OrcParams* P, float dt, float* dp4);
 /* One PBF step of the reference CPU path with a fixed number of Jacobi iterations, collisions and the
  * optional XSPH pass; lambda_sum (may be NULL) receives sum_k lambda_k; dens the last lambda-pass density. */
 void orc_pbf_step_f(float* pos4, float* vel4, int n, const OrcParams* P, float* dens, float* lambda_sum, OrcPbfStats* st);

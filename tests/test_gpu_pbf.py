@@ -42,7 +42,8 @@ def test_lambda_and_dp_parity(lib):
     # sign: lambda has the sign of -C
     C_ = rd / sp.Density - 1
     assert np.all(np.sign(rl[np.abs(C_) > 1e-4]) == -np.sign(C_[np.abs(C_) > 1e-4]))
-    lib.CuPbdSphPositionCorrection(dpos.ptr, DevBuf.from_array(rl).ptr, ddp.ptr, c)
+    drl = DevBuf.from_array(rl)  # held until the call returns
+    lib.CuPbdSphPositionCorrection(dpos.ptr, drl.ptr, ddp.ptr, c)
     check(lib)
     rdp = np.zeros((n, 4), np.float32)
     orc.orc_pbf_position_correction_f(oracle.ptr(pos), oracle.ptr(rl), n, C.byref(P), np.float32(sp.Dt), oracle.ptr(rdp))
@@ -71,14 +72,15 @@ def test_ext_force_and_xsph(lib):
     racc = np.zeros((n, 4), np.float32)
     orc.orc_pbf_ext_force_f(oracle.ptr(pos), oracle.ptr(vel), oracle.ptr(rd), n, C.byref(P), oracle.ptr(racc))
     df = DevBuf(16 * n)
-    lib.CuPbdSphExternalForces(DevBuf.from_array(rd).ptr, df.ptr, c, sc.dt)
+    drd = DevBuf.from_array(rd)  # held until the last call that reads it
+    lib.CuPbdSphExternalForces(drd.ptr, df.ptr, c, sc.dt)
     check(lib)
     acc = df.download(np.float32).reshape(n, 4)
     assert np.abs(acc[:, :3] - racc[:, :3]).max() <= 1e-4 * np.abs(racc[:, :3]).max()
     rv = np.zeros((n, 4), np.float32)
     orc.orc_pbf_xsph_f(oracle.ptr(pos), oracle.ptr(vel), oracle.ptr(rd), n, C.byref(P), np.float32(0.01), oracle.ptr(rv))
     dnv = DevBuf(16 * n)
-    lib.CuXSphViscosity(dpos.ptr, dvel.ptr, dnv.ptr, DevBuf.from_array(rd).ptr, np.float32(0.01), c)
+    lib.CuXSphViscosity(dpos.ptr, dvel.ptr, dnv.ptr, drd.ptr, np.float32(0.01), c)
     check(lib)
     nv = dnv.download(np.float32).reshape(n, 4)
     assert np.abs(nv[:, :3] - rv[:, :3]).max() <= TOL * np.abs(rv[:, :3]).max()

@@ -79,7 +79,8 @@ def test_integrate_free_fall_and_walls(lib):
     acc[:, 1] = -9.8
     dpos, dvel, dacc = DevBuf.from_array(pos), DevBuf.from_array(vel), DevBuf.from_array(acc)
     dt = np.float32(1e-3)
-    lib.CuSphIntegrate(dpos.ptr, dvel.ptr, dacc.ptr, DevBuf(4 * n).ptr, None, dt, n)
+    ddens = DevBuf(4 * n)  # held until the call returns
+    lib.CuSphIntegrate(dpos.ptr, dvel.ptr, dacc.ptr, ddens.ptr, None, dt, n)
     check(lib)
     x = dpos.download(np.float32).reshape(n, 4)
     v = dvel.download(np.float32).reshape(n, 4)

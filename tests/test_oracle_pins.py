@@ -206,7 +206,8 @@ def test_lattice_interior_feels_only_gravity(orc):
     pres = np.zeros(n, np.float32)
     acc = np.zeros((n, 4), np.float32)
     orc.orc_sph_density_f(ptr(p), n, C.byref(P), ptr(dens), ptr(pres))
-    orc.orc_sph_force_f(ptr(p), ptr(np.zeros((n, 4), np.float32)), ptr(dens), ptr(pres), n, C.byref(P), ptr(acc))
+    vel = np.zeros((n, 4), np.float32)
+    orc.orc_sph_force_f(ptr(p), ptr(vel), ptr(dens), ptr(pres), n, C.byref(P), ptr(acc))
     centre = np.argmin(np.linalg.norm(x - x.mean(0), axis=1))
     np.testing.assert_allclose(acc[centre, :3], [0.0, -9.8, 0.0], atol=1e-3)
 
@@ -326,7 +327,8 @@ def _heat(orc, p, T, P, kappa, dt):
     dens = np.zeros(n, np.float32)
     orc.orc_pbf_density_f(ptr(p), n, C.byref(P), ptr(dens))
     out = np.zeros(n, np.float32)
-    orc.orc_heat_diffuse_f(ptr(p), ptr(dens), ptr(np.ascontiguousarray(T, np.float32)), n, C.byref(P), kappa, dt, ptr(out))
+    T32 = np.ascontiguousarray(T, np.float32)
+    orc.orc_heat_diffuse_f(ptr(p), ptr(dens), ptr(T32), n, C.byref(P), kappa, dt, ptr(out))
     return dens, out
 
 

@@ -133,8 +133,9 @@ def test_beta_one_volume_normalised_linear_map(orc):
 
 def _masked_step(orc, p, v, cs, mem, rest, S, active):
     p, v = p.astype(np.float32).copy(), v.astype(np.float32).copy()
+    rest32 = np.ascontiguousarray(rest, np.float32)
     orc.orc_sm_step_masked_f(oracle.ptr(p), oracle.ptr(v), len(p), oracle.ptr(cs), oracle.ptr(mem), len(cs) - 1,
-                             oracle.ptr(np.ascontiguousarray(rest, np.float32)), C.byref(S), oracle.ptr(active))
+                             oracle.ptr(rest32), C.byref(S), oracle.ptr(active))
     return p, v
 
 
