@@ -133,6 +133,11 @@ SIGNATURES: dict[str, tuple] = {
     "CuSphIntegrate": (None, [vp, vp, vp, vp, vp, flt, u32]),
     "CuSphCheckDelete": (None, [vp, vp, vp, fp, fp, fp, u32]),
     "CuSphCheckDeleteX": (None, [vp, vp, vp, flt, fp, u32]),
+    "CuSphIntegrateWithPolygon": (None, [vp, vp, vp, vp, vp, vp, vp, i32, flt, rxParticleCell]),
+    "CuPbdSphIntegrateWithPolygon": (None, [vp, vp, vp, vp, vp, vp, vp, vp, i32, flt, rxParticleCell]),
+    "CuSphGridDensity": (None, [vp, rxParticleCell, i32, i32, i32, flt, flt, flt, flt, flt, flt]),
+    "CuPbdSphGridDensity": (None, [vp, rxParticleCell, i32, i32, i32, flt, flt, flt, flt, flt, flt]),
+    "CuIceMeshMake": (None, [vp, rxParticleCell, i32, i32, i32, flt, flt, flt, flt, flt, flt, vp]),
     "CuSphBoundaryVolume": (None, [vp, flt, rxParticleCell]),
     "CuSphBoundaryDensity": (None, [vp, vp, vp, vp, rxParticleCell, u32]),
     "CuSphBoundaryForces": (None, [vp, vp, vp, vp, vp, rxParticleCell, u32]),

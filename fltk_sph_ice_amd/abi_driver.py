@@ -83,6 +83,52 @@ class BoundaryGrid:
         check(lib)
 
 
+class PolygonCells:
+    """The triangles of a mesh listed per cell of the particle grid, on the device, for the *WithPolygon
+    integrators (rxParticleCell::dSortedPolyIdx / dGridPolyHash / dPolyCellStart / dPolyCellEnd).  A triangle is
+    listed in every cell its bounding box overlaps (rxSPH_GPU::setPolysToCell, rx_sph_gpu.cpp:2301-2420, keeps
+    only the cells the triangle itself cuts; a superset lists more candidates and finds the same hits).
+    vrts: (nv, 3) float32, tris: (nt, 3) int32."""
+
+    def __init__(self, sp, vrts: np.ndarray, tris: np.ndarray, lib=None):
+        self.lib = lib or load()
+        self.vrts = np.ascontiguousarray(vrts, np.float32)
+        self.tris = np.ascontiguousarray(tris, np.int32)
+        org = np.array([sp.WorldOrigin[k] for k in range(3)], np.float32)
+        cw = np.array([sp.CellWidth[k] for k in range(3)], np.float32)
+        gs = np.array([sp.GridSize[k] for k in range(3)], np.int64)
+        tv = self.vrts[self.tris]  # (nt, 3, 3)
+        lo = np.clip(np.floor((tv.min(1) - org) / cw).astype(np.int64), 0, gs - 1)
+        hi = np.clip(np.floor((tv.max(1) - org) / cw).astype(np.int64), 0, gs - 1)
+        hashes, idx = [], []
+        for t in range(len(self.tris)):
+            xs, ys, zs = (np.arange(lo[t, k], hi[t, k] + 1) for k in range(3))
+            z, y, x = np.meshgrid(zs, ys, xs, indexing="ij")
+            h = ((z * gs[1] + y) * gs[0] + x).ravel()
+            hashes.append(h)
+            idx.append(np.full(len(h), t))
+        h = np.concatenate(hashes).astype(np.uint32)
+        ti = np.concatenate(idx).astype(np.uint32)
+        order = np.argsort(h, kind="stable")
+        self.hash, self.index = h[order], ti[order]
+        ncells = int(np.prod(gs))
+        self.start = np.full(ncells, 0xFFFFFFFF, np.uint32)
+        self.end = np.full(ncells, 0xFFFFFFFF, np.uint32)
+        first = np.r_[True, self.hash[1:] != self.hash[:-1]]
+        last = np.r_[self.hash[1:] != self.hash[:-1], True]
+        self.start[self.hash[first]] = np.nonzero(first)[0]
+        self.end[self.hash[last]] = np.nonzero(last)[0] + 1
+        self.d_vrts, self.d_tris = DevBuf.from_array(self.vrts), DevBuf.from_array(self.tris)
+        self.d_index, self.d_hash = DevBuf.from_array(self.index), DevBuf.from_array(self.hash)
+        self.d_start, self.d_end = DevBuf.from_array(self.start), DevBuf.from_array(self.end)
+
+    def attach(self, cell: rxParticleCell) -> rxParticleCell:
+        cell.dSortedPolyIdx, cell.dGridPolyHash = self.d_index.ptr, self.d_hash.ptr
+        cell.dPolyCellStart, cell.dPolyCellEnd = self.d_start.ptr, self.d_end.ptr
+        cell.uNumPolyHash = len(self.hash)
+        return cell
+
+
 def sph_step(scene, pos4: np.ndarray, vel4: np.ndarray, steps: int = 1):
     """WCSPH steps through the ABI: hash -> sort -> reorder -> density -> forces -> integrate."""
     lib = load()

@@ -506,6 +506,123 @@ __global__ __launch_bounds__(256) void k_err_final(const float* __restrict__ pma
 }
 
 
+
+// ------------------------------------------------------------------------------------------ polygon obstacles
+__device__ inline float3 cross3(float3 a, float3 b) { return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+
+// Segment P0 -> P1 against triangle (V0, V1, V2), entered from the side its normal (V1-V0) x (V2-V0) faces
+// (intersectSegmentTriangle, rx_cu_common.cu:891-960, return value 1).  I = the crossing point on the plane,
+// n = the unit normal.  Degenerate triangles and segments parallel to the plane do not hit.
+__device__ inline bool segment_hits_triangle(float3 P0, float3 P1, float3 V0, float3 V1, float3 V2, float3& I, float3& n) {
+    float3 u = V1 - V0, v = V2 - V0;
+    float3 c = cross3(u, v);
+    float cl = sqrtf(dot3(c, c));
+    if (!(cl > 0.0f)) return false;
+    n = (1.0f / cl) * c;
+    float3 dir = P1 - P0;
+    float a = dot3(n, P0 - V0), b = dot3(n, dir);
+    if (fabsf(b) < 1e-10f) return false;
+    float r = -a / b;
+    if (r < 0.0f || fabsf(a) > fabsf(b) || b > 0.0f) return false;
+    I = P0 + r * dir;
+    float uu = dot3(u, u), uv = dot3(u, v), vv = dot3(v, v);
+    float3 w = I - V0;
+    float wu = dot3(w, u), wv = dot3(w, v);
+    float D = uv * uv - uu * vv;
+    float s = (uv * wv - vv * wu) / D;
+    if (s < 0.0f || s > 1.0f) return false;
+    float t = (uv * wu - uu * wv) / D;
+    if (t < 0.0f || s + t > 1.0f) return false;
+    return true;
+}
+
+// rxSPH::calCollisionPolygon (rx_sph_cpu.cpp:956-993) over the triangles listed in one cell of the polygon grid
+// (rxParticleCell::dPolyCellStart/End/dSortedPolyIdx): on a hit the new position is put back at the crossing
+// point plus the reflected velocity times dt d / l, and the velocity gains the reflection
+// vr = -(1 + e') n (n.v), e' = e d / (dt |v|) (d = penetration depth, l = step length, e = Restitution).
+__device__ inline void collide_cell_polygons(uint hsh, float3 x0, float3& x1, float3& v, const float3* __restrict__ vrts,
+                                             const int3* __restrict__ tris, int ntri, const rxParticleCell& c, float dt, float e) {
+    uint s = c.dPolyCellStart[hsh];
+    if (s == kEmpty) return;
+    uint end = c.dPolyCellEnd[hsh];
+    for (uint j = s; j < end; ++j) {
+        uint t = c.dSortedPolyIdx[j];
+        if (t >= (uint)ntri) continue;
+        int3 tri = tris[t];
+        float3 cp, n;
+        if (!segment_hits_triangle(x0, x1, vrts[tri.x], vrts[tri.y], vrts[tri.z], cp, n)) continue;
+        float3 pen = x1 - cp;
+        float d = sqrtf(dot3(pen, pen));
+        float res = (e > 0.0f) ? e * fabsf(d) / (dt * sqrtf(dot3(v, v))) : 0.0f;
+        float3 vr = (-(1.0f + res) * dot3(n, v)) * n;
+        float3 step = x1 - x0;
+        float l = sqrtf(dot3(step, step));
+        x1 = cp + (dt * d / l) * vr;
+        v += vr;
+    }
+}
+
+// rxSPH::integrate with polygons (rx_sph_cpu.cpp:1040-1077; GPU forms sphIntegrateWithPolygon,
+// rx_sph_kernel.cu:851-895, and pbdsphIntegrateWithPolygon, rx_pbdsph_kernel.cu:985-1030): symplectic Euler, the
+// triangles of the old position's cell, then those of the new position's cell when it differs, then the walls.
+// Results go to (npos, nvel); attr == -1 particles are left as they are (copied when npos != pos).
+__global__ void k_integrate_polygon(const float4* pos, const float4* vel, const float4* __restrict__ acc, const int* __restrict__ attr,
+                                    float4* npos, float4* nvel, const float3* __restrict__ vrts, const int3* __restrict__ tris,
+                                    int ntri, float dt, rxParticleCell c, SphK k, float e, int keep_vel_w) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.uNumParticles) return;
+    float4 x4 = pos[i], v4 = vel[i];
+    if (attr && attr[i] == -1) {
+        npos[i] = x4;
+        nvel[i] = v4;
+        return;
+    }
+    float3 x0 = f3(x4), v = f3(v4) + dt * f3(acc[i]);
+    float3 x = x0 + dt * v;
+    if (ntri > 0 && c.dPolyCellStart && c.dPolyCellEnd && c.dSortedPolyIdx) {
+        uint h0 = grid_hash(grid_pos(x0, k), k);
+        collide_cell_polygons(h0, x0, x, v, vrts, tris, ntri, c, dt, e);
+        uint h1 = grid_hash(grid_pos(x, k), k);
+        if (h1 != h0) collide_cell_polygons(h1, x0, x, v, vrts, tris, ntri, c, dt, e);
+    }
+    x = wall_clamp(x, k);
+    npos[i] = make_float4(x.x, x.y, x.z, x4.w);
+    nvel[i] = make_float4(v.x, v.y, v.z, keep_vel_w ? v4.w : 0.0f);
+}
+
+// ------------------------------------------------------------------------------------------ density on a grid
+// sphCalDensityInGrid / pbdsphCalDensityInGrid / sphCalDensityInGridIceMesh (rx_sph_kernel.cu:1039-1126,
+// rx_pbdsph_kernel.cu:1263-1300): rho at the nodes gmin + (x, y, z) glen of an nx x ny x nz lattice, stored at
+// x + y nx + z nx ny.  With `ice`, particles whose flag is negative are left out.
+__global__ __launch_bounds__(kTpb) void k_grid_density(float* __restrict__ out, rxParticleCell c, SphK k, uint3 gnum, float3 gmin,
+                                                       float3 glen, const float* __restrict__ ice) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t total = (size_t)gnum.x * gnum.y * gnum.z;
+    if (idx >= total) return;
+    uint gx = (uint)(idx % gnum.x), gy = (uint)((idx / gnum.x) % gnum.y), gz = (uint)(idx / ((size_t)gnum.x * gnum.y));
+    float3 p = f3(gmin.x + gx * glen.x, gmin.y + gy * glen.y, gmin.z + gz * glen.z);
+    struct Pair {
+        bool in;
+    };
+    float d = 0.0f;
+    for_each_candidate(
+        p, k, c.dCellStart, c.dCellEnd, [&](uint j) { return PosIdx{c.dSortedPos[j], c.dSortedIndex[j]}; },
+        [&](uint, const PosIdx& a) {
+            Pair q{true};
+            if (ice) q.in = !(ice[a.o] < 0.0f);
+            return q;
+        },
+        [&](uint, const PosIdx& a, const Pair& q) {
+            float3 r = p - f3(a.x);
+            float r2 = dot3(r, r);
+            if (q.in && r2 <= k.h2) {
+                float w = k.h2 - r2;
+                d += k.mass * k.wpoly6 * w * w * w;
+            }
+        });
+    out[idx] = d;
+}
+
 // ------------------------------------------------------------------------------------------ boundary particles
 // Boundary particles (Akinci et al. 2012, "Versatile rigid-fluid coupling for incompressible SPH") sit in a
 // second sorted grid whose geometry is rxSimParams::WorldOriginB / CellWidthB / GridSizeB
@@ -1019,6 +1136,57 @@ void CuPbdSphPositionCorrectionWithBoundary(float* dPos, float* dScl, float* dDp
     k_pbf_dp_wb<<<rx_blocks(cell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>(dScl, (float4*)dDp, cell, rx_make_consts(rx_params()),
                                                                                 dVolB, dSclB, bcell, bgrid_consts(rx_params()));
     finish("CuPbdSphPositionCorrectionWithBoundary");
+}
+
+
+// ---------------------------------------------------------------- polygon obstacles (rx_cu_funcs.cuh:74-75, 118-120)
+// rx_sph.cu:307-320.  vrts: float3 per vertex, tris: 3 vertex indices per triangle, cell.dPolyCellStart/End/
+// dSortedPolyIdx: the triangles listed per cell of the fluid grid (built by the caller).  In place.
+void CuSphIntegrateWithPolygon(float* pos, float* vel, float* frc, float* dens, int* attr, float* vrts, int* tris, int tri_num,
+                               float dt, rxParticleCell cell) {
+    if (!need_params("CuSphIntegrateWithPolygon") || !cell.uNumParticles) return;
+    k_integrate_polygon<<<rx_blocks(cell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>(
+        (const float4*)pos, (const float4*)vel, (const float4*)frc, attr, (float4*)pos, (float4*)vel, (const float3*)vrts,
+        (const int3*)tris, tri_num, dt, cell, rx_make_consts(rx_params()), rx_params().Restitution, 1);
+    finish("CuSphIntegrateWithPolygon");
+}
+
+// rx_sph.cu:893-906.  Results in (new_pos, new_vel); pos/new_pos may alias.
+void CuPbdSphIntegrateWithPolygon(float* pos, float* vel, float* acc, int* attr, float* new_pos, float* new_vel, float* vrts, int* tris,
+                                  int tri_num, float dt, rxParticleCell cell) {
+    if (!need_params("CuPbdSphIntegrateWithPolygon") || !cell.uNumParticles) return;
+    k_integrate_polygon<<<rx_blocks(cell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>(
+        (const float4*)pos, (const float4*)vel, (const float4*)acc, attr, (float4*)new_pos, (float4*)new_vel, (const float3*)vrts,
+        (const int3*)tris, tri_num, dt, cell, rx_make_consts(rx_params()), rx_params().Restitution, 0);
+    finish("CuPbdSphIntegrateWithPolygon");
+}
+
+// ---------------------------------------------------------------- density on a grid (rx_cu_funcs.cuh:86-91, 126)
+static void grid_density(const char* fn, float* dGridD, const rxParticleCell& cell, int nx, int ny, int nz, float x0, float y0, float z0,
+                         float dx, float dy, float dz, const float* ice) {
+    if (!need_params(fn)) return;
+    if (nx <= 0 || ny <= 0 || nz <= 0) return;
+    size_t total = (size_t)nx * ny * nz;
+    k_grid_density<<<rx_blocks(total, kTpb), kTpb, 0, rx_stream()>>>(dGridD, cell, rx_make_consts(rx_params()),
+                                                                     make_uint3((uint)nx, (uint)ny, (uint)nz), f3(x0, y0, z0),
+                                                                     f3(dx, dy, dz), ice);
+    finish(fn);
+}
+
+// rx_sph.cu:390-425
+void CuSphGridDensity(float* dGridD, rxParticleCell cell, int nx, int ny, int nz, float x0, float y0, float z0, float dx, float dy,
+                      float dz) {
+    grid_density("CuSphGridDensity", dGridD, cell, nx, ny, nz, x0, y0, z0, dx, dy, dz, nullptr);
+}
+// rx_sph.cu:1054-1086
+void CuPbdSphGridDensity(float* dGridD, rxParticleCell cell, int nx, int ny, int nz, float x0, float y0, float z0, float dx, float dy,
+                         float dz) {
+    grid_density("CuPbdSphGridDensity", dGridD, cell, nx, ny, nz, x0, y0, z0, dx, dy, dz, nullptr);
+}
+// rx_sph.cu:433-468: particles with bIceFlag < 0 (not ice) are left out.
+void CuIceMeshMake(float* dGridD, rxParticleCell cell, int nx, int ny, int nz, float x0, float y0, float z0, float dx, float dy,
+                   float dz, float* bIceFlag) {
+    grid_density("CuIceMeshMake", dGridD, cell, nx, ny, nz, x0, y0, z0, dx, dy, dz, bIceFlag);
 }
 
 }  // extern "C"

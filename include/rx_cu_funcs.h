@@ -84,6 +84,15 @@ void CuSphNormal(float* dNrms, float* dDens, rxParticleCell cell);
 /* v += dt a; x += dt v; walls keep x at least ParticleRadius inside [BoundaryMin, BoundaryMax]; attr == -1
  * particles are skipped (:73). */
 void CuSphIntegrate(float* pos, float* vel, float* frc, float* dens, int* attr, float dt, uint numParticles);
+/* CuSphIntegrate with polygon obstacles (:74-75): vrts = float3 per vertex, tris = 3 vertex indices per triangle,
+ * cell.dPolyCellStart/End/dSortedPolyIdx = the triangles listed per cell of the fluid grid (start 0xFFFFFFFF when
+ * empty).  After the Euler step the segment old -> new position is tested against the triangles of the old
+ * position's cell, then of the new position's cell when it differs (rx_sph_cpu.cpp:1040-1077); a triangle is hit
+ * only from the side its normal (v1 - v0) x (v2 - v0) faces.  On a hit the position returns to the crossing
+ * point plus vr dt d / l and v += vr, vr = -(1 + e') n (n.v), e' = Restitution d / (dt |v|), d = penetration,
+ * l = step length.  Then the walls as in CuSphIntegrate.  Particles: cell.uNumParticles, original order, in place. */
+void CuSphIntegrateWithPolygon(float* pos, float* vel, float* frc, float* dens, int* attr,
+                               float* vrts, int* tris, int tri_num, float dt, rxParticleCell cell);
 /* Particles strictly inside the deletion region (minp, maxp) -- or beyond x > xmax for the X variant -- get
  * attr = -1 and are parked at far_point with zero velocity; attr == -1 particles are skipped (:77-78). */
 void CuSphCheckDelete(float* pos, float* vel, int* attr, float minp[3], float maxp[3], float far_point[3], uint nprts);
@@ -100,6 +109,13 @@ void CuSphBoundaryDensity(float* dDens, float* dPres, float* dPos, float* dVolB,
 /* dFrc_i += sum_b -rho0 V_b (p_i / rho_i^2) grad W_spiky(r_ib) for 1e-4 < r <= h (an acceleration, as the GPU
  * kernel computes it) (:83). */
 void CuSphBoundaryForces(float* dDens, float* dPres, float* dPos, float* dVolB, float* dFrc, rxParticleCell bcell, uint pnum);
+
+/* Density on a lattice, as CuPbdSphGridDensity (:86).  CuIceMeshMake leaves out particles whose bIceFlag
+ * (original order) is negative (:90-91). */
+void CuSphGridDensity(float* dGridD, rxParticleCell cell,
+                      int nx, int ny, int nz, float x0, float y0, float z0, float dx, float dy, float dz);
+void CuIceMeshMake(float* dGridD, rxParticleCell cell,
+                   int nx, int ny, int nz, float x0, float y0, float z0, float dx, float dy, float dz, float* bIceFlag);
 
 /* Neighbour list of the ice/heat path (rx_cu_funcs.cuh:98): neights[i*30 + k] = neighbour k of particle i
  * (original index, r <= h as in the reference; radius is unused), unused slots -1, slot 29 = count. */
@@ -126,9 +142,18 @@ float CuPbdSphCalDensityFluctuation(float* dErrScan, float* dErr, float* dDens, 
 /* new = integrate(pos, vel, acc) with wall handling; pos/new_pos may alias (:117). */
 void CuPbdSphIntegrate(float* pos, float* vel, float* acc, int* attr,
                        float* new_pos, float* new_vel, float dt, uint numParticles);
+/* CuPbdSphIntegrate with the polygon obstacles of CuSphIntegrateWithPolygon; results in new_pos/new_vel, which
+ * may alias pos/vel (:118-120). */
+void CuPbdSphIntegrateWithPolygon(float* pos, float* vel, float* acc, int* attr,
+                                  float* new_pos, float* new_vel,
+                                  float* vrts, int* tris, int tri_num, float dt, rxParticleCell cell);
 void CuPbdSphCheckDelete(float* pos, float* vel, int* attr, float minp[3], float maxp[3], float far_point[3], uint nprts); /* :123 */
 void CuPbdSphCheckDeleteX(float* pos, float* vel, int* attr, float xmax, float farpoint[3], uint nprts);                   /* :124 */
 void CuPbdSphNormal(float* dNrms, float* dDens, rxParticleCell cell);                                                     /* :128 */
+/* Density sum_j m W_poly6 at the nodes (x0 + i dx, y0 + j dy, z0 + k dz) of an nx x ny x nz lattice, written
+ * to dGridD[i + j nx + k nx ny] (:126; CuSphGridDensity :86 is the same function). */
+void CuPbdSphGridDensity(float* dGridD, rxParticleCell cell,
+                         int nx, int ny, int nz, float x0, float y0, float z0, float dx, float dy, float dz);
 /* rho_i += sum_b rho0 V_b W_poly6(r_ib); original fluid order (:131). */
 void CuPbdSphBoundaryDensity(float* dDens, float* dPos, float* dVolB, rxParticleCell bcell, uint pnum);
 /* Fluid: lambda_i and rho_i of CuPbdSphScalingFactor with the boundary terms added to rho_i and to the

@@ -58,6 +58,8 @@ _SIGS = {
     "orc_sph_step_f": (None, [P, P, I, P, F, P, P, P]),
     "orc_sph_step_d": (None, [P, P, I, P, D, P]),
     "orc_pbf_density_f": (None, [P, I, P, P]),
+    "orc_integrate_polygon_f": (None, [P, P, P, P, I, P, P, P, I, P, P, P, D, F, P, P]),
+    "orc_grid_density_f": (None, [P, I, P, I, I, I, F, F, F, F, F, F, P, P]),
     "orc_boundary_volume_f": (None, [P, I, P, P]),
     "orc_boundary_density_f": (None, [P, I, P, P, I, P, P, P]),
     "orc_boundary_force_f": (None, [P, I, P, P, I, P, P, P, P]),

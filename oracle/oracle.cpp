@@ -204,14 +204,86 @@ V3 wall_contact(V3 pos, const OrcParams& P, bool* hit) {
     return pos + std::fabs(d) * nrm;
 }
 
+// ------------------------------------------------------------------------------------------ polygons
+// Polygon obstacles: vertices (3 floats each), triangles (3 vertex indices each), and the triangles listed per
+// cell of the particle grid (start/end per cell, start 0xFFFFFFFF = none; idx = triangle numbers);
+// e = the restitution coefficient.
+struct Polys {
+    const float* vrts;
+    const int* tris;
+    int ntri;
+    const uint32_t* start;
+    const uint32_t* end;
+    const uint32_t* idx;
+    double e;
+};
+
+inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+// Segment P0 -> P1 against a triangle, in double: the published test of intersectSegmentTriangle
+// (rx_cu_common.cu:891-960; the CPU build's copy sits in a header the reference does not ship).  A hit (return
+// value 1) needs the segment to enter from the side the normal (V1-V0) x (V2-V0) faces and to end past the plane.
+bool segment_triangle(V3 P0, V3 P1, V3 V0, V3 V1, V3 V2, V3& I, V3& n) {
+    V3 u = V1 - V0, v = V2 - V0;
+    V3 c = cross(u, v);
+    double cl = norm(c);
+    if (!(cl > 0.0)) return false;
+    n = (1.0 / cl) * c;
+    V3 dir = P1 - P0;
+    double a = dot(n, P0 - V0), b = dot(n, dir);
+    if (std::fabs(b) < 1e-10) return false;
+    double r = -a / b;
+    if (r < 0.0 || std::fabs(a) > std::fabs(b) || b > 0.0) return false;
+    I = P0 + r * dir;
+    double uu = dot(u, u), uv = dot(u, v), vv = dot(v, v);
+    V3 w = I - V0;
+    double wu = dot(w, u), wv = dot(w, v), D = uv * uv - uu * vv;
+    double s = (uv * wv - vv * wu) / D;
+    if (s < 0.0 || s > 1.0) return false;
+    double t = (uv * wu - uu * wv) / D;
+    if (t < 0.0 || s + t > 1.0) return false;
+    return true;
+}
+
+// rxSPH::calCollisionPolygon (rx_sph_cpu.cpp:956-993): the triangles of one cell, in list order.
+void collide_cell(uint32_t h, V3 x0, V3& x1, V3& v, const Polys& Q, double dt) {
+    uint32_t s = Q.start[h];
+    if (s == 0xFFFFFFFFu) return;
+    for (uint32_t j = s; j < Q.end[h]; ++j) {
+        uint32_t t = Q.idx[j];
+        if (t >= (uint32_t)Q.ntri) continue;
+        V3 vt[3];
+        for (int q = 0; q < 3; ++q) {
+            int vi = Q.tris[3 * t + q];
+            vt[q] = {(double)Q.vrts[3 * vi], (double)Q.vrts[3 * vi + 1], (double)Q.vrts[3 * vi + 2]};
+        }
+        V3 cp, n;
+        if (!segment_triangle(x0, x1, vt[0], vt[1], vt[2], cp, n)) continue;
+        double d = norm(x1 - cp);
+        double res = (Q.e > 0) ? (Q.e * std::fabs(d) / (dt * norm(v))) : 0.0;
+        V3 vr = (-(1.0 + res) * dot(n, v)) * n;
+        double l = norm(x1 - x0);
+        x1 = cp + (dt * d / l) * vr;
+        v = v + vr;
+    }
+}
+
 // rxSPH::integrate / rxPBDSPH::integrate (rx_sph_cpu.cpp:1040-1077, rx_sph_pbd.cpp:1409-1446) with
-// calCollisionSolid (rx_sph_cpu.cpp:1005-1029): the velocity is left untouched by the wall.
+// calCollisionSolid (rx_sph_cpu.cpp:1005-1029): the velocity is left untouched by the wall.  With polygons the
+// triangles of the old position's cell are tested, then those of the new position's cell when it differs.
 template <class T> void integrate(const T* pos, const T* vel, const T* acc, T* pos_new, T* vel_new, int n,
-                                  const OrcParams& P, double dt) {
+                                  const OrcParams& P, double dt, const Polys* Q = nullptr) {
     for (int i = 0; i < n; ++i) {
         V3 x = ld(pos, i), v = ld(vel, i), a = ld(acc, i);
+        V3 x_old = x;
         v = v + dt * a;
         x = x + dt * v;
+        if (Q && Q->ntri > 0) {
+            uint32_t h0 = hash_of((float)x_old.x, (float)x_old.y, (float)x_old.z, P);
+            collide_cell(h0, x_old, x, v, *Q, dt);
+            uint32_t h1 = hash_of((float)x.x, (float)x.y, (float)x.z, P);
+            if (h1 != h0) collide_cell(h1, x_old, x, v, *Q, dt);
+        }
         bool hit;
         V3 c = wall_contact(x, P, &hit);
         if (hit) x = c;
@@ -931,6 +1003,41 @@ void orc_pbf_position_correction_wb_f(const float* pos4, const float* lambda, in
     Boundary<float> B{b4, vol, nb};
     pbf_dp(pos4, lambda, n, nbl, *P, (double)dt, dp4, &B, lambda_b);
 }
+void orc_integrate_polygon_f(const float* pos4, const float* vel4, const float* acc4, const int* attr, int n, const OrcParams* P,
+                             const float* vrts, const int* tris, int ntri, const uint32_t* pstart, const uint32_t* pend,
+                             const uint32_t* pidx, double restitution, float dt, float* out_pos4, float* out_vel4) {
+    Polys Q{vrts, tris, ntri, pstart, pend, pidx, restitution};
+    for (int i = 0; i < n; ++i) {
+        if (attr && attr[i] == -1) {
+            for (int k = 0; k < 4; ++k) { out_pos4[4 * i + k] = pos4[4 * i + k]; out_vel4[4 * i + k] = vel4[4 * i + k]; }
+            continue;
+        }
+        integrate(pos4 + 4 * i, vel4 + 4 * i, acc4 + 4 * i, out_pos4 + 4 * i, out_vel4 + 4 * i, 1, *P, (double)dt, &Q);
+    }
+}
+
+// rxSPH::CalGridDensity's GPU kernels (rx_sph_kernel.cu:1039-1126): sum_j m W_poly6 at the lattice nodes
+// (x0 + i dx, y0 + j dy, z0 + k dz), node (i, j, k) at i + j nx + k nx ny; particles with ice[j] < 0 are left
+// out when `ice` is given.  A scan of all particles (Dist2 <= h^2 in float).
+void orc_grid_density_f(const float* pos4, int n, const OrcParams* P, int nx, int ny, int nz, float x0, float y0, float z0, float dx,
+                        float dy, float dz, const float* ice, float* out) {
+    double h = R<float>(P->h), m = R<float>(P->mass), aw = R<float>(coef(0, h, 1)), h2 = R<float>(h * h);
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                V3 g{(double)(float)((double)x0 + (double)i * dx), (double)(float)((double)y0 + (double)j * dy),
+                     (double)(float)((double)z0 + (double)k * dz)};
+                float d = 0;
+                for (int q = 0; q < n; ++q) {
+                    if (ice && ice[q] < 0.0f) continue;
+                    V3 r = g - ld(pos4, q);
+                    double r2 = R<float>(dot(r, r));
+                    if (r2 <= h2) d = (float)((double)d + m * W_poly6(std::sqrt(r2), h, aw));
+                }
+                out[(size_t)i + (size_t)j * nx + (size_t)k * nx * ny] = d;
+            }
+}
+
 void orc_pbf_step_f(float* pos4, float* vel4, int n, const OrcParams* P, float* dens, float* lambda_sum, OrcPbfStats* st) {
     pbf_step(pos4, vel4, n, *P, dens, lambda_sum, st);
 }

@@ -77,6 +77,14 @@ void orc_pbf_ext_force_f(const float* pos4, const float* vel4, const float* dens
 void orc_pbf_scaling_factor_f(const float* pos4, int n, const OrcParams* P, float* dens, float* lambda);
 void orc_pbf_position_correction_f(const float* pos4, const float* lambda, int n, const OrcParams* P, float dt, float* dp4);
 void orc_pbf_xsph_f(const float* pos4, const float* vel4, const float* dens, int n, const OrcParams* P, float c, float* out_vel4);
+/* rxSPH::integrate with polygon obstacles (rx_sph_cpu.cpp:956-1077): vrts 3 floats per vertex, tris 3 ints per
+ * triangle, pstart/pend per grid cell (start 0xFFFFFFFF = none), pidx triangle numbers; walls as in the steps. */
+void orc_integrate_polygon_f(const float* pos4, const float* vel4, const float* acc4, const int* attr, int n, const OrcParams* P,
+                             const float* vrts, const int* tris, int ntri, const uint32_t* pstart, const uint32_t* pend,
+                             const uint32_t* pidx, double restitution, float dt, float* out_pos4, float* out_vel4);
+/* Density on a lattice (rx_sph_kernel.cu:1039-1126); ice may be null. */
+void orc_grid_density_f(const float* pos4, int n, const OrcParams* P, int nx, int ny, int nz, float x0, float y0, float z0, float dx,
+                        float dy, float dz, const float* ice, float* out);
 /* Boundary particles (RX_USE_BOUNDARY_PARTICLE): b4 positions, vol volumes, nb of them.
  * rx_sph_cpu.cpp:667-693 (volumes), 527-548 (density term), rx_sph_kernel.cu:662-727 (pressure term, GPU form),
  * rx_sph_pbd.cpp:699-914 (multipliers with boundary, boundary multipliers), 925-1026 (position correction). */
