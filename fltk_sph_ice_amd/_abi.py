@@ -163,6 +163,56 @@ SIGNATURES: dict[str, tuple] = {
     "CuGetLastErrorString": (C.c_char_p, []),
     "CuClearLastError": (None, []),
     "CuGetStream": (C.c_void_p, []),
+    # entry points of rx_cu_funcs.cuh:138-253 exported as error-reporting stubs (include/rx_cu_unsupported.h)
+    "CuSphCalUpdatedPosition": (None, []),
+    "CuSphCalCovarianceMatrix": (None, []),
+    "CuSphSVDecomposition": (None, []),
+    "CuSphCalTransformMatrix": (None, []),
+    "CuSphGridDensityAniso": (None, []),
+    "CuSphES": (None, []),
+    "CuAddWTurb3D": (None, []),
+    "CuSetWaveletTile3D": (None, []),
+    "CuSetWaveletTile3DB": (None, []),
+    "CuSetWaveletTile3DBT": (None, []),
+    "CuSubInit": (None, []),
+    "CuSubCalEt": (None, []),
+    "CuSubCalPos": (None, []),
+    "CuSubUpdateChildAxis": (None, []),
+    "CuSubInit2": (None, []),
+    "CuSubAdd": (None, []),
+    "CuSubUpdate": (None, []),
+    "CuSubSetUnsortArray": (None, []),
+    "CuSubCalcHash": (None, []),
+    "CuSubReorderDataAndFindCellStart": (None, []),
+    "CuSubSphGridDensity": (None, []),
+    "CuSubCheckRatio": (None, []),
+    "CuCheckNumValidHashData": (u32, []),
+    "CuMCCreateMesh": (None, []),
+    "CuMCInit": (C.c_bool, []),
+    "CuInitMCTable": (None, []),
+    "CuCleanMCTable": (None, []),
+    "CuMCCalTriNum": (None, []),
+    "CuMCCalEdgeVrts": (None, []),
+    "CuMCCalTri": (None, []),
+    "CuMCCalNrm": (None, []),
+    "CuInitFloatArray": (None, []),
+    "CuInitRadArray": (None, []),
+    "CuInitDepthMap": (None, []),
+    "CuCreateDepthMap": (None, []),
+    "CuDepthSmoothing": (None, []),
+    "CuDetectSilhouetteEdgeVertex": (None, []),
+    "CuCalNodeVertex": (None, []),
+    "CuCalMesh": (None, []),
+    "CuSilhouetteSmoothing": (None, []),
+    "CuTransformBack": (None, []),
+    "CuCalVertexNormal": (None, []),
+    "CuSetSSMParameters": (None, []),
+    "CuInitVPackf": (None, []),
+    "CuCleanVPackf": (None, []),
+    "CuInitVPacke": (None, []),
+    "CuCleanVPacke": (None, []),
+    "CuInitTable": (None, []),
+    "CuCleanTable": (None, []),
     "CuPbdSphDensityErrorStats": (None, [vp, flt, u32, fp, fp]),
     # rx_sph_host.h
     "rxSphEnvDefault": (None, [C.POINTER(rxSphEnv)]),

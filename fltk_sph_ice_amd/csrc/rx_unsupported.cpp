@@ -1,0 +1,63 @@
+// rx_unsupported.cpp -- the entry points of rx_cu_funcs.cuh:138-253 that this library does not implement
+// (anisotropic kernels, wavelet turbulence, sub-particles, marching cubes, screen-space meshing).  With C
+// linkage the symbol carries no parameter types, so each stub is defined without parameters: a caller that
+// passes the reference's arguments links and runs, the stub records error 5 and returns 0 / false.
+#include "rx_common.h"
+
+namespace {
+void unsupported(const char* fn) {
+    rx_set_error(5, "%s: not part of this library (meshing, turbulence, sub-particles and screen-space meshes are out of scope)", fn);
+}
+}  // namespace
+
+extern "C" {
+void CuSphCalUpdatedPosition() { unsupported("CuSphCalUpdatedPosition"); }
+void CuSphCalCovarianceMatrix() { unsupported("CuSphCalCovarianceMatrix"); }
+void CuSphSVDecomposition() { unsupported("CuSphSVDecomposition"); }
+void CuSphCalTransformMatrix() { unsupported("CuSphCalTransformMatrix"); }
+void CuSphGridDensityAniso() { unsupported("CuSphGridDensityAniso"); }
+void CuSphES() { unsupported("CuSphES"); }
+void CuAddWTurb3D() { unsupported("CuAddWTurb3D"); }
+void CuSetWaveletTile3D() { unsupported("CuSetWaveletTile3D"); }
+void CuSetWaveletTile3DB() { unsupported("CuSetWaveletTile3DB"); }
+void CuSetWaveletTile3DBT() { unsupported("CuSetWaveletTile3DBT"); }
+void CuSubInit() { unsupported("CuSubInit"); }
+void CuSubCalEt() { unsupported("CuSubCalEt"); }
+void CuSubCalPos() { unsupported("CuSubCalPos"); }
+void CuSubUpdateChildAxis() { unsupported("CuSubUpdateChildAxis"); }
+void CuSubInit2() { unsupported("CuSubInit2"); }
+void CuSubAdd() { unsupported("CuSubAdd"); }
+void CuSubUpdate() { unsupported("CuSubUpdate"); }
+void CuSubSetUnsortArray() { unsupported("CuSubSetUnsortArray"); }
+void CuSubCalcHash() { unsupported("CuSubCalcHash"); }
+void CuSubReorderDataAndFindCellStart() { unsupported("CuSubReorderDataAndFindCellStart"); }
+void CuSubSphGridDensity() { unsupported("CuSubSphGridDensity"); }
+void CuSubCheckRatio() { unsupported("CuSubCheckRatio"); }
+unsigned int CuCheckNumValidHashData() { unsupported("CuCheckNumValidHashData"); return 0u; }
+void CuMCCreateMesh() { unsupported("CuMCCreateMesh"); }
+bool CuMCInit() { unsupported("CuMCInit"); return false; }
+void CuInitMCTable() { unsupported("CuInitMCTable"); }
+void CuCleanMCTable() { unsupported("CuCleanMCTable"); }
+void CuMCCalTriNum() { unsupported("CuMCCalTriNum"); }
+void CuMCCalEdgeVrts() { unsupported("CuMCCalEdgeVrts"); }
+void CuMCCalTri() { unsupported("CuMCCalTri"); }
+void CuMCCalNrm() { unsupported("CuMCCalNrm"); }
+void CuInitFloatArray() { unsupported("CuInitFloatArray"); }
+void CuInitRadArray() { unsupported("CuInitRadArray"); }
+void CuInitDepthMap() { unsupported("CuInitDepthMap"); }
+void CuCreateDepthMap() { unsupported("CuCreateDepthMap"); }
+void CuDepthSmoothing() { unsupported("CuDepthSmoothing"); }
+void CuDetectSilhouetteEdgeVertex() { unsupported("CuDetectSilhouetteEdgeVertex"); }
+void CuCalNodeVertex() { unsupported("CuCalNodeVertex"); }
+void CuCalMesh() { unsupported("CuCalMesh"); }
+void CuSilhouetteSmoothing() { unsupported("CuSilhouetteSmoothing"); }
+void CuTransformBack() { unsupported("CuTransformBack"); }
+void CuCalVertexNormal() { unsupported("CuCalVertexNormal"); }
+void CuSetSSMParameters() { unsupported("CuSetSSMParameters"); }
+void CuInitVPackf() { unsupported("CuInitVPackf"); }
+void CuCleanVPackf() { unsupported("CuCleanVPackf"); }
+void CuInitVPacke() { unsupported("CuInitVPacke"); }
+void CuCleanVPacke() { unsupported("CuCleanVPacke"); }
+void CuInitTable() { unsupported("CuInitTable"); }
+void CuCleanTable() { unsupported("CuCleanTable"); }
+}  // extern "C"

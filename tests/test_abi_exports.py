@@ -61,3 +61,16 @@ def test_reference_struct_size():
     from fltk_sph_ice_amd._abi import rxSimParams
 
     assert C.sizeof(rxSimParams) == 1640
+
+
+def test_out_of_scope_entry_points_report_an_error(lib):
+    # meshing / turbulence / sub-particle / screen-space entry points link and report error 5 without work
+    lib.CuClearLastError()
+    lib.CuSubInit()
+    assert lib.CuGetLastErrorCode() == 5
+    assert b"CuSubInit" in lib.CuGetLastErrorString()
+    lib.CuClearLastError()
+    assert lib.CuMCInit() is False and lib.CuGetLastErrorCode() == 5
+    lib.CuClearLastError()
+    assert lib.CuCheckNumValidHashData() == 0
+    lib.CuClearLastError()
