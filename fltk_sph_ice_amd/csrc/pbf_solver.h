@@ -91,8 +91,8 @@ private:
     float4* m_ftmp16 = nullptr;
     uint* m_id = nullptr;       // slot -> original particle index
     uint* m_idt = nullptr;
-    uint* m_list = nullptr;     // skin neighbour list: uint4 chunks of 4 entries, [chunk][slot]
-    uint* m_cnt = nullptr;
+    uint* m_list = nullptr;     // skin neighbour list: uint4 chunks of 8 16-bit entries, [chunk][slot]
+    uint4* m_cnt = nullptr;     // list head per slot: 3 plane bases + count (bit 31: use the grid instead)
     float* m_stats = nullptr;
     // profiling
     bool m_profile = false;

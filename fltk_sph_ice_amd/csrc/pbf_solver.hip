@@ -177,12 +177,39 @@ __device__ inline void stencil_batched(int3 c, const FGridK& g, const uint* __re
 // every direction, which holds every such j; the radius is capped at kMaxStencil cells (displacements beyond
 // that raise the stats flag checked by the prediction pass).  `p` is the current position of i, `xb` the
 // build positions.
-// The list walk: groups of G chunks (4 G neighbours); the next group's indices are loaded while the current
+// List format.  The candidates of slot i come from three z-planes of cells (z - 1, z, z + 1 around its cell); within
+// a plane they lie in [b_p, e_p), b_p = the first slot of the plane's first row.  An entry is 16 bits: the plane
+// p in bits 14-15 and j - b_p in bits 0-13.  Eight entries make a uint4 chunk; chunk q of slot i is at
+// [q * n + i] (a wavefront reads 1 KB contiguous per chunk).  The head of slot i is (b_0, b_1, b_2, count), the
+// count with bit 31 set when an entry did not fit (the passes then walk the grid).
+constexpr uint kListOverflow = 0x80000000u;
+constexpr uint kWide = 0x40000000u;  // list build: an entry's offset did not fit in 14 bits
+
+__device__ inline uint list_slot(uint e, const uint4& hd) {
+    // plane 1 sets bit 14, plane 2 bit 15: two selects on bits (an equality chain on p is turned into a
+    // table lookup through scratch memory)
+    uint b = (e & 0x4000u) ? hd.y : hd.x;
+    b = (e & 0x8000u) ? hd.z : b;
+    return b + (e & 0x3FFFu);
+}
+__device__ inline void list_unpack(const uint4& w, const uint4& hd, uint* j) {
+    j[0] = list_slot(w.x & 0xFFFFu, hd);
+    j[1] = list_slot(w.x >> 16, hd);
+    j[2] = list_slot(w.y & 0xFFFFu, hd);
+    j[3] = list_slot(w.y >> 16, hd);
+    j[4] = list_slot(w.z & 0xFFFFu, hd);
+    j[5] = list_slot(w.z >> 16, hd);
+    j[6] = list_slot(w.w & 0xFFFFu, hd);
+    j[7] = list_slot(w.w >> 16, hd);
+}
+
+// The list walk: groups of G chunks (8 G neighbours); the next group's chunks are loaded while the current
 // group's neighbours are fetched, and all fetches of a group go out before any of them is used.
 template <int G, class Fetch, class Body>
-__device__ inline void list_walk(const uint4* __restrict__ l4, uint c, uint n, Fetch&& fetch, Body&& body) {
+__device__ inline void list_walk(const uint4* __restrict__ l4, const uint4& hd, uint n, Fetch&& fetch, Body&& body) {
     using D = decltype(fetch(0u));
-    const uint full = c >> 2;
+    const uint c = hd.w;
+    const uint full = c >> 3;
     uint q = 0;
     if (full) {
         uint4 e[G];
@@ -192,66 +219,54 @@ __device__ inline void list_walk(const uint4* __restrict__ l4, uint c, uint n, F
             uint4 en[G];
 #pragma unroll
             for (int g = 0; g < G; ++g) en[g] = l4[(size_t)min(q + G + g, full - 1) * n];
-            D d[4 * G];
+            uint j[8 * G];
+            D d[8 * G];
 #pragma unroll
-            for (int g = 0; g < G; ++g) {
-                d[4 * g] = fetch(e[g].x);
-                d[4 * g + 1] = fetch(e[g].y);
-                d[4 * g + 2] = fetch(e[g].z);
-                d[4 * g + 3] = fetch(e[g].w);
-            }
+            for (int g = 0; g < G; ++g) list_unpack(e[g], hd, j + 8 * g);
 #pragma unroll
-            for (int g = 0; g < G; ++g) {
-                body(e[g].x, d[4 * g]);
-                body(e[g].y, d[4 * g + 1]);
-                body(e[g].z, d[4 * g + 2]);
-                body(e[g].w, d[4 * g + 3]);
-            }
+            for (int t = 0; t < 8 * G; ++t) d[t] = fetch(j[t]);
+#pragma unroll
+            for (int t = 0; t < 8 * G; ++t) body(j[t], d[t]);
 #pragma unroll
             for (int g = 0; g < G; ++g) e[g] = en[g];
         }
 #pragma unroll
         for (int g = 0; g + 1 < G; ++g) {  // fewer than G whole chunks left: e[0 ..] hold them
             if (q + g < full) {
-                D a = fetch(e[g].x), b = fetch(e[g].y), cc = fetch(e[g].z), dd = fetch(e[g].w);
-                body(e[g].x, a);
-                body(e[g].y, b);
-                body(e[g].z, cc);
-                body(e[g].w, dd);
+                uint j[8];
+                D d[8];
+                list_unpack(e[g], hd, j);
+#pragma unroll
+                for (int t = 0; t < 8; ++t) d[t] = fetch(j[t]);
+#pragma unroll
+                for (int t = 0; t < 8; ++t) body(j[t], d[t]);
             }
         }
     }
-    const uint r = c & 3u;
+    const uint r = c & 7u;
     if (r) {
-        uint4 t = l4[(size_t)full * n];
-        D a = fetch(t.x);
-        if (r > 1) {
-            D b = fetch(t.y);
-            if (r > 2) {
-                D cc = fetch(t.z);
-                body(t.x, a);
-                body(t.y, b);
-                body(t.z, cc);
-            } else {
-                body(t.x, a);
-                body(t.y, b);
-            }
-        } else {
-            body(t.x, a);
-        }
+        uint j[8];
+        D d[8];
+        list_unpack(l4[(size_t)full * n], hd, j);
+#pragma unroll
+        for (int t = 0; t < 7; ++t)
+            if ((uint)t < r) d[t] = fetch(j[t]);
+#pragma unroll
+        for (int t = 0; t < 7; ++t)
+            if ((uint)t < r) body(j[t], d[t]);
     }
 }
 
 template <int G = 1, class Fetch, class Body>
-__device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint* __restrict__ list, const uint* __restrict__ cnt,
+__device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint* __restrict__ list, const uint4* __restrict__ cnt,
                                      const uint* __restrict__ bhash, const uint* __restrict__ off, float disp2, float3 p,
                                      const float4* __restrict__ xb, Fetch&& fetch, Body&& body) {
     if (mode == 1) {
         stencil_batched(fg_unhash(bhash[i], k.g), k.g, off, fetch, body, 1);
         return;
     }
-    uint c = cnt[i];
-    bool ok = c <= (uint)k.maxn;
+    const uint4 hd = cnt[i];
+    bool ok = hd.w <= (uint)k.maxn;
     float di = 0.0f;
     if (disp2 > k.half_skin2) {
         float3 dd = p - f3(xb[i]);
@@ -259,9 +274,7 @@ __device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint
         ok = ok && di + sqrtf(disp2) <= k.skin;
     }
     if (ok) {
-        // list layout: uint4 chunks of 4 entries, chunk q of slot i at [q * n + i] (a wavefront reads 1 KB
-        // contiguous per chunk)
-        list_walk<G>((const uint4*)list + i, c, k.n, fetch, body);
+        list_walk<G>((const uint4*)list + i, hd, k.n, fetch, body);
     } else {
         int rad = (int)ceilf((k.s.h + di + sqrtf(disp2)) * k.g.inv_cell * 1.0001f);
         rad = min(max(rad, 1), kMaxStencil);
@@ -278,7 +291,7 @@ struct XV {  // position and velocity of a neighbour
 // Reads x, v (w unused); writes (x*, rho_start) to xs.  Also judges the previous step: max (rho - rho0)+ / rho0.
 __global__ __launch_bounds__(kTpb) void k_predict(const float4* __restrict__ x, const float4* __restrict__ v, float4* __restrict__ xs,
                                                   float* __restrict__ dens, float* __restrict__ stats, int mode, PbfK k,
-                                                  const uint* __restrict__ list, const uint* __restrict__ cnt,
+                                                  const uint* __restrict__ list, const uint4* __restrict__ cnt,
                                                   const uint* __restrict__ bhash, const uint* __restrict__ off,
                                                   const float4* __restrict__ xb) {
     bool valid;
@@ -290,7 +303,7 @@ __global__ __launch_bounds__(kTpb) void k_predict(const float4* __restrict__ x, 
         float3 p = f3(x[i]), vi = f3(v[i]);
         float d = 0.0f;
         float3 fv = f3(0, 0, 0);
-        for_neighbors<2>(
+        for_neighbors<1>(
             i, mode, k, list, cnt, bhash, off, disp2, p, xb, [&](uint j) { return XV{x[j], v[j]}; },
             [&](uint, const XV& q) {
                 float3 r = p - f3(q.x);
@@ -332,33 +345,45 @@ __global__ void k_fold_run(float* __restrict__ stats, PbfK k) {
 // Builds the skin list of slot i from the new grid (cell = h + skin) and computes lambda at x*
 // (rx_sph_pbd.cpp:699-822: C = rho/rho0 - 1, sd with the self term).  Writes xl = (x*, lambda).  When feat is
 // set it also writes the 16 features of the learned warm start (12 used, 4 zero padding) for slot i.
-// `cands(f)` calls f(j, xs[j]) for every candidate j of the 27 cells around slot i, rows in z, y order and
-// slots ascending within a row.
+// `cands(plane, f)` visits the 27 cells around slot i as 9 rows of contiguous slots in z, y order: for each
+// z-plane pl (0, 1, 2 = z - 1, z, z + 1) inside the grid it calls plane(pl, base, span) with the first slot of the
+// plane's first row and the slots up to the end of its last row, then f(j, xs[j], base - (pl << 14)) for the
+// slots j of its rows in ascending order.
 template <class Cands>
 __device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, const float4* __restrict__ xs, float4* __restrict__ xl,
                                         float4* __restrict__ xbuild,
-                                        float* __restrict__ dens, uint* __restrict__ list, uint* __restrict__ cnt,
+                                        float* __restrict__ dens, uint* __restrict__ list, uint4* __restrict__ cnt,
                                         float* __restrict__ stats, const PbfK& k, float* __restrict__ feat,
                                         const float4* __restrict__ v, float& emax, float& esum) {
     float3 p = f3(p4);
     float d = 0.0f, sd = 0.0f;
     float3 gs = f3(0, 0, 0);
     uint c = 0;
+    // plane bases for the list head; a plane spanning more than 2^14 slots sets kWide in c (no entries are
+    // stored and the passes walk the grid)
+    uint base0 = 0u, base1 = 0u, base2 = 0u;
+    auto plane = [&](uint pl, uint base, uint span) {
+        base0 = pl == 0u ? base : base0;
+        base1 = pl == 1u ? base : base1;
+        base2 = pl == 2u ? base : base2;
+        c = span > 0x4000u ? (c | kWide) : c;
+    };
     // learned-start extras
     float nn = 0.0f, wsum = 0.0f, wcs = 0.0f, ring = 0.0f, nring = 0.0f;
     const float h2q = 0.25f * k.s.h2;
     uint4* l4 = (uint4*)list + i;  // chunk q of slot i at l4[q * n]
-    uint b0 = 0, b1 = 0, b2 = 0, b3 = 0;  // the open chunk, newest entry in b3
-    cands([&](uint j, float4 q4) {  // q4 = (x*_j, density at the step start)
+    uint b0 = 0, b1 = 0, b2 = 0, b3 = 0;  // the open chunk as a 128-bit shift register, newest entry at the top
+    cands(plane, [&](uint j, float4 q4, uint pl_base) {  // q4 = (x*_j, density at the step start)
         float3 r = p - f3(q4);
         float r2 = dot3(r, r);
         if (r2 <= k.rc2) {
             if (c < (uint)k.maxn) {
-                b0 = b1;
-                b1 = b2;
-                b2 = b3;
-                b3 = j;
-                if ((c & 3u) == 3u) l4[(size_t)(c >> 2) * k.n] = make_uint4(b0, b1, b2, b3);
+                const uint e = j - pl_base;  // pl_base = base - (plane << 14): the entry's plane bits come with it
+                b0 = (b0 >> 16) | (b1 << 16);
+                b1 = (b1 >> 16) | (b2 << 16);
+                b2 = (b2 >> 16) | (b3 << 16);
+                b3 = (b3 >> 16) | (e << 16);
+                if ((c & 7u) == 7u) l4[(size_t)(c >> 3) * k.n] = make_uint4(b0, b1, b2, b3);
             }
             ++c;
             if (r2 <= k.s.h2) {
@@ -386,13 +411,22 @@ __device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, const 
             }
         }
     });
-    if (c <= (uint)k.maxn && (c & 3u)) {  // the last, partial chunk (entries beyond c are never read)
-        const uint r = c & 3u;
-        uint4 e = r == 1 ? make_uint4(b3, 0, 0, 0) : (r == 2 ? make_uint4(b2, b3, 0, 0) : make_uint4(b1, b2, b3, 0));
-        l4[(size_t)(c >> 2) * k.n] = e;
+    if (c <= (uint)k.maxn && (c & 7u)) {  // the last, partial chunk: shift its r entries to the bottom
+        const uint r = c & 7u;
+#pragma unroll
+        for (uint t = 0; t < 7u; ++t) {
+            if (t < 8u - r) {
+                b0 = (b0 >> 16) | (b1 << 16);
+                b1 = (b1 >> 16) | (b2 << 16);
+                b2 = (b2 >> 16) | (b3 << 16);
+                b3 = b3 >> 16;
+            }
+        }
+        l4[(size_t)(c >> 3) * k.n] = make_uint4(b0, b1, b2, b3);
     }
-    cnt[i] = c;
-    if (c > (uint)k.maxn) atomicAdd((unsigned int*)&stats[ST_OVERFLOW], 1u);
+    const bool over = c > (uint)k.maxn;  // (kWide makes c larger than any capacity)
+    cnt[i] = make_uint4(base0, base1, base2, over ? (c | kListOverflow) : c);
+    if (over) atomicAdd((unsigned int*)&stats[ST_OVERFLOW], 1u);
     d *= k.s.mass * k.s.wpoly6;
     float sgs = dot3(gs, gs);
     sd += sgs;
@@ -428,21 +462,51 @@ __device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, const 
     }
 }
 
+// The candidates of the list build (see list_lambda_slot for the contract).
+template <class P, class F>
+__device__ inline void build_candidates(uint i, const PbfK& k, const float4* __restrict__ xs, const uint* __restrict__ bhash,
+                                        const uint* __restrict__ off, P&& plane, F&& f) {
+    const FGridK& g = k.g;
+    const int3 c = fg_unhash(bhash[i], g);
+    const int x0 = max(c.x - 1, 0), x1 = min(c.x + 1, g.gx - 1), y0 = max(c.y - 1, 0), y1 = min(c.y + 1, g.gy - 1);
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int z = c.z + dz;
+        if (z < 0 || z >= g.gz) continue;
+        const uint pl = (uint)(dz + 1);
+        const uint zb = (uint)(z * g.gy);
+        const uint base = off[(zb + (uint)y0) * g.gx + x0];
+        plane(pl, base, off[(zb + (uint)y1) * g.gx + x1 + 1] - base);
+        const uint pb = base - (pl << 14);
+        for (int y = y0; y <= y1; ++y) {
+            const uint hb = (zb + (uint)y) * g.gx;
+            uint j = off[hb + x0];
+            const uint e = off[hb + x1 + 1];
+            for (; j + 3 < e; j += 4) {
+                const float4 a = xs[j], b = xs[j + 1], cc = xs[j + 2], dd = xs[j + 3];
+                f(j, a, pb);
+                f(j + 1, b, pb);
+                f(j + 2, cc, pb);
+                f(j + 3, dd, pb);
+            }
+            for (; j < e; ++j) f(j, xs[j], pb);
+        }
+    }
+}
+
 // One thread per slot, in slot order (the default): the 27-cell walk reads candidates from global memory; the
 // state is in cell order, so a wavefront's candidates are few distinct cells and mostly L1/L2 hits.
 __global__ __launch_bounds__(kTpb) void k_list_lambda_slots(const float4* __restrict__ xs, float4* __restrict__ xl,
                                                             float4* __restrict__ xbuild, float* __restrict__ dens, uint* __restrict__ list,
-                                                            uint* __restrict__ cnt, float* __restrict__ stats, PbfK k,
+                                                            uint4* __restrict__ cnt, float* __restrict__ stats, PbfK k,
                                                             const uint* __restrict__ bhash, const uint* __restrict__ off,
                                                             float* __restrict__ feat, const float4* __restrict__ v) {
     bool valid;
     uint i = slot_of(k, valid);
     float emax = 0.0f, esum = 0.0f;
     if (valid) {
-        auto cands = [&](auto&& f) {
-            stencil_batched(fg_unhash(bhash[i], k.g), k.g, off, [&](uint j) { return xs[j]; }, [&](uint j, const float4& q) { f(j, q); }, 1);
-        };
-        list_lambda_slot(i, xs[i], cands, xs, xl, xbuild, dens, list, cnt, stats, k, feat, v, emax, esum);
+        const float4 p4 = xs[i];
+        auto cands = [&](auto&& plane, auto&& f) { build_candidates(i, k, xs, bhash, off, plane, f); };
+        list_lambda_slot(i, p4, cands, xs, xl, xbuild, dens, list, cnt, stats, k, feat, v, emax, esum);
     }
     block_max_atomic(emax, &stats[ST_ERR_ITER + 0]);
     block_sum_atomic(esum, &stats[ST_MPOS_ITER + 0]);
@@ -460,12 +524,12 @@ inline unsigned list_segments(const FGridK& g) { return (unsigned)(((g.gx + kSeg
 // kSegW cells per segment, kTileCap staged float4 entries, kThreads threads per block (a multiple of 64, <= 256).
 template <int kSegW, int kTileCap, int kThreads>
 __global__ __launch_bounds__(kThreads) void k_list_lambda(const float4* __restrict__ xs, float4* __restrict__ xl, float4* __restrict__ xbuild,
-                                                      float* __restrict__ dens, uint* __restrict__ list, uint* __restrict__ cnt,
+                                                      float* __restrict__ dens, uint* __restrict__ list, uint4* __restrict__ cnt,
                                                       float* __restrict__ stats, PbfK k, const uint* __restrict__ bhash,
                                                       const uint* __restrict__ off, float* __restrict__ feat,
                                                       const float4* __restrict__ v) {
     __shared__ float4 tile[kTileCap > 0 ? kTileCap : 1];
-    __shared__ uint s_lds0[10], s_glob0[9], s_len[9];
+    __shared__ uint s_lds0[10], s_glob0[9], s_len[9], s_valid[9];
     __shared__ uint s_coff[9][kSegW + 3];
     const FGridK& g = k.g;
     const int nsx = (g.gx + kSegW - 1) / kSegW;
@@ -481,6 +545,7 @@ __global__ __launch_bounds__(kThreads) void k_list_lambda(const float4* __restri
     if (threadIdx.x < 9) {
         const int r = threadIdx.x, zz = z + r / 3 - 1, yy = y + r % 3 - 1;
         uint g0 = 0, len = 0;
+        s_valid[r] = (zz >= 0 && zz < g.gz && yy >= 0 && yy < g.gy) ? 1u : 0u;
         if (zz >= 0 && zz < g.gz && yy >= 0 && yy < g.gy) {
             const uint rb = (uint)((zz * g.gy + yy) * g.gx);
             g0 = off[rb + xl0];
@@ -517,14 +582,26 @@ __global__ __launch_bounds__(kThreads) void k_list_lambda(const float4* __restri
         const int cx = (int)(bhash[i] % (uint)g.gx);
         const int a = max(cx - 1, 0) - xl0, b = min(cx + 1, g.gx - 1) - xl0 + 1;
         const float4 p4 = tiled ? tile[s_lds0[4] + (i - s_glob0[4])] : xs[i];
-        auto cands = [&](auto&& f) {
-            for (int r = 0; r < 9; ++r) {
-                const uint cs = s_coff[r][a], ce = s_coff[r][b], gb = s_glob0[r];
-                if (tiled) {
-                    const uint lb = s_lds0[r];
-                    for (uint t = cs; t < ce; ++t) f(gb + t, tile[lb + t]);
-                } else {
-                    for (uint t = cs; t < ce; ++t) f(gb + t, xs[gb + t]);
+        auto cands = [&](auto&& plane, auto&& f) {
+            for (int t = 0; t < 3; ++t) {
+                int rf = -1, rl = -1;
+                for (int r = 3 * t; r < 3 * t + 3; ++r)
+                    if (s_valid[r]) {
+                        if (rf < 0) rf = r;
+                        rl = r;
+                    }
+                if (rf < 0) continue;
+                const uint base = s_glob0[rf] + s_coff[rf][a];
+                plane((uint)t, base, s_glob0[rl] + s_coff[rl][b] - base);
+                const uint pb = base - ((uint)t << 14);
+                for (int r = rf; r <= rl; ++r) {
+                    const uint cs = s_coff[r][a], ce = s_coff[r][b], gb = s_glob0[r];
+                    if (tiled) {
+                        const uint lb = s_lds0[r];
+                        for (uint u = cs; u < ce; ++u) f(gb + u, tile[lb + u], pb);
+                    } else {
+                        for (uint u = cs; u < ce; ++u) f(gb + u, xs[gb + u], pb);
+                    }
                 }
             }
         };
@@ -538,7 +615,7 @@ __global__ __launch_bounds__(kThreads) void k_list_lambda(const float4* __restri
 // Adaptive solve: once a check met the tolerance the remaining passes exit at once.
 __global__ __launch_bounds__(kTpb) void k_lambda(const float4* __restrict__ xa, float4* __restrict__ xl, float* __restrict__ dens,
                                                  float* __restrict__ stats, int it, PbfK k, const uint* __restrict__ list,
-                                                 const uint* __restrict__ cnt, const uint* __restrict__ bhash,
+                                                 const uint4* __restrict__ cnt, const uint* __restrict__ bhash,
                                                  const uint* __restrict__ off, int adaptive, const float4* __restrict__ xb) {
     if (adaptive && it > 1 && converged(stats, it - 1, k)) return;  // solve already converged
     bool valid;
@@ -549,7 +626,7 @@ __global__ __launch_bounds__(kTpb) void k_lambda(const float4* __restrict__ xa, 
         float3 p = f3(xa[i]);
         float d = 0.0f, sd = 0.0f;
         float3 gs = f3(0, 0, 0);
-        for_neighbors<4>(
+        for_neighbors<2>(
             i, 0, k, list, cnt, bhash, off, disp2, p, xb, [&](uint j) { return xa[j]; },
             [&](uint, const float4& qj) {
                 float3 r = p - f3(qj);
@@ -583,7 +660,7 @@ __global__ __launch_bounds__(kTpb) void k_lambda(const float4* __restrict__ xa, 
 // skipped once the previous check met the tolerance.
 __global__ __launch_bounds__(kTpb) void k_dp(const float4* __restrict__ xl, float4* __restrict__ xa, const float4* __restrict__ xbuild,
                                              float* __restrict__ lamacc, float* __restrict__ stats, int it, PbfK k,
-                                             const uint* __restrict__ list, const uint* __restrict__ cnt,
+                                             const uint* __restrict__ list, const uint4* __restrict__ cnt,
                                              const uint* __restrict__ bhash, const uint* __restrict__ off, int adaptive) {
     if (adaptive && it > 0 && converged(stats, it, k)) return;
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -600,7 +677,7 @@ __global__ __launch_bounds__(kTpb) void k_dp(const float4* __restrict__ xl, floa
         float li = p4.w;
         float dt2 = k.dt * k.dt;
         float3 acc = f3(0, 0, 0);
-        for_neighbors<4>(
+        for_neighbors<2>(
             i, 0, k, list, cnt, bhash, off, disp2, p, xbuild, [&](uint j) { return xl[j]; },
             [&](uint, const float4& q4) {
                 float3 r = p - f3(q4);
@@ -638,7 +715,7 @@ __global__ void k_velocity(const float4* __restrict__ xa, const float4* __restri
 // XSPH (rx_pbdsph_kernel.cu:1090-1162 form): reads v = (v, rho), writes vout.
 __global__ __launch_bounds__(kTpb) void k_xsph(const float4* __restrict__ xa, const float4* __restrict__ v, float4* __restrict__ vout,
                                                float* __restrict__ stats, PbfK k, const uint* __restrict__ list,
-                                               const uint* __restrict__ cnt, const uint* __restrict__ bhash,
+                                               const uint4* __restrict__ cnt, const uint* __restrict__ bhash,
                                                const uint* __restrict__ off, const float4* __restrict__ xb) {
     bool valid;
     uint i = slot_of(k, valid);
@@ -647,7 +724,7 @@ __global__ __launch_bounds__(kTpb) void k_xsph(const float4* __restrict__ xa, co
     float3 p = f3(xa[i]);
     float3 vi = f3(v[i]);
     float3 acc = f3(0, 0, 0);
-    for_neighbors<2>(
+    for_neighbors<1>(
         i, 0, k, list, cnt, bhash, off, disp2, p, xb, [&](uint j) { return XV{xa[j], v[j]}; },
         [&](uint, const XV& q4) {
             float3 r = p - f3(q4.x);
@@ -668,7 +745,7 @@ __global__ __launch_bounds__(kTpb) void k_xsph(const float4* __restrict__ xa, co
 __global__ __launch_bounds__(kTpb) void k_heat_diffuse(const float4* __restrict__ x, const float* __restrict__ dens,
                                                        const float4* __restrict__ therm, float* __restrict__ dtemp,
                                                        const float* __restrict__ stats, PbfK k, const uint* __restrict__ list,
-                                                       const uint* __restrict__ cnt, const uint* __restrict__ bhash,
+                                                       const uint4* __restrict__ cnt, const uint* __restrict__ bhash,
                                                        const uint* __restrict__ off, const float4* __restrict__ xb, float td,
                                                        float hdt) {
     bool valid;
@@ -796,12 +873,15 @@ int rxPBDSPH_GPU::Initialize(const rxSphEnv& env) {
     return 0;
 }
 
+// Bytes of the skin list for n slots of capacity maxn: ceil(maxn / 8) uint4 chunks of 8 16-bit entries per slot.
+static size_t list_bytes(size_t n, int maxn) { return sizeof(uint4) * n * (size_t)((maxn + 7) / 8); }
+
 bool rxPBDSPH_GPU::alloc(uint n) {
     size_t v4 = sizeof(float4) * n, f = sizeof(float) * n, u = sizeof(uint) * n;
     bool ok = RX_HIP(hipMalloc(&m_x, v4)) && RX_HIP(hipMalloc(&m_xs, v4)) && RX_HIP(hipMalloc(&m_xl, v4)) &&
               RX_HIP(hipMalloc(&m_xb, v4)) && RX_HIP(hipMalloc(&m_v, v4)) && RX_HIP(hipMalloc(&m_t, v4)) &&
               RX_HIP(hipMalloc(&m_dens, f)) && RX_HIP(hipMalloc(&m_lamacc, f)) && RX_HIP(hipMalloc(&m_id, u)) &&
-              RX_HIP(hipMalloc(&m_idt, u)) && RX_HIP(hipMalloc(&m_list, u * (size_t)m_maxn)) && RX_HIP(hipMalloc(&m_cnt, u)) &&
+              RX_HIP(hipMalloc(&m_idt, u)) && RX_HIP(hipMalloc(&m_list, list_bytes(n, m_maxn))) && RX_HIP(hipMalloc(&m_cnt, v4)) &&
               RX_HIP(hipMalloc(&m_stats, sizeof(float) * ST_N)) && RX_HIP(hipMalloc(&m_ftmp, f)) &&
               RX_HIP(hipMalloc(&m_feat, v4 * 4)) &&
               RX_HIP(hipMalloc(&m_ftmp16, v4 * 4));
@@ -813,6 +893,10 @@ bool rxPBDSPH_GPU::alloc(uint n) {
 int rxPBDSPH_GPU::SetParticles(const float* pos4, const float* vel4, int n) {
     if (n <= 0 || !pos4) {
         rx_set_error(1, "rxPBDSPH_GPU::SetParticles: bad arguments");
+        return 1;
+    }
+    if (n >= (1 << 30)) {  // the list build packs a plane number above 30-bit slot numbers
+        rx_set_error(1, "rxPBDSPH_GPU::SetParticles: at most 2^30 - 1 particles");
         return 1;
     }
     if ((uint)n != m_n) {
@@ -1270,7 +1354,7 @@ int rxPBDSPH_GPU::SetOption(int opt, float value) {
             RX_HIP(hipStreamSynchronize(m_stream));
             RX_HIP(hipFree(m_list));
             m_list = nullptr;
-            if (!RX_HIP(hipMalloc(&m_list, sizeof(uint) * (size_t)m_n * m))) return 1;
+            if (!RX_HIP(hipMalloc(&m_list, list_bytes(m_n, m)))) return 1;
         }
         m_maxn = m;
         m_have_list = false;
