@@ -108,5 +108,25 @@ private:
     long m_prof_cnt[16] = {0};
     int begin_mark(int cls);
     void end_mark(int id);
+    // hipGraph replay of the step (RX_PBF_OPT_GRAPH): the step rotates some buffers, so one instantiated graph
+    // is kept per configuration of those buffers, keyed by the pointers and every value the launches capture
+    // (a step rotates them through six configurations).
+    struct StepPtrs {
+        float4 *x, *xs, *xl, *xb, *v, *t, *therm, *therm_t;
+        float *dens, *lamacc, *ftmp;
+        uint *id, *idt, *perm, *slot;
+    };
+    struct StepGraph {
+        std::vector<unsigned char> key;
+        StepPtrs after;  // the buffer configuration the step leaves
+        hipGraphExec_t exec;
+    };
+    std::vector<StepGraph> m_graphs;
+    bool m_use_graph = false;
+    StepPtrs save_ptrs() const;
+    void load_ptrs(const StepPtrs& p);
+    std::vector<unsigned char> graph_key(float dt) const;
+    void drop_graphs();
+    int enqueue_step(float dt);
     void collect_marks();
 };

@@ -106,6 +106,7 @@ __device__ inline void block_max_atomic(float v, float* dst) {
         for (uint w = 1; w < (blockDim.x >> 6); ++w) m = fmaxf(m, wm[w]);
         atomic_max_pos(dst, m);
     }
+    __syncthreads();  // wm is read by thread 0 above: the next reduction of the block must not overwrite it first
 }
 __device__ inline void block_sum_atomic(float v, float* dst) {
     const int lane = threadIdx.x & 63;
@@ -119,6 +120,7 @@ __device__ inline void block_sum_atomic(float v, float* dst) {
         for (uint w = 0; w < (blockDim.x >> 6); ++w) m += ws[w];
         atomicAdd(dst, m);
     }
+    __syncthreads();  // ws is read by thread 0 above: the next reduction of the block must not overwrite it first
 }
 
 __device__ inline uint slot_of(const PbfK& k, bool& valid) {
@@ -841,6 +843,7 @@ rxPBDSPH_GPU::~rxPBDSPH_GPU() { Finalize(); }
 
 void rxPBDSPH_GPU::Finalize() {
     if (m_stream) RX_HIP(hipStreamSynchronize(m_stream));
+    drop_graphs();
     for (void** p : {(void**)&m_x, (void**)&m_xs, (void**)&m_xl, (void**)&m_xb, (void**)&m_v, (void**)&m_t, (void**)&m_dens,
                      (void**)&m_lamacc, (void**)&m_id, (void**)&m_idt, (void**)&m_list, (void**)&m_cnt, (void**)&m_stats,
                      (void**)&m_ftmp, (void**)&m_feat, (void**)&m_ftmp16, (void**)&m_therm, (void**)&m_therm_t})
@@ -899,6 +902,7 @@ int rxPBDSPH_GPU::SetParticles(const float* pos4, const float* vel4, int n) {
         rx_set_error(1, "rxPBDSPH_GPU::SetParticles: at most 2^30 - 1 particles");
         return 1;
     }
+    drop_graphs();
     if ((uint)n != m_n) {
         for (void** p : {(void**)&m_x, (void**)&m_xs, (void**)&m_xl, (void**)&m_xb, (void**)&m_v, (void**)&m_t, (void**)&m_dens,
                          (void**)&m_lamacc, (void**)&m_id, (void**)&m_idt, (void**)&m_list, (void**)&m_cnt, (void**)&m_stats,
@@ -947,6 +951,7 @@ void rxPBDSPH_GPU::lattice_refs() {
 
 PbfK rxPBDSPH_GPU::consts(float dt) const {
     PbfK k;
+    memset(&k, 0, sizeof(k));  // padding included: graph keys compare the bytes
     rxSimParams p = m_params;
     p.Dt = dt;
     k.s = rx_make_consts(p);
@@ -980,6 +985,39 @@ void rxPBDSPH_GPU::reorder(float4*& a, hipStream_t s) {
     std::swap(a, m_t);
 }
 
+rxPBDSPH_GPU::StepPtrs rxPBDSPH_GPU::save_ptrs() const {
+    return StepPtrs{m_x, m_xs, m_xl, m_xb, m_v, m_t, m_therm, m_therm_t, m_dens, m_lamacc, m_ftmp, m_id, m_idt, m_grid.perm, m_grid.slot};
+}
+void rxPBDSPH_GPU::load_ptrs(const StepPtrs& p) {
+    m_x = p.x, m_xs = p.xs, m_xl = p.xl, m_xb = p.xb, m_v = p.v, m_t = p.t, m_therm = p.therm, m_therm_t = p.therm_t;
+    m_dens = p.dens, m_lamacc = p.lamacc, m_ftmp = p.ftmp, m_id = p.id, m_idt = p.idt, m_grid.perm = p.perm, m_grid.slot = p.slot;
+}
+// Everything the launches of one step take from the host: buffers, the kernel constants, the network view and
+// the choices the step makes on the host.
+std::vector<unsigned char> rxPBDSPH_GPU::graph_key(float dt) const {
+    std::vector<unsigned char> key;
+    auto put = [&](const void* p, size_t n) { key.insert(key.end(), (const unsigned char*)p, (const unsigned char*)p + n); };
+    StepPtrs ptrs = save_ptrs();
+    put(&ptrs, sizeof(ptrs));
+    PbfK k = consts(dt);
+    put(&k, sizeof(k));
+    put(&m_mlp, sizeof(m_mlp));
+    const int flags[] = {m_mode, std::min(m_env.max_iter, 14), m_adaptive ? 1 : 0, m_list_tile, m_record_features ? 1 : 0,
+                         m_step > 0 ? 1 : 0, m_have_list ? 1 : 0};
+    put(flags, sizeof(flags));
+    const float vals[] = {m_warm_scale, m_heat_td, m_heat_dt > 0.0f ? m_heat_dt : dt};
+    put(vals, sizeof(vals));
+    const void* bufs[] = {m_list, m_cnt, m_stats, m_feat, m_grid.hash, m_grid.off};
+    put(bufs, sizeof(bufs));
+    return key;
+}
+void rxPBDSPH_GPU::drop_graphs() {
+    if (m_graphs.empty()) return;
+    RX_HIP(hipStreamSynchronize(m_stream));  // no launch of these graphs is still pending
+    for (StepGraph& g : m_graphs) RX_HIP(hipGraphExecDestroy(g.exec));
+    m_graphs.clear();
+}
+
 int rxPBDSPH_GPU::Update(float dt, int step) {
     if (!m_n) {
         rx_set_error(1, "rxPBDSPH_GPU::Update: no particles");
@@ -989,6 +1027,50 @@ int rxPBDSPH_GPU::Update(float dt, int step) {
         rx_set_error(1, "rxPBDSPH_GPU::Update: learned mode needs network weights (rxPBDSPH_GPU_SetNetwork)");
         return 1;
     }
+    int rc = 0;
+    if (!m_use_graph || m_profile || !m_have_list) {
+        rc = enqueue_step(dt);  // the first step (no list yet) and profiled steps run eagerly
+    } else {
+        std::vector<unsigned char> key = graph_key(dt);
+        StepGraph* hit = nullptr;
+        for (StepGraph& g : m_graphs)
+            if (g.key == key) hit = &g;
+        if (hit) {
+            if (!RX_HIP(hipGraphLaunch(hit->exec, m_stream))) return 1;
+            load_ptrs(hit->after);
+            m_have_list = true;
+            ++m_step;
+        } else {
+            // capture this step's launches, then run it from the graph
+            if (m_graphs.size() >= 12) drop_graphs();
+            const StepPtrs before = save_ptrs();
+            const int step0 = m_step;
+            hipGraph_t g = nullptr;
+            if (!RX_HIP(hipStreamBeginCapture(m_stream, hipStreamCaptureModeThreadLocal))) return 1;
+            rc = enqueue_step(dt);
+            const bool ended = RX_HIP(hipStreamEndCapture(m_stream, &g));
+            hipGraphExec_t exec = nullptr;
+            const bool ok = rc == 0 && ended && g && RX_HIP(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+            if (g) RX_HIP(hipGraphDestroy(g));
+            if (!ok) {  // undo the host side of the captured step and run it eagerly
+                load_ptrs(before);
+                m_step = step0;
+                m_use_graph = false;
+                rc = enqueue_step(dt);
+            } else {
+                m_graphs.push_back(StepGraph{key, save_ptrs(), exec});
+                if (!RX_HIP(hipGraphLaunch(exec, m_stream))) return 1;
+            }
+        }
+    }
+    if (rc) return rc;
+    if (!RX_HIP(hipGetLastError())) return 1;
+    if (m_sync_each_step && !RX_HIP(hipStreamSynchronize(m_stream))) return 1;
+    return 0;
+}
+
+// The launches of one step on the solver's stream (eager, or recorded into a graph by Update).
+int rxPBDSPH_GPU::enqueue_step(float dt) {
     hipStream_t s = m_stream;
     PbfK k = consts(dt);
     uint n = m_n;
@@ -1102,8 +1184,6 @@ int rxPBDSPH_GPU::Update(float dt, int step) {
     }
     m_have_list = true;
     ++m_step;
-    if (!RX_HIP(hipGetLastError())) return 1;
-    if (m_sync_each_step && !RX_HIP(hipStreamSynchronize(s))) return 1;
     return 0;
 }
 
@@ -1164,6 +1244,7 @@ int rxPBDSPH_GPU::GetArray(int type, float* host, int count) {
 // Thermal state in original particle order: temperature (reference scale, melting at 250), heat, phase (-2 ice,
 // -1 melting, 1 freezing, 2 water; NULL: -2 below 250, 2 otherwise).  NULL temps removes the thermal state.
 int rxPBDSPH_GPU::SetThermal(const float* temps, const float* heats, const int* phase) {
+    drop_graphs();
     if (!m_n) {
         rx_set_error(1, "rxPBDSPH_GPU::SetThermal: set the particles first");
         return 1;
@@ -1214,6 +1295,7 @@ int rxPBDSPH_GPU::GetStats(float* out, int count) {
 int rxPBDSPH_GPU::Sync() { return RX_HIP(hipStreamSynchronize(m_stream)) ? 0 : 1; }
 
 int rxPBDSPH_GPU::SetNetwork(void* mlp) {
+    drop_graphs();
     if (!rx_mlp_view(mlp, &m_mlp)) {
         rx_set_error(1, "rxPBDSPH_GPU::SetNetwork: null network");
         m_mlp_ready = false;
@@ -1314,6 +1396,10 @@ int rxPBDSPH_GPU_FeatureScales(void* h, float* sd_ref, float* nref) {
 int rxPBDSPH_GPU::SetOption(int opt, float value) {
     switch (opt) {
     case RX_PBF_OPT_SYNC_EACH_STEP: m_sync_each_step = value != 0.0f; return 0;
+    case RX_PBF_OPT_GRAPH:
+        m_use_graph = value != 0.0f;
+        if (!m_use_graph) drop_graphs();
+        return 0;
     case RX_PBF_OPT_RECORD_LAMBDA: return 0;  // the applied multiplier is always accumulated
     case RX_PBF_OPT_RECORD_FEATURES: m_record_features = value != 0.0f; return 0;
     case RX_PBF_OPT_MODE:
@@ -1351,6 +1437,7 @@ int rxPBDSPH_GPU::SetOption(int opt, float value) {
             return 1;
         }
         if (m != m_maxn && m_list) {
+            drop_graphs();
             RX_HIP(hipStreamSynchronize(m_stream));
             RX_HIP(hipFree(m_list));
             m_list = nullptr;

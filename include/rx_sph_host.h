@@ -105,7 +105,10 @@ enum rxPbfOption {
                                       cells, 24 KB, 128 threads; 4 = 16-cell segments without the tile     */
     RX_PBF_OPT_HEAT_TD = 14,       /* thermal diffusion coefficient (HeatTransfar mTD; default 0)         */
     RX_PBF_OPT_HEAT_DT = 15,       /* heat time step (0: the step's dt)                                    */
-    RX_PBF_OPT_MAX_NEIGHBORS = 16  /* neighbour-list capacity per particle, a multiple of 4 (default 96)    */
+    RX_PBF_OPT_MAX_NEIGHBORS = 16, /* neighbour-list capacity per particle, a multiple of 4 (default 96)    */
+    RX_PBF_OPT_GRAPH = 17          /* 1: after the first step, each step's launches are recorded once into a
+                                      hipGraph per buffer configuration and replayed (for launch-bound sizes);
+                                      profiled steps (RX_PBF_OPT_PROFILE) run eagerly.  0 (default): eager */
 };
 
 /* Kernel classes of rxPBDSPH_GPU_GetProfile. */
