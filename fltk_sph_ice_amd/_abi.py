@@ -163,6 +163,8 @@ SIGNATURES: dict[str, tuple] = {
     "CuGetLastErrorString": (C.c_char_p, []),
     "CuClearLastError": (None, []),
     "CuGetStream": (C.c_void_p, []),
+    "rxParticlesWrite": (i32, [C.c_char_p, vp, vp, i32]),
+    "rxParticlesRead": (i32, [C.c_char_p, vp, vp, i32]),
     # entry points of rx_cu_funcs.cuh:138-253 exported as error-reporting stubs (include/rx_cu_unsupported.h)
     "CuSphCalUpdatedPosition": (None, []),
     "CuSphCalCovarianceMatrix": (None, []),

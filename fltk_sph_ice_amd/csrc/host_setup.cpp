@@ -1,8 +1,11 @@
 // host_setup.cpp -- scene parameters -> rxSimParams, ABI layout probe and the seeded scene generator.
 #include <math.h>
+#include <stdint.h>
+#include <stdio.h>
 #include <stddef.h>
 #include <string.h>
 
+#include <algorithm>
 #include <random>
 
 #include "rx_common.h"
@@ -166,6 +169,72 @@ int rxSceneBox(float* pos4, float* vel4, int max_n, const float origin[3], int n
                 ++n;
             }
     return n;
+}
+
+// rxParticleSystemBase::OutputParticles (rx_ps.cpp:581-607): uint32 n, then x y z of every particle, then the
+// velocities, all float32 in the host's byte order (little-endian on x86-64).
+int rxParticlesWrite(const char* path, const float* pos4, const float* vel4, int n) {
+    if (!path || !pos4 || n < 0) {
+        rx_set_error(1, "rxParticlesWrite: bad arguments");
+        return 1;
+    }
+    FILE* f = fopen(path, "wb");
+    if (!f) {
+        rx_set_error(1, "rxParticlesWrite: %s couldn't open", path);
+        return 1;
+    }
+    const uint32_t un = (uint32_t)n;
+    bool ok = fwrite(&un, sizeof(un), 1, f) == 1;
+    for (int pass = 0; pass < 2 && ok; ++pass) {
+        const float* a = pass == 0 ? pos4 : vel4;
+        for (int i = 0; i < n && ok; ++i) {
+            const float zero[3] = {0.0f, 0.0f, 0.0f};
+            ok = fwrite(a ? a + 4 * (size_t)i : zero, sizeof(float), 3, f) == 3;
+        }
+    }
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) rx_set_error(1, "rxParticlesWrite: write to %s failed", path);
+    return ok ? 0 : 1;
+}
+
+// rxParticleSystemBase::InputParticles (rx_ps.cpp:615-644): returns the particle count stored in the file
+// (-1 on error).  Reads at most max_n particles into pos4/vel4 (float4, w = 0); a file that ends after the
+// positions leaves the velocities zero.  max_n = 0 (or null arrays) only returns the count.
+int rxParticlesRead(const char* path, float* pos4, float* vel4, int max_n) {
+    FILE* f = path ? fopen(path, "rb") : nullptr;
+    if (!f) {
+        rx_set_error(1, "rxParticlesRead: %s couldn't find", path ? path : "(null)");
+        return -1;
+    }
+    uint32_t n = 0;
+    if (fread(&n, sizeof(n), 1, f) != 1 || n > 0x7FFFFFFFu) {
+        fclose(f);
+        rx_set_error(1, "rxParticlesRead: %s has no particle count", path);
+        return -1;
+    }
+    const int m = (pos4 && max_n > 0) ? (int)std::min<uint32_t>(n, (uint32_t)max_n) : 0;
+    bool ok = true;
+    for (int pass = 0; pass < 2 && ok; ++pass) {
+        float* a = pass == 0 ? pos4 : vel4;
+        for (uint32_t i = 0; i < n; ++i) {
+            float v[3];
+            if (fread(v, sizeof(float), 3, f) != 3) {
+                ok = pass == 1 && i == 0;  // no velocities at all is accepted
+                if (pass == 1 && ok && vel4)
+                    for (int j = 0; j < m; ++j) vel4[4 * j] = vel4[4 * j + 1] = vel4[4 * j + 2] = vel4[4 * j + 3] = 0.0f;
+                break;
+            }
+            if (a && (int)i < m) {
+                a[4 * i] = v[0], a[4 * i + 1] = v[1], a[4 * i + 2] = v[2], a[4 * i + 3] = 0.0f;
+            }
+        }
+    }
+    fclose(f);
+    if (!ok) {
+        rx_set_error(1, "rxParticlesRead: %s ends inside the particle data", path);
+        return -1;
+    }
+    return (int)n;
 }
 
 }  // extern "C"

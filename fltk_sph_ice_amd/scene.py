@@ -7,6 +7,9 @@ reference application (rx_fltk_glcanvas.cpp:193).
 from __future__ import annotations
 
 import ctypes as C
+import math
+import os
+import re
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -39,10 +42,13 @@ class Scene:
     ap_n: float = 4.0
     ap_q: float = 0.2
     xsph_c: float = 0.0
+    mass: float = 0.0        # 0: solved from the lattice sum at `spacing` so that the lattice has rest density
     extra: dict = field(default_factory=dict)
 
     @property
     def n(self) -> int:
+        if "blocks" in self.extra:
+            return int(sum(np.prod(b[0]) for b in self.extra["blocks"]))
         return int(np.prod(self.block))
 
     def env(self) -> rxSphEnv:
@@ -56,7 +62,7 @@ class Scene:
             e.gravity[k] = self.gravity[k]
         e.h = self.h
         e.rest_dens = self.rest_dens
-        e.mass = 0.0
+        e.mass = self.mass
         e.spacing = self.spacing
         e.particle_radius = 0.0
         e.dt = self.dt
@@ -83,6 +89,10 @@ class Scene:
 
     def particles(self) -> tuple[np.ndarray, np.ndarray]:
         """(pos4, vel4) float32 arrays of shape (n, 4)."""
+        if "blocks" in self.extra:  # several boxes (scene files): one seeded lattice each, velocity per box
+            parts = [_box(blk, org, self.spacing, self.jitter, self.vel_sigma, self.seed + t, v0)
+                     for t, (blk, org, v0) in enumerate(self.extra["blocks"])]
+            return np.concatenate([p for p, _ in parts]), np.concatenate([v for _, v in parts])
         lib = load()
         n = self.n
         pos = np.zeros((n, 4), np.float32)
@@ -266,4 +276,112 @@ def box_surface_particles(box_min, box_max, spacing: float) -> np.ndarray:
     out = np.zeros((int(on.sum()), 4), np.float32)
     out[:, :3] = g[on]
     return out
+
+
+def _box(block, origin, spacing, jitter, vel_sigma, seed, v0):
+    lib = load()
+    n = int(np.prod(block))
+    pos = np.zeros((n, 4), np.float32)
+    vel = np.zeros((n, 4), np.float32)
+    org = (C.c_float * 3)(*origin)
+    got = lib.rxSceneBox(pos.ctypes.data, vel.ctypes.data, n, org, *block, spacing, jitter, vel_sigma, seed)
+    assert got == n
+    vel[:, :3] += np.asarray(v0, np.float32)
+    return pos, vel
+
+
+# --------------------------------------------------------------------------------------------- scene files
+_NUM = re.compile(r"[-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?")
+
+
+def read_config(path) -> list[tuple[str, dict]]:
+    """Sections of a scene file in the reference's INI form (rxINI, read by rxSPHConfig, rx_sph_config.h:428-470):
+    `[header]` lines open a section, `name = value` lines fill it; `#` and `//` start comments."""
+    out: list[tuple[str, dict]] = []
+    for raw in open(path, encoding="utf-8", errors="replace"):
+        line = raw.split("//")[0].split("#")[0].strip()
+        if not line:
+            continue
+        if line.startswith("[") and line.endswith("]"):
+            out.append((line[1:-1].strip(), {}))
+        elif "=" in line and out:
+            k, v = line.split("=", 1)
+            out[-1][1][k.strip()] = v.strip()
+    return out
+
+
+def _nums(v: str) -> list[float]:
+    return [float(x) for x in _NUM.findall(v)]
+
+
+def scene_from_config(path, seed: int = 1, name: str | None = None) -> Scene:
+    """A Scene from a scene file: the `[space]` section (rxSPHConfig::SetSphSpace, rx_sph_config.h:193-262) and
+    the `[liquid box]` / `[liquid box (r)]` sections (SetLiquidBox, :266-279; `(r)`: centre and extent relative
+    to the space, in units of its half extent).
+
+    Parameters follow rxPBDSPH_GPU::Initialize (rx_sph_pbd_gpu.cpp:166-200): h = (3 k m / (4 pi rho))^(1/3) for
+    k kernel particles of mass m at density rho, particle radius r = (pi / (6 k))^(1/3) h.  A liquid box is the
+    lattice of rxParticleSystemBase::AddBox (rx_ps.cpp:195-261) at spacing 2r: 2 int(ext / 2r) - 1 particles per
+    axis centred on the box, jitter 0.001 of the spacing (seeded here), the box's velocity.  Defaults are those of
+    rxSPHEnviroment (rx_sph.h:146-180)."""
+    sections = read_config(path)
+    space = next((kv for hd, kv in sections if hd == "space"), None)
+    if space is None:
+        raise ValueError(f"{path}: no [space] section")
+    cen = np.array(_nums(space.get("cen", "0 0 0")) or [0, 0, 0], np.float64)
+    ext = np.array(_nums(space.get("ext", "2.0 0.8 0.8")), np.float64)
+    rho = float(space.get("density", 998.29))
+    m = float(space.get("mass", 0.04))
+    kp = float(space.get("kernel_particles", 20.0))
+    h = (3.0 * kp * m / rho / (4.0 * math.pi)) ** (1.0 / 3.0)
+    r = (math.pi / (6.0 * kp)) ** (1.0 / 3.0) * h
+    spacing = 2.0 * r
+    blocks = []
+    for hd, kv in sections:
+        if hd not in ("liquid box", "liquid box (r)"):
+            continue
+        bc = np.array(_nums(kv.get("cen", "0 0 0")), np.float64)
+        be = np.array(_nums(kv.get("ext", "0 0 0")), np.float64)
+        if hd.endswith("(r)"):
+            bc, be = cen + bc * ext, be * ext
+        v0 = tuple(_nums(kv.get("vel", "0 0 0")) or [0.0, 0.0, 0.0])
+        sx = [int(be[k] / spacing) - 1 for k in range(3)]
+        if min(sx) < 0:
+            continue
+        blk = tuple(2 * v + 1 for v in sx)
+        org = tuple(float(bc[k] - (sx[k] + 0.5) * spacing) for k in range(3))
+        blocks.append((blk, org, v0))
+    if not blocks:
+        raise ValueError(f"{path}: no liquid box with particles")
+    sc = Scene(name or os.path.splitext(os.path.basename(str(path)))[0], tuple(cen - ext), tuple(cen + ext), h, spacing,
+               blocks[0][0], blocks[0][1], 0.001 * spacing, 0.0, seed, float(space.get("dt", 0.01)), rest_dens=rho,
+               gas_k=float(space.get("gas_stiffness", 3.0)), viscosity=float(space.get("viscosity", 1.0e-3)),
+               epsilon=float(space.get("epsilon", 0.001)), iterations=int(space.get("max_iterations", 10)),
+               use_ap=int(space.get("use_artificial_pressure", 1)), ap_k=float(space.get("ap_k", 0.1)),
+               ap_n=float(space.get("ap_n", 4.0)), ap_q=float(space.get("ap_q", 0.2)), mass=m)
+    sc.extra["blocks"] = blocks
+    return sc
+
+
+# --------------------------------------------------------------------------------------------- particle files
+def write_particles(path, pos4: np.ndarray, vel4: np.ndarray | None = None) -> None:
+    """rxParticleSystemBase::OutputParticles format (rxParticlesWrite)."""
+    lib = load()
+    p = np.ascontiguousarray(pos4, np.float32)
+    v = None if vel4 is None else np.ascontiguousarray(vel4, np.float32)
+    if lib.rxParticlesWrite(str(path).encode(), p.ctypes.data, None if v is None else v.ctypes.data, len(p)):
+        check(lib)
+
+
+def read_particles(path) -> tuple[np.ndarray, np.ndarray]:
+    """rxParticleSystemBase::InputParticles format (rxParticlesRead): (pos4, vel4)."""
+    lib = load()
+    n = lib.rxParticlesRead(str(path).encode(), None, None, 0)
+    if n < 0:
+        check(lib)
+    pos = np.zeros((n, 4), np.float32)
+    vel = np.zeros((n, 4), np.float32)
+    if lib.rxParticlesRead(str(path).encode(), pos.ctypes.data, vel.ctypes.data, n) != n:
+        check(lib)
+    return pos, vel
 

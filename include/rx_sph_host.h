@@ -58,6 +58,13 @@ long rxAbiLayout(int which);
 int rxSceneBox(float* pos4, float* vel4, int max_n, const float origin[3], int nx, int ny, int nz, float spacing,
                float jitter, float vel_sigma, unsigned seed);
 
+/* Particle files of rxParticleSystemBase::OutputParticles / InputParticles (rx_ps.cpp:581-644): uint32 n, the
+ * n positions (3 float32 each), then the n velocities.  Write: vel4 may be NULL (zeros are written); returns 0
+ * on success.  Read: returns the count in the file (-1 on error) and fills at most max_n particles (float4, w = 0);
+ * a file that ends after the positions gives zero velocities. */
+int rxParticlesWrite(const char* path, const float* pos4, const float* vel4, int n);
+int rxParticlesRead(const char* path, float* pos4, float* vel4, int max_n);
+
 /* ---------------------------------------------------------------- solver objects */
 /* Arrays returned by the *_GetArray entry points (original particle order). */
 enum rxArrayType {
