@@ -126,61 +126,80 @@ __device__ inline f32x4 forward16(bf16x4 a1, const MlpDev& m, unsigned short* ld
     return p;
 }
 
-// Inference in transposed form, no LDS: with the weights as the A operand and the activations as the B
-// operand, H^T = W X^T leaves lane l holding H[row l % 16][units 4 (l / 16) .. + 3] of a 16-unit tile, which is
-// the B fragment of the next layer's 16-deep K step.  Every lane keeps the weight fragments in registers
-// and a wave walks 16-row tiles with a grid stride.
+// Four floats to bf16 with the hardware's round-to-nearest-even conversion (two v_cvt_pk_bf16_f32); the same
+// rounding as f2bf.
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef unsigned u32x2_t __attribute__((ext_vector_type(2)));
+__device__ inline bf16x4 pack4_hw(float a, float b, float c, float d) {
+    const f32x2_t lo = {a, b}, hi = {c, d};
+    const u32x2_t u = {__builtin_bit_cast(unsigned, __builtin_convertvector(lo, bf16x2_t)),
+                       __builtin_bit_cast(unsigned, __builtin_convertvector(hi, bf16x2_t))};
+    return __builtin_bit_cast(bf16x4, u);
+}
+
+// Inference in transposed form: with the weights as the A operand and the activations as the B operand,
+// H^T = W X^T leaves lane l holding H[row l % 16][units 4 (l / 16) .. + 3] of a 16-unit tile, which is the B
+// fragment of the next layer's 16-deep K step.  Every lane keeps the weight fragments in registers; biases and
+// the output weights sit in LDS (read as float4 per 4 units).  A wave walks 16-row tiles with a grid stride and
+// loads the next tile's features before computing the current one.
 __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ X, uint n, MlpDev m, float* __restrict__ y,
                                                    float4* __restrict__ xl, float w_scale) {
-    const int l = threadIdx.x & 63, i = l & 15, kb = l >> 4;
-    bf16x4 w1[4], w2[4][4];
-    float b1[4][4], b2[4][4], w3[4][4], mu[4], isd[4];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        w1[t] = ld_bf4(m.w1 + (16 * t + i) * kMlpIn + 4 * kb);  // A = W1[16t + i][4kb + e]
-#pragma unroll
-        for (int ks = 0; ks < 4; ++ks) w2[t][ks] = ld_bf4(m.w2 + (16 * t + i) * kMlpH + 16 * ks + 4 * kb);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            int o = 16 * t + 4 * kb + e;  // unit held by this lane in D
-            b1[t][e] = m.b1[o];
-            b2[t][e] = m.b2[o];
-            w3[t][e] = m.w3[o];
-        }
+    __shared__ __align__(16) float sb[3 * kMlpH];  // b1, b2, w3
+    constexpr int kS = kMlpH + 8;  // padded row: the 16 rows a fragment read touches fall in different banks
+    __shared__ __align__(16) unsigned short sw2[kMlpH * kS];  // W2, bf16 [unit][input]
+    for (int t = threadIdx.x; t < 3 * kMlpH; t += blockDim.x)
+        sb[t] = t < kMlpH ? m.b1[t] : (t < 2 * kMlpH ? m.b2[t - kMlpH] : m.w3[t - 2 * kMlpH]);
+    for (int t = threadIdx.x; t < kMlpH * kMlpH / 4; t += blockDim.x) {
+        const int row = t / (kMlpH / 4), c4 = t % (kMlpH / 4);
+        ((uint2*)(sw2 + row * kS))[c4] = ((const uint2*)m.w2)[t];
     }
+    __syncthreads();
+    const int l = threadIdx.x & 63, i = l & 15, kb = l >> 4;
+    bf16x4 w1[4];
+    float mu[4], isd[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) w1[t] = ld_bf4(m.w1 + (16 * t + i) * kMlpIn + 4 * kb);  // A = W1[16t + i][4kb + e]
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         mu[e] = m.in_mu[4 * kb + e];
         isd[e] = m.in_isd[4 * kb + e];
     }
     const float b3 = m.b3[0];
+    const float4* sb4 = (const float4*)sb;  // sb4[4t + kb] = units 16t + 4kb .. + 3
     const uint waves = gridDim.x * (blockDim.x >> 6);
-    for (uint tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); (size_t)tile * 16 < n; tile += waves) {
-        uint r = tile * 16 + i;
-        float v[4] = {0, 0, 0, 0};
-        if (r < n) {
-            float4 x = *(const float4*)(X + (size_t)r * kMlpIn + 4 * kb);
-            v[0] = (x.x - mu[0]) * isd[0];
-            v[1] = (x.y - mu[1]) * isd[1];
-            v[2] = (x.z - mu[2]) * isd[2];
-            v[3] = (x.w - mu[3]) * isd[3];
-        }
-        const bf16x4 xb = pack4(v[0], v[1], v[2], v[3]);  // B = X^T[4kb + e][row i]
+    uint tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    auto load = [&](uint tl) {
+        const uint r = tl * 16 + i;
+        return r < n ? *(const float4*)(X + (size_t)r * kMlpIn + 4 * kb) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    };
+    float4 x = load(tile);
+    for (; (size_t)tile * 16 < n; tile += waves) {
+        const uint r = tile * 16 + i;
+        const float4 xn = load(tile + waves);  // the next tile's features, in flight during this tile
+        // an offset the compiler cannot see through keeps the bias reads in the loop (hoisted, they would hold
+        // 48 registers across it)
+        uint o = (uint)kb;
+        asm volatile("" : "+v"(o));
+        const bf16x4 xb = pack4_hw((x.x - mu[0]) * isd[0], (x.y - mu[1]) * isd[1], (x.z - mu[2]) * isd[2],
+                                   (x.w - mu[3]) * isd[3]);  // B = X^T[4kb + e][row i]; rows past n are 0
         bf16x4 h1[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             f32x4 d = {0, 0, 0, 0};
             d = mfma(w1[t], xb, d);  // D[unit 4kb + e][row i] of units 16t..16t+15
-            h1[t] = pack4(silu(d[0] + b1[t][0]), silu(d[1] + b1[t][1]), silu(d[2] + b1[t][2]), silu(d[3] + b1[t][3]));
+            const float4 b = sb4[4 * t + o];
+            h1[t] = pack4_hw(silu(d[0] + b.x), silu(d[1] + b.y), silu(d[2] + b.z), silu(d[3] + b.w));
         }
         float p = 0.0f;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             f32x4 acc = {0, 0, 0, 0};
 #pragma unroll
-            for (int ks = 0; ks < 4; ++ks) acc = mfma(w2[t][ks], h1[ks], acc);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) p += silu(acc[e] + b2[t][e]) * w3[t][e];
+            for (int ks = 0; ks < 4; ++ks)  // A = W2[16t + i][16ks + 4kb + e] from LDS
+                acc = mfma(*(const bf16x4*)(sw2 + (16 * t + i) * kS + 16 * ks + 4 * o), h1[ks], acc);
+            const float4 b = sb4[16 + 4 * t + o], w = sb4[32 + 4 * t + o];
+            p += silu(acc[0] + b.x) * w.x + silu(acc[1] + b.y) * w.y + silu(acc[2] + b.z) * w.z + silu(acc[3] + b.w) * w.w;
         }
         p += __shfl_xor(p, 16, 64);
         p += __shfl_xor(p, 32, 64);
@@ -189,6 +208,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ X, 
             if (y) y[r] = out;
             if (xl) xl[r].w = out * w_scale;
         }
+        x = xn;
     }
 }
 
