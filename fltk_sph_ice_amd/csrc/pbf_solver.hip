@@ -30,6 +30,7 @@ struct PbfK {
     float half_skin2; // (skin / 2)^2
     float skin;       // list cutoff minus h
     float wide2;      // largest displacement^2 the capped fallback walk still covers
+    uint* fallback;   // counts the passes of particles that walked the grid instead of their list
     int maxn;
     float eps;
     float dt, inv_dt;
@@ -147,15 +148,13 @@ __device__ inline float3 clamp_wall(float3 x, const SphK& s) {
 // build positions.
 // Walks the cells within `rad` of cell c (rows of contiguous slots), fetching four candidates ahead of their use:
 // the loads of a group are issued together, so a wavefront keeps several gathers in flight.
+// The box of cells [lo, hi] (inclusive, inside the grid).
 template <class Fetch, class Body>
-__device__ inline void stencil_batched(int3 c, const FGridK& g, const uint* __restrict__ off, Fetch&& fetch, Body&& body, int rad) {
-    int x0 = max(c.x - rad, 0), x1 = min(c.x + rad, g.gx - 1);
-    int y0 = max(c.y - rad, 0), y1 = min(c.y + rad, g.gy - 1);
-    int z0 = max(c.z - rad, 0), z1 = min(c.z + rad, g.gz - 1);
-    for (int z = z0; z <= z1; ++z)
-        for (int y = y0; y <= y1; ++y) {
+__device__ inline void stencil_box(int3 lo, int3 hi, const FGridK& g, const uint* __restrict__ off, Fetch&& fetch, Body&& body) {
+    for (int z = lo.z; z <= hi.z; ++z)
+        for (int y = lo.y; y <= hi.y; ++y) {
             uint hb = (uint)((z * g.gy + y) * g.gx);
-            uint j = off[hb + x0], e = off[hb + x1 + 1];
+            uint j = off[hb + lo.x], e = off[hb + hi.x + 1];
             for (; j + 3 < e; j += 4) {
                 auto a = fetch(j);
                 auto b = fetch(j + 1);
@@ -168,6 +167,12 @@ __device__ inline void stencil_batched(int3 c, const FGridK& g, const uint* __re
             }
             for (; j < e; ++j) body(j, fetch(j));
         }
+}
+
+template <class Fetch, class Body>
+__device__ inline void stencil_batched(int3 c, const FGridK& g, const uint* __restrict__ off, Fetch&& fetch, Body&& body, int rad) {
+    stencil_box(make_int3(max(c.x - rad, 0), max(c.y - rad, 0), max(c.z - rad, 0)),
+                make_int3(min(c.x + rad, g.gx - 1), min(c.y + rad, g.gy - 1), min(c.z + rad, g.gz - 1)), g, off, fetch, body);
 }
 
 // Neighbour candidates of slot i for a pass that needs every j with |x_i - x_j| <= h at the current positions;
@@ -278,9 +283,13 @@ __device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint
     if (ok) {
         list_walk<G>((const uint4*)list + i, hd, k.n, fetch, body);
     } else {
-        int rad = (int)ceilf((k.s.h + di + sqrtf(disp2)) * k.g.inv_cell * 1.0001f);
-        rad = min(max(rad, 1), kMaxStencil);
-        stencil_batched(fg_unhash(bhash[i], k.g), k.g, off, fetch, body, rad);
+        // every j within L of i's build position: the cells that the box [x_b - L, x_b + L] touches (3 or 4 per
+        // axis once L exceeds a cell, not the 5 of a cube of radius 2 around the cell)
+        const uint64_t wave = __ballot(1);
+        if (__lane_id() == (uint)__builtin_ctzll(wave)) atomicAdd(k.fallback, (uint)__builtin_popcountll(wave));
+        const float L = fminf((k.s.h + di + sqrtf(disp2)) * 1.0001f, kMaxStencil * k.g.cell);
+        const float3 c = f3(xb[i]);
+        stencil_box(fg_pos(c - f3(L, L, L), k.g), fg_pos(c + f3(L, L, L), k.g), k.g, off, fetch, body);
     }
 }
 
@@ -963,6 +972,7 @@ PbfK rxPBDSPH_GPU::consts(float dt) const {
     k.skin = m_skin;
     float wide = 0.5f * ((float)kMaxStencil * m_grid.k.cell - m_env.h);  // largest fallback reach minus h, per particle
     k.wide2 = wide * wide;
+    k.fallback = (uint*)(m_stats + ST_FALLBACK);
     k.maxn = m_maxn;
     k.eps = m_env.epsilon;
     k.dt = dt;
@@ -1281,6 +1291,7 @@ int rxPBDSPH_GPU::GetStats(float* out, int count) {
     if (!RX_HIP(hipStreamSynchronize(m_stream))) return 1;
     // uint counters are returned as floats
     h[ST_OVERFLOW] = (float)__builtin_bit_cast(uint, h[ST_OVERFLOW]);
+    h[ST_FALLBACK] = (float)__builtin_bit_cast(uint, h[ST_FALLBACK]);
     h[ST_ITERS] = (float)__builtin_bit_cast(uint, h[ST_ITERS]);
     h[ST_BAD] = (float)__builtin_bit_cast(uint, h[ST_BAD]);
     h[ST_MEAN_ABS_START] = h[ST_MEAN_ABS_START] / (float)m_n;
