@@ -163,6 +163,16 @@ SIGNATURES: dict[str, tuple] = {
     "CuGetLastErrorString": (C.c_char_p, []),
     "CuClearLastError": (None, []),
     "CuGetStream": (C.c_void_p, []),
+    # rx_cu_mesh.h (marching cubes; C++ references are passed as pointers)
+    "CuMCInit": (C.c_bool, []),
+    "CuInitMCTable": (None, []),
+    "CuCleanMCTable": (None, []),
+    "CuMCCalTriNum": (None, [vp, vp, vp, vp, vp, vp, vp, vp, vp, uint3, u32, float3, flt, C.POINTER(u32), C.POINTER(u32),
+                             C.POINTER(u32)]),
+    "CuMCCalEdgeVrts": (None, [vp, vp, vp, vp, vp, C.POINTER(uint3), C.POINTER(u32), uint3, u32, float3, float3, flt,
+                               C.POINTER(u32)]),
+    "CuMCCalTri": (None, [vp, vp, vp, vp, vp, C.POINTER(uint3), C.POINTER(u32), uint3, u32, float3, flt, u32, u32, u32]),
+    "CuMCCalNrm": (None, [vp, vp, vp, u32, u32]),
     "rxParticlesWrite": (i32, [C.c_char_p, vp, vp, i32]),
     "rxParticlesRead": (i32, [C.c_char_p, vp, vp, i32]),
     # entry points of rx_cu_funcs.cuh:138-253 exported as error-reporting stubs (include/rx_cu_unsupported.h)
@@ -190,13 +200,6 @@ SIGNATURES: dict[str, tuple] = {
     "CuSubCheckRatio": (None, []),
     "CuCheckNumValidHashData": (u32, []),
     "CuMCCreateMesh": (None, []),
-    "CuMCInit": (C.c_bool, []),
-    "CuInitMCTable": (None, []),
-    "CuCleanMCTable": (None, []),
-    "CuMCCalTriNum": (None, []),
-    "CuMCCalEdgeVrts": (None, []),
-    "CuMCCalTri": (None, []),
-    "CuMCCalNrm": (None, []),
     "CuInitFloatArray": (None, []),
     "CuInitRadArray": (None, []),
     "CuInitDepthMap": (None, []),

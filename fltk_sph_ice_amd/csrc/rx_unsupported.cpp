@@ -35,13 +35,6 @@ void CuSubSphGridDensity() { unsupported("CuSubSphGridDensity"); }
 void CuSubCheckRatio() { unsupported("CuSubCheckRatio"); }
 unsigned int CuCheckNumValidHashData() { unsupported("CuCheckNumValidHashData"); return 0u; }
 void CuMCCreateMesh() { unsupported("CuMCCreateMesh"); }
-bool CuMCInit() { unsupported("CuMCInit"); return false; }
-void CuInitMCTable() { unsupported("CuInitMCTable"); }
-void CuCleanMCTable() { unsupported("CuCleanMCTable"); }
-void CuMCCalTriNum() { unsupported("CuMCCalTriNum"); }
-void CuMCCalEdgeVrts() { unsupported("CuMCCalEdgeVrts"); }
-void CuMCCalTri() { unsupported("CuMCCalTri"); }
-void CuMCCalNrm() { unsupported("CuMCCalNrm"); }
 void CuInitFloatArray() { unsupported("CuInitFloatArray"); }
 void CuInitRadArray() { unsupported("CuInitRadArray"); }
 void CuInitDepthMap() { unsupported("CuInitDepthMap"); }

@@ -1,0 +1,61 @@
+/* rx_cu_mesh.h -- surface extraction downstream of the particle path: the marching-cubes entry points of
+ * rx_cu_funcs.cuh:184-212 in their RX_CUMC_USE_GEOMETRY form (rx_cu_common.cuh:35), implemented by HIP kernels
+ * for gfx950 (fltk_sph_ice_amd/csrc/mc.hip).
+ *
+ * Same names, argument order and meaning as the reference; C++ linkage rules as there (some outputs are
+ * references, which the C ABI passes as pointers).  All arrays are device arrays the caller allocates with the
+ * sizes rxMCMeshGPU::Set gives them (rx_mc_gpu.cpp:104-215); every call returns after its device work is done.
+ *
+ * Lattice of the volume: grid_size = (nx, ny, nz) samples, x fastest; voxel i has its minimum corner at sample i,
+ * corners past the last sample read the last sample.  Edges: edge_size[0] = (nx, ny+1, nz+1) x-edges, then
+ * edge_size[1] = (nx+1, ny, nz+1) y-edges and edge_size[2] = (nx+1, ny+1, nz) z-edges, concatenated; num_edge[d]
+ * is the product of edge_size[d], num_edge[3] their sum.  Vertex positions are grid_min + index * grid_width. */
+#ifndef RX_CU_MESH_H_
+#define RX_CU_MESH_H_
+
+#include "rx_cu_common.h"
+
+#ifdef __cplusplus
+extern "C" {
+
+/* rx_cu_funcs.cuh:192 (rx_mc.cu:49-65): selects nothing and returns true -- the device is the caller's current one. */
+bool CuMCInit(void);
+/* rx_cu_funcs.cuh:194-195 (rx_mc.cu:88-121): the case tables are compiled into the library; both are no-ops. */
+void CuInitMCTable(void);
+void CuCleanMCTable(void);
+
+/* rx_cu_funcs.cuh:197-200 (rx_mc.cu:148-231).  Per voxel: dVoxBit = case index (bit k: corner k < threshold),
+ * dVoxVNum = 3 x triangles, dVoxTNum = triangles, dVoxOcc = (triangles > 0).  dVoxOccScan, dVoxVNumScan,
+ * dVoxTNumScan are their exclusive scans, dCompactedVox the indices of the occupied voxels in order.
+ * Returns the occupied voxel count, the sum of dVoxVNum and the sum of dVoxTNum; with no occupied voxel all
+ * three are 0 and the vertex and triangle scans are not computed. */
+void CuMCCalTriNum(float* dVolume, uint* dVoxBit, uint* dVoxVNum, uint* dVoxVNumScan,
+                   uint* dVoxTNum, uint* dVoxTNumScan, uint* dVoxOcc, uint* dVoxOccScan, uint* dCompactedVox,
+                   uint3 grid_size, uint num_voxels, float3 grid_width, float threshold,
+                   uint& num_active_voxels, uint& nvrts, uint& ntris);
+
+/* rx_cu_funcs.cuh:201-204 (rx_mc.cu:252-315).  One vertex per edge whose two samples lie on different sides of
+ * the threshold: dEdgeVrts[e] = (p0 + t (p1 - p0), 1), t = (threshold - f0) / (f1 - f0) (entries of other
+ * edges are left untouched), dEdgeOcc[e] = 1 for those edges, dEdgeOccScan its exclusive scan,
+ * dCompactedEdgeVrts the occupied entries in edge order.  Returns the vertex count in nvrts. */
+void CuMCCalEdgeVrts(float* dVolume, float* dEdgeVrts, float* dCompactedEdgeVrts,
+                     uint* dEdgeOcc, uint* dEdgeOccScan, uint3 edge_size[3], uint num_edge[4],
+                     uint3 grid_size, uint num_voxels, float3 grid_width, float3 grid_min, float threshold, uint& nvrts);
+
+/* rx_cu_funcs.cuh:205-208 (rx_mc.cu:332-356).  dTris (uint3 per triangle): triangle t of occupied voxel v is
+ * written at dVoxTNumScan[v] + t as (V(e1), V(e2), V(e0)) for the case's edges (e0, e1, e2), V(e) =
+ * dEdgeOccScan at that edge, clamped to nvrts - 1; writes at indices >= ntris are dropped. */
+void CuMCCalTri(uint* dTris, uint* dVoxBit, uint* dVoxTNumScan, uint* dCompactedVox,
+                uint* dEdgeOccScan, uint3 edge_size[3], uint num_edge[4],
+                uint3 grid_size, uint num_voxels, float3 grid_width, float threshold,
+                uint num_active_voxels, uint nvrts, uint ntris);
+
+/* rx_cu_funcs.cuh:209 (rx_mc.cu:366-389).  dNrms (3 floats per vertex): the sum over the triangles that use the
+ * vertex of (v1 - v0) x (v2 - v0), divided by its length (left at zero when the sum is zero).  The sum is taken
+ * in triangle order, so the result is the same on every run. */
+void CuMCCalNrm(float* dNrms, uint* dTris, float* dCompactedEdgeVrts, uint nvrts, uint ntris);
+
+}
+#endif /* __cplusplus */
+
+#endif /* RX_CU_MESH_H_ */

@@ -1,5 +1,6 @@
 /* rx_cu_unsupported.h -- the entry points of rx_cu_funcs.cuh:138-253 that this library does not implement:
- * anisotropic kernels, wavelet turbulence, sub-particles, marching-cubes meshing and screen-space meshing.
+ * anisotropic kernels, wavelet turbulence, sub-particles, the one-call marching-cubes mesh builder and
+ * screen-space meshing.
  * They are exported so that host code written against the reference links unchanged; each one records error 5
  * (CuGetLastErrorCode) naming itself, prints it, leaves its outputs untouched and returns 0 / false.
  * The declarations keep the reference's signatures (C++: some take references); the types they name are only
@@ -50,23 +51,9 @@ void CuSubSphGridDensity(float* dGridDl, rxSubParticleCell subCell,
                          int nx, int ny, int nz, float x0, float y0, float z0, float dx, float dy, float dz);
 void CuSubCheckRatio(rxSubParticleCell subCell);
 uint CuCheckNumValidHashData(rxSubParticleCell subCell);
-/* marching cubes (:184-212) */
+/* marching cubes (:184-190): the GL-free mesh builder is declared there but defined nowhere in the reference
+ * (rx_mc.cu:392-478 defines only the GL form); the table and per-stage entry points are in rx_cu_mesh.h */
 void CuMCCreateMesh(float threshold, unsigned int& nvrts, unsigned int& ntris);
-bool CuMCInit(void);
-void CuInitMCTable(void);
-void CuCleanMCTable(void);
-void CuMCCalTriNum(float* dVolume, uint* dVoxBit, uint* dVoxVNum, uint* dVoxVNumScan,
-                   uint* dVoxTNum, uint* dVoxTNumScan, uint* dVoxOcc, uint* dVoxOccScan, uint* dCompactedVox,
-                   uint3 grid_size, uint num_voxels, float3 grid_width, float threshold,
-                   uint& num_active_voxels, uint& nvrts, uint& ntris);
-void CuMCCalEdgeVrts(float* dVolume, float* dEdgeVrts, float* dCompactedEdgeVrts,
-                     uint* dEdgeOcc, uint* dEdgeOccScan, uint3 edge_size[3], uint num_edge[4],
-                     uint3 grid_size, uint num_voxels, float3 grid_width, float3 grid_min, float threshold, uint& nvrts);
-void CuMCCalTri(uint* dTris, uint* dVoxBit, uint* dVoxTNumScan, uint* dCompactedVox,
-                uint* dEdgeOccScan, uint3 edge_size[3], uint num_edge[4],
-                uint3 grid_size, uint num_voxels, float3 grid_width, float threshold,
-                uint num_active_voxels, uint nvrts, uint ntris);
-void CuMCCalNrm(float* dNrms, uint* dTris, float* dCompactedEdgeVrts, uint nvrts, uint ntris);
 /* screen-space meshing (:215-250) */
 void CuInitFloatArray(float* dArray, float val, int n);
 void CuInitRadArray(float* drad, float* dval, int n);

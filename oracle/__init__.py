@@ -83,6 +83,11 @@ _SIGS = {
     "orc_detect_surface_f": (None, [P, P, P, I, F, F]),
     "orc_mlp_forward_d": (None, [P, I, I, P, P, I, P, P, I, P, P, P]),
     "orc_mlp_backward_d": (None, [P, P, I, I, P, P, I, P, P, I, P, P, P, P, P, P, P, P, P]),
+    "orc_mc_tables": (None, [P, P, P]),
+    "orc_mc_tri_num": (None, [P, P, C.c_uint, F, P, P, P, P, P, P, P, P, P]),
+    "orc_mc_edge_vrts": (C.c_uint, [P, P, P, P, P, P, F, P, P, P, P]),
+    "orc_mc_tri": (None, [P, P, P, P, P, P, P, P, C.c_uint, C.c_uint, C.c_uint]),
+    "orc_mc_nrm": (None, [P, P, P, C.c_uint, C.c_uint]),
 }
 
 
@@ -171,3 +176,48 @@ def pbf_step(pos4: np.ndarray, vel4: np.ndarray, P: OrcParams, double: bool = Fa
     if want_lambda:
         return pos, vel, dens, st, lam
     return pos, vel, dens, st
+
+
+def mc_layout(n):
+    """Edge sizes and counts of a lattice of n = (nx, ny, nz) samples (rx_mc_gpu.cpp:126-133)."""
+    nx, ny, nz = map(int, n)
+    esize = np.array([[nx, ny + 1, nz + 1], [nx + 1, ny, nz + 1], [nx + 1, ny + 1, nz]], np.uint32)
+    ne = [int(np.prod(e)) for e in esize]
+    return esize, np.array(ne + [sum(ne)], np.uint32)
+
+
+def mc_mesh(vol: np.ndarray, n, h, lo, thr: float) -> dict:
+    """The four marching-cubes stages of the CPU path on one volume (x fastest, shape nz, ny, nx or flat).
+
+    Returns every array the stages write (as the reference host keeps them) plus the counts."""
+    lib = load()
+    n3 = np.array(n, np.uint32)
+    nvox = int(np.prod(n3))
+    vol = np.ascontiguousarray(vol, np.float32).reshape(-1)
+    assert vol.size == nvox
+    u = lambda k: np.zeros(k, np.uint32)
+    r = {k: u(nvox) for k in ("voxbit", "vnum", "vscan", "tnum", "tscan", "occ", "occscan", "compacted")}
+    out3 = u(3)
+    lib.orc_mc_tri_num(ptr(vol), ptr(n3), nvox, thr, ptr(r["voxbit"]), ptr(r["vnum"]), ptr(r["vscan"]), ptr(r["tnum"]),
+                       ptr(r["tscan"]), ptr(r["occ"]), ptr(r["occscan"]), ptr(r["compacted"]), ptr(out3))
+    r["active"], r["nverts_vox"], r["ntris"] = map(int, out3)
+    esize, ne = mc_layout(n3)
+    h3 = np.array(h, np.float32)
+    lo3 = np.array(lo, np.float32)
+    r["edge_vrts"] = np.zeros((int(ne[3]), 4), np.float32)
+    r["vrts"] = np.zeros((int(ne[3]), 4), np.float32)
+    r["eocc"] = u(int(ne[3]))
+    r["escan"] = u(int(ne[3]))
+    nv = lib.orc_mc_edge_vrts(ptr(vol), ptr(esize), ptr(ne), ptr(n3), ptr(h3), ptr(lo3), thr, ptr(r["edge_vrts"]), ptr(r["vrts"]),
+                              ptr(r["eocc"]), ptr(r["escan"]))
+    r["nvrts"] = int(nv)
+    r["vrts"] = r["vrts"][:nv]
+    nt = r["ntris"] if nv else 0
+    r["tris"] = np.zeros((nt, 3), np.uint32)
+    r["nrms"] = np.zeros((nv, 3), np.float32)
+    if nv and nt:
+        lib.orc_mc_tri(ptr(r["tris"]), ptr(r["voxbit"]), ptr(r["tscan"]), ptr(r["compacted"]), ptr(r["escan"]), ptr(esize), ptr(ne),
+                       ptr(n3), r["active"], nv, nt)
+        lib.orc_mc_nrm(ptr(r["nrms"]), ptr(r["tris"]), ptr(r["vrts"]), nv, nt)
+    r["ntris"] = nt
+    return r
