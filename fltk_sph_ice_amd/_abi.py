@@ -163,7 +163,12 @@ SIGNATURES: dict[str, tuple] = {
     "CuGetLastErrorString": (C.c_char_p, []),
     "CuClearLastError": (None, []),
     "CuGetStream": (C.c_void_p, []),
-    # rx_cu_mesh.h (marching cubes; C++ references are passed as pointers)
+    # rx_cu_mesh.h (anisotropic kernels; marching cubes, whose C++ references are passed as pointers)
+    "CuSphCalUpdatedPosition": (None, [vp, vp, flt, flt, rxParticleCell]),
+    "CuSphCalCovarianceMatrix": (None, [vp, vp, flt, rxParticleCell]),
+    "CuSphSVDecomposition": (None, [vp, vp, vp, vp, u32]),
+    "CuSphCalTransformMatrix": (None, [vp, vp, vp, u32]),
+    "CuSphGridDensityAniso": (None, [vp, vp, flt, rxParticleCell, i32, i32, i32, flt, flt, flt, flt, flt, flt]),
     "CuMCInit": (C.c_bool, []),
     "CuInitMCTable": (None, []),
     "CuCleanMCTable": (None, []),
@@ -176,11 +181,6 @@ SIGNATURES: dict[str, tuple] = {
     "rxParticlesWrite": (i32, [C.c_char_p, vp, vp, i32]),
     "rxParticlesRead": (i32, [C.c_char_p, vp, vp, i32]),
     # entry points of rx_cu_funcs.cuh:138-253 exported as error-reporting stubs (include/rx_cu_unsupported.h)
-    "CuSphCalUpdatedPosition": (None, []),
-    "CuSphCalCovarianceMatrix": (None, []),
-    "CuSphSVDecomposition": (None, []),
-    "CuSphCalTransformMatrix": (None, []),
-    "CuSphGridDensityAniso": (None, []),
     "CuSphES": (None, []),
     "CuAddWTurb3D": (None, []),
     "CuSetWaveletTile3D": (None, []),

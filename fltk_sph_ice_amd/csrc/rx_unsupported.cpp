@@ -1,21 +1,16 @@
 // rx_unsupported.cpp -- the entry points of rx_cu_funcs.cuh:138-253 that this library does not implement
-// (anisotropic kernels, wavelet turbulence, sub-particles, marching cubes, screen-space meshing).  With C
+// (wavelet turbulence, sub-particles, the one-call marching-cubes mesh builder, screen-space meshing).  With C
 // linkage the symbol carries no parameter types, so each stub is defined without parameters: a caller that
 // passes the reference's arguments links and runs, the stub records error 5 and returns 0 / false.
 #include "rx_common.h"
 
 namespace {
 void unsupported(const char* fn) {
-    rx_set_error(5, "%s: not part of this library (meshing, turbulence, sub-particles and screen-space meshes are out of scope)", fn);
+    rx_set_error(5, "%s: not part of this library (turbulence, sub-particles and screen-space meshes are out of scope)", fn);
 }
 }  // namespace
 
 extern "C" {
-void CuSphCalUpdatedPosition() { unsupported("CuSphCalUpdatedPosition"); }
-void CuSphCalCovarianceMatrix() { unsupported("CuSphCalCovarianceMatrix"); }
-void CuSphSVDecomposition() { unsupported("CuSphSVDecomposition"); }
-void CuSphCalTransformMatrix() { unsupported("CuSphCalTransformMatrix"); }
-void CuSphGridDensityAniso() { unsupported("CuSphGridDensityAniso"); }
 void CuSphES() { unsupported("CuSphES"); }
 void CuAddWTurb3D() { unsupported("CuAddWTurb3D"); }
 void CuSetWaveletTile3D() { unsupported("CuSetWaveletTile3D"); }

@@ -1,6 +1,7 @@
-/* rx_cu_mesh.h -- surface extraction downstream of the particle path: the marching-cubes entry points of
- * rx_cu_funcs.cuh:184-212 in their RX_CUMC_USE_GEOMETRY form (rx_cu_common.cuh:35), implemented by HIP kernels
- * for gfx950 (fltk_sph_ice_amd/csrc/mc.hip).
+/* rx_cu_mesh.h -- surface extraction downstream of the particle path: the anisotropic-kernel entry points of
+ * rx_cu_funcs.cuh:138-144 (fltk_sph_ice_amd/csrc/aniso.hip) and the marching-cubes entry points of
+ * rx_cu_funcs.cuh:184-212 in their RX_CUMC_USE_GEOMETRY form (rx_cu_common.cuh:35, fltk_sph_ice_amd/csrc/mc.hip),
+ * implemented by HIP kernels for gfx950.
  *
  * Same names, argument order and meaning as the reference; C++ linkage rules as there (some outputs are
  * references, which the C ABI passes as pointers).  All arrays are device arrays the caller allocates with the
@@ -17,6 +18,28 @@
 
 #ifdef __cplusplus
 extern "C" {
+
+/* Anisotropic kernels (Yu & Turk 2013).  Particle arrays are float4, matrices 9 floats row-major (matrix3x3),
+ * eigenvalues 3 floats per particle; outputs are in original particle order.  The grid is the one of
+ * CuSetParameters; the neighbour loops visit the cells of the clamped box p +- h (p +- 3h for the lattice). */
+
+/* rx_cu_funcs.cuh:140 (rx_sph.cu:1126-1150): dPosW = (x_w, 0) with x_w = sum w x_j / sum w over |x - x_j| < h,
+ * w = (1 - r/h)^3 (the particle itself included); dUpPos = ((1 - lambda) x + lambda x_w, 0). */
+void CuSphCalUpdatedPosition(float* dUpPos, float* dPosW, float lambda, float h, rxParticleCell cell);
+/* rx_cu_funcs.cuh:141 (rx_sph.cu:1159-1183): dPosW = (x_w, number of neighbours within h),
+ * dCMat = sum w (x_j - x_w)(x_j - x_w)^T / sum w. */
+void CuSphCalCovarianceMatrix(float* dPosW, float* dCMat, float h, rxParticleCell cell);
+/* rx_cu_funcs.cuh:142 (rx_sph.cu:1193-1206): C = R diag(s) R^T with s descending and R's columns signed as the
+ * reference's SVD signs them; dEigen = 1400 (s_1, max(s_2, s_1/4), max(s_3, s_1/4)) when the particle has more
+ * than 10 neighbours (dPosW.w), else (0.5, 0.5, 0.5); dR = R. */
+void CuSphSVDecomposition(float* dC, float* dPosW, float* dEigen, float* dR, uint numParticles);
+/* rx_cu_funcs.cuh:143 (rx_sph.cu:1215-1229): dG = R diag(1/eigen) R^T divided by its largest diagonal entry. */
+void CuSphCalTransformMatrix(float* dEigen, float* dR, float* dG, uint numParticles);
+/* rx_cu_funcs.cuh:145-146 (rx_sph.cu:1239-1276): lattice node (x0 + i dx, y0 + j dy, z0 + k dz) at
+ * dGridD[i + j nx + k nx ny] = sum_j det(G_j) Mass Wpoly6 (h^2 - |G_j (p - x_j)|^2)^3 over |G_j (p - x_j)| <= h,
+ * h = EffectiveRadius, over the particles of the cells within 3h.  Emax is not used (as in the reference). */
+void CuSphGridDensityAniso(float* dGridD, float* dG, float Emax, rxParticleCell cell,
+                           int nx, int ny, int nz, float x0, float y0, float z0, float dx, float dy, float dz);
 
 /* rx_cu_funcs.cuh:192 (rx_mc.cu:49-65): selects nothing and returns true -- the device is the caller's current one. */
 bool CuMCInit(void);

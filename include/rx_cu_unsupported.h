@@ -1,6 +1,6 @@
 /* rx_cu_unsupported.h -- the entry points of rx_cu_funcs.cuh:138-253 that this library does not implement:
- * anisotropic kernels, wavelet turbulence, sub-particles, the one-call marching-cubes mesh builder and
- * screen-space meshing.
+ * wavelet turbulence, sub-particles, the one-call marching-cubes mesh builder and screen-space meshing
+ * (the anisotropic kernels and the marching-cubes stages are in rx_cu_mesh.h).
  * They are exported so that host code written against the reference links unchanged; each one records error 5
  * (CuGetLastErrorCode) naming itself, prints it, leaves its outputs untouched and returns 0 / false.
  * The declarations keep the reference's signatures (C++: some take references); the types they name are only
@@ -18,13 +18,6 @@ struct rxSSGridG;
 struct rxSsmParams;
 
 extern "C" {
-/* anisotropic kernels (rx_cu_funcs.cuh:138-144) */
-void CuSphCalUpdatedPosition(float* dUpPos, float* dPosW, float lambda, float h, rxParticleCell cell);
-void CuSphCalCovarianceMatrix(float* dPosW, float* dCMat, float h, rxParticleCell cell);
-void CuSphSVDecomposition(float* dC, float* dPosW, float* dEigen, float* dR, uint numParticles);
-void CuSphCalTransformMatrix(float* dEigen, float* dR, float* dG, uint numParticles);
-void CuSphGridDensityAniso(float* dGridD, float* dG, float Emax, rxParticleCell cell,
-                           int nx, int ny, int nz, float x0, float y0, float z0, float dx, float dy, float dz);
 /* wavelet turbulence (:147-155) */
 void CuSphES(float* dEt, float scale, float coef_et, float max_et, rxParticleCell cell, float3 cell_size);
 void CuAddWTurb3D(float* dVturb, float* dFrc, float* dEt, float* dDens, int d,

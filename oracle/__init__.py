@@ -84,6 +84,12 @@ _SIGS = {
     "orc_mlp_forward_d": (None, [P, I, I, P, P, I, P, P, I, P, P, P]),
     "orc_mlp_backward_d": (None, [P, P, I, I, P, P, I, P, P, I, P, P, P, P, P, P, P, P, P]),
     "orc_mc_tables": (None, [P, P, P]),
+    "orc_aniso_smooth": (None, [P, P, P, P, C.c_uint, P, P, F, F, P, P]),
+    "orc_aniso_cov": (None, [P, P, P, P, C.c_uint, P, P, F, P, P]),
+    "orc_sym_eigen3": (None, [P, P]),
+    "orc_aniso_svd": (None, [P, P, C.c_uint, P, P]),
+    "orc_aniso_transform": (None, [P, P, C.c_uint, P]),
+    "orc_aniso_grid_density": (None, [P, P, P, P, C.c_uint, P, P, F, F, F, P, I, I, I, F, F, F, F, F, F, P]),
     "orc_mc_tri_num": (None, [P, P, C.c_uint, F, P, P, P, P, P, P, P, P, P]),
     "orc_mc_edge_vrts": (C.c_uint, [P, P, P, P, P, P, F, P, P, P, P]),
     "orc_mc_tri": (None, [P, P, P, P, P, P, P, P, C.c_uint, C.c_uint, C.c_uint]),
@@ -220,4 +226,32 @@ def mc_mesh(vol: np.ndarray, n, h, lo, thr: float) -> dict:
                        ptr(n3), r["active"], nv, nt)
         lib.orc_mc_nrm(ptr(r["nrms"]), ptr(r["tris"]), ptr(r["vrts"]), nv, nt)
     r["ntris"] = nt
+    return r
+
+
+def aniso_pipeline(spos4, sidx, cs, ce, geo, gsize, lam: float, h: float, h_eff: float, mass: float, wpoly6: float, lattice=None):
+    """The five anisotropic-kernel stages of the CPU path on sorted particle data (positions in slot order,
+    slot -> particle index, cell start / end with 0xFFFFFFFF for empty cells).  geo = origin xyz + cell width xyz.
+    Returns the stage outputs in particle order, and the lattice density when lattice = (n3, lo3, d3) is given."""
+    lib = load()
+    n = len(sidx)
+    spos4 = np.ascontiguousarray(spos4, np.float32)
+    sidx = np.ascontiguousarray(sidx, np.uint32)
+    cs = np.ascontiguousarray(cs, np.uint32)
+    ce = np.ascontiguousarray(ce, np.uint32)
+    geo = np.ascontiguousarray(geo, np.float32)
+    gsize = np.ascontiguousarray(gsize, np.int32)
+    r = {"up": np.zeros((n, 4), np.float32), "wpos": np.zeros((n, 4), np.float32), "posw": np.zeros((n, 4), np.float32),
+         "cmat": np.zeros((n, 9), np.float32), "eigen": np.zeros((n, 3), np.float32), "rmat": np.zeros((n, 9), np.float32),
+         "gmat": np.zeros((n, 9), np.float32)}
+    lib.orc_aniso_smooth(ptr(spos4), ptr(sidx), ptr(cs), ptr(ce), n, ptr(geo), ptr(gsize), lam, h, ptr(r["up"]), ptr(r["wpos"]))
+    lib.orc_aniso_cov(ptr(spos4), ptr(sidx), ptr(cs), ptr(ce), n, ptr(geo), ptr(gsize), h, ptr(r["posw"]), ptr(r["cmat"]))
+    lib.orc_aniso_svd(ptr(r["cmat"]), ptr(r["posw"]), n, ptr(r["eigen"]), ptr(r["rmat"]))
+    lib.orc_aniso_transform(ptr(r["eigen"]), ptr(r["rmat"]), n, ptr(r["gmat"]))
+    if lattice is not None:
+        (nx, ny, nz), lo, d = lattice
+        out = np.zeros(nx * ny * nz, np.float32)
+        lib.orc_aniso_grid_density(ptr(spos4), ptr(sidx), ptr(cs), ptr(ce), n, ptr(geo), ptr(gsize), h_eff, mass, wpoly6,
+                                   ptr(r["gmat"]), nx, ny, nz, *map(float, lo), *map(float, d), ptr(out))
+        r["grid"] = out
     return r

@@ -1,0 +1,291 @@
+// aniso.cpp -- CPU restatement of the reference's anisotropic-kernel stages (rx_sph.cu:1126-1276,
+// rx_aniso_kernel.cu:20-718).  TEST INFRASTRUCTURE ONLY: the checker of fltk_sph_ice_amd/csrc/aniso.hip, loaded
+// by tests/ through liboracle.so.
+//
+// Pinned by (tests/test_oracle_aniso.py): the weighted means and covariances against a brute-force numpy sum
+// over all particle pairs; the decomposition against numpy.linalg.eigh / svd on random symmetric positive
+// semi-definite matrices (values, vectors up to the sign rule, reconstruction); closed forms for diagonal and
+// rotated-diagonal covariances; the transform against the explicit R diag(1/sigma) R^T; the lattice density
+// against a brute-force numpy sum, and for G = I against the isotropic lattice density.
+//
+// Loops run in the reference's order (cells z, y, x over the clamped box; per-cell partial sums added to the
+// total) and every operation rounds on its own, as the library computes them.  The decomposition is a cyclic
+// Jacobi iteration in double on the symmetric matrix, ordered and signed as the reference's SVD (see aniso.hip).
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <utility>
+
+namespace {
+
+struct Geo {
+    float ox, oy, oz, cx, cy, cz;
+    int gx, gy, gz;
+};
+
+void cell_of(float x, float y, float z, const Geo& g, int* c) {
+    auto one = [](float p, float o, float w, int n) {
+        int v = (int)std::floor((p - o) / w);
+        return v < 0 ? 0 : (v > n - 1 ? n - 1 : v);
+    };
+    c[0] = one(x, g.ox, g.cx, g.gx);
+    c[1] = one(y, g.oy, g.cy, g.gy);
+    c[2] = one(z, g.oz, g.cz, g.gz);
+}
+
+float len3(float x, float y, float z) { return std::sqrt(x * x + y * y + z * z); }
+
+// rx_aniso_kernel.cu:33-63 over the cells of p +- h: (sum w x, sum w)
+void weighted_sum(const float* sp, const uint32_t* cs, const uint32_t* ce, const Geo& g, const float* p, float h, float* tot) {
+    int lo[3], hi[3];
+    cell_of(p[0] - h, p[1] - h, p[2] - h, g, lo);
+    cell_of(p[0] + h, p[1] + h, p[2] + h, g, hi);
+    tot[0] = tot[1] = tot[2] = tot[3] = 0.0f;
+    for (int z = lo[2]; z <= hi[2]; ++z)
+        for (int y = lo[1]; y <= hi[1]; ++y)
+            for (int x = lo[0]; x <= hi[0]; ++x) {
+                const uint32_t c = (uint32_t)((z * g.gy + y) * g.gx + x);
+                if (cs[c] == 0xFFFFFFFFu) continue;
+                float part[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                for (uint32_t j = cs[c]; j < ce[c]; ++j) {
+                    const float* q = sp + 4 * (size_t)j;
+                    const float r = len3(p[0] - q[0], p[1] - q[1], p[2] - q[2]);
+                    if (r < h) {
+                        const float a = 1.0f - r / h;
+                        const float w = a * a * a;
+                        part[0] = part[0] + q[0] * w;
+                        part[1] = part[1] + q[1] * w;
+                        part[2] = part[2] + q[2] * w;
+                        part[3] = part[3] + w;
+                    }
+                }
+                for (int k = 0; k < 4; ++k) tot[k] = tot[k] + part[k];
+            }
+}
+
+void jacobi_rotate(double a[3][3], double v[3][3], int p, int q) {
+    const double apq = a[p][q];
+    if (apq == 0.0) return;
+    const double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
+    const double t = (theta >= 0.0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+    const double cs = 1.0 / std::sqrt(t * t + 1.0);
+    const double sn = t * cs;
+    const double tau = sn / (1.0 + cs);
+    a[p][p] = a[p][p] - t * apq;
+    a[q][q] = a[q][q] + t * apq;
+    a[p][q] = 0.0;
+    a[q][p] = 0.0;
+    const int r = 3 - p - q;
+    const double arp = a[r][p], arq = a[r][q];
+    a[r][p] = arp - sn * (arq + tau * arp);
+    a[p][r] = a[r][p];
+    a[r][q] = arq + sn * (arp - tau * arq);
+    a[q][r] = a[r][q];
+    for (int m = 0; m < 3; ++m) {
+        const double vp = v[m][p], vq = v[m][q];
+        v[m][p] = vp - sn * (vq + tau * vp);
+        v[m][q] = vq + sn * (vp - tau * vq);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// geo = (origin xyz, cell width xyz), gsize = grid cells per axis
+void orc_aniso_smooth(const float* spos4, const uint32_t* sidx, const uint32_t* cs, const uint32_t* ce, uint32_t n, const float* geo,
+                      const int* gsize, float lambda, float h, float* up4, float* wpos4) {
+    const Geo g{geo[0], geo[1], geo[2], geo[3], geo[4], geo[5], gsize[0], gsize[1], gsize[2]};
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* p = spos4 + 4 * (size_t)i;
+        float s[4];
+        weighted_sum(spos4, cs, ce, g, p, h, s);
+        const float m[3] = {s[0] / s[3], s[1] / s[3], s[2] / s[3]};
+        const uint32_t o = sidx[i];
+        float* w = wpos4 + 4 * (size_t)o;
+        w[0] = m[0];
+        w[1] = m[1];
+        w[2] = m[2];
+        w[3] = 0.0f;
+        const float a = (float)(1.0 - (double)lambda);
+        float* u = up4 + 4 * (size_t)o;
+        for (int k = 0; k < 3; ++k) u[k] = a * p[k] + lambda * m[k];
+        u[3] = 0.0f;
+    }
+}
+
+// rx_aniso_kernel.cu:118-229
+void orc_aniso_cov(const float* spos4, const uint32_t* sidx, const uint32_t* cs, const uint32_t* ce, uint32_t n, const float* geo,
+                   const int* gsize, float h, float* posw4, float* cmat9) {
+    const Geo g{geo[0], geo[1], geo[2], geo[3], geo[4], geo[5], gsize[0], gsize[1], gsize[2]};
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* p = spos4 + 4 * (size_t)i;
+        float s[4];
+        weighted_sum(spos4, cs, ce, g, p, h, s);
+        const float m[3] = {s[0] / s[3], s[1] / s[3], s[2] / s[3]};
+        int lo[3], hi[3];
+        cell_of(p[0] - h, p[1] - h, p[2] - h, g, lo);
+        cell_of(p[0] + h, p[1] + h, p[2] + h, g, hi);
+        float cv[9] = {0};
+        float wsum = 0.0f, cnt = 0.0f;
+        for (int z = lo[2]; z <= hi[2]; ++z)
+            for (int y = lo[1]; y <= hi[1]; ++y)
+                for (int x = lo[0]; x <= hi[0]; ++x) {
+                    const uint32_t c = (uint32_t)((z * g.gy + y) * g.gx + x);
+                    if (cs[c] == 0xFFFFFFFFu) continue;
+                    float pw = 0.0f, pc = 0.0f;
+                    for (uint32_t j = cs[c]; j < ce[c]; ++j) {
+                        const float* q = spos4 + 4 * (size_t)j;
+                        const float r = len3(p[0] - q[0], p[1] - q[1], p[2] - q[2]);
+                        if (r < h) {
+                            const float a = 1.0f - r / h;
+                            const float w = a * a * a;
+                            const float d[3] = {q[0] - m[0], q[1] - m[1], q[2] - m[2]};
+                            for (int r3 = 0; r3 < 3; ++r3)
+                                for (int c3 = 0; c3 < 3; ++c3) cv[3 * r3 + c3] = cv[3 * r3 + c3] + w * d[r3] * d[c3];
+                            pw = pw + w;
+                            pc = (float)((double)pc + 1.0);
+                        }
+                    }
+                    wsum = wsum + pw;
+                    cnt = cnt + pc;
+                }
+        const uint32_t o = sidx[i];
+        for (int t = 0; t < 9; ++t) cmat9[9 * (size_t)o + t] = cv[t] / wsum;
+        float* pw4 = posw4 + 4 * (size_t)o;
+        pw4[0] = m[0];
+        pw4[1] = m[1];
+        pw4[2] = m[2];
+        pw4[3] = cnt;
+    }
+}
+
+// Symmetric eigen-decomposition (cyclic Jacobi, double) of one 3x3 matrix: values on the diagonal of a, vectors
+// in the columns of v (exported for the checks against a library routine).
+void orc_sym_eigen3(double* a9, double* v9) {
+    double a[3][3], v[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            a[r][c] = a9[3 * r + c];
+            v[r][c] = r == c ? 1.0 : 0.0;
+        }
+    for (int sweep = 0; sweep < 16; ++sweep) {
+        const double off = std::fabs(a[0][1]) + std::fabs(a[0][2]) + std::fabs(a[1][2]);
+        const double dg = std::fabs(a[0][0]) + std::fabs(a[1][1]) + std::fabs(a[2][2]);
+        if (off == 0.0 || off <= 1e-15 * dg) break;
+        jacobi_rotate(a, v, 0, 1);
+        jacobi_rotate(a, v, 0, 2);
+        jacobi_rotate(a, v, 1, 2);
+    }
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            a9[3 * r + c] = a[r][c];
+            v9[3 * r + c] = v[r][c];
+        }
+}
+
+// rx_aniso_kernel.cu:490-534
+void orc_aniso_svd(const float* cmat9, const float* posw4, uint32_t n, float* eigen3, float* r9) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const int nn = (int)posw4[4 * (size_t)i + 3];
+        double a9[9], u9[9];
+        for (int t = 0; t < 9; ++t) a9[t] = (double)cmat9[9 * (size_t)i + t];
+        a9[1] = a9[3] = 0.5 * (a9[1] + a9[3]);
+        a9[2] = a9[6] = 0.5 * (a9[2] + a9[6]);
+        a9[5] = a9[7] = 0.5 * (a9[5] + a9[7]);
+        orc_sym_eigen3(a9, u9);
+        float w[3], vs[3], uf[3][3];
+        for (int k = 0; k < 3; ++k) {
+            w[k] = (float)std::fabs(a9[4 * k]);
+            vs[k] = a9[4 * k] < 0.0 ? -1.0f : 1.0f;
+            for (int r = 0; r < 3; ++r) uf[r][k] = (float)u9[3 * r + k];
+        }
+        for (int k = 1; k < 3; ++k)
+            for (int j = k; j > 0 && w[j - 1] < w[j]; --j) {
+                std::swap(w[j - 1], w[j]);
+                std::swap(vs[j - 1], vs[j]);
+                for (int r = 0; r < 3; ++r) std::swap(uf[r][j - 1], uf[r][j]);
+            }
+        for (int k = 0; k < 3; ++k) {
+            int neg = 0;
+            for (int r = 0; r < 3; ++r) neg += (uf[r][k] < 0.0f) + (vs[k] * uf[r][k] < 0.0f);
+            if (neg > 3)
+                for (int r = 0; r < 3; ++r) uf[r][k] = -uf[r][k];
+        }
+        float s0 = w[0], s1 = w[1], s2 = w[2];
+        if (nn > 10) {
+            const float lo = s0 / 4.0f;
+            s1 = s1 >= lo ? s1 : lo;
+            s2 = s2 >= lo ? s2 : lo;
+            s0 = s0 * 1400.0f;
+            s1 = s1 * 1400.0f;
+            s2 = s2 * 1400.0f;
+        } else {
+            s0 = s1 = s2 = 0.5f;
+        }
+        eigen3[3 * (size_t)i] = s0;
+        eigen3[3 * (size_t)i + 1] = s1;
+        eigen3[3 * (size_t)i + 2] = s2;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) r9[9 * (size_t)i + 3 * r + c] = uf[r][c];
+    }
+}
+
+// rx_aniso_kernel.cu:583-616
+void orc_aniso_transform(const float* eigen3, const float* r9, uint32_t n, float* g9) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const float sx = eigen3[3 * (size_t)i], sy = eigen3[3 * (size_t)i + 1], sz = eigen3[3 * (size_t)i + 2];
+        const float* R = r9 + 9 * (size_t)i;
+        float G[9];
+        for (int j = 0; j < 3; ++j)
+            for (int c = 0; c < 3; ++c) G[3 * j + c] = R[3 * j] * R[3 * c] / sx + R[3 * j + 1] * R[3 * c + 1] / sy + R[3 * j + 2] * R[3 * c + 2] / sz;
+        float md = -100000000.0f;
+        if (G[0] > md) md = G[0];
+        if (G[4] > md) md = G[4];
+        if (G[8] > md) md = G[8];
+        for (int t = 0; t < 9; ++t) g9[9 * (size_t)i + t] = G[t] / md;
+    }
+}
+
+// rx_aniso_kernel.cu:537-547 (calDeterminant), 625-715
+void orc_aniso_grid_density(const float* spos4, const uint32_t* sidx, const uint32_t* cs, const uint32_t* ce, uint32_t n, const float* geo,
+                            const int* gsize, float h, float mass, float wpoly6, const float* g9, int nx, int ny, int nz, float x0,
+                            float y0, float z0, float dx, float dy, float dz, float* out) {
+    const Geo g{geo[0], geo[1], geo[2], geo[3], geo[4], geo[5], gsize[0], gsize[1], gsize[2]};
+    const float h3 = h * 3.0f;
+    for (int gz = 0; gz < nz; ++gz)
+        for (int gy = 0; gy < ny; ++gy)
+            for (int gx = 0; gx < nx; ++gx) {
+                const float p[3] = {x0 + (float)(uint32_t)gx * dx, y0 + (float)(uint32_t)gy * dy, z0 + (float)(uint32_t)gz * dz};
+                int lo[3], hi[3];
+                cell_of(p[0] - h3, p[1] - h3, p[2] - h3, g, lo);
+                cell_of(p[0] + h3, p[1] + h3, p[2] + h3, g, hi);
+                float P = 0.0f;
+                for (int z = lo[2]; z <= hi[2]; ++z)
+                    for (int y = lo[1]; y <= hi[1]; ++y)
+                        for (int x = lo[0]; x <= hi[0]; ++x) {
+                            const uint32_t c = (uint32_t)((z * g.gy + y) * g.gx + x);
+                            if (cs[c] == 0xFFFFFFFFu) continue;
+                            float part = 0.0f;
+                            for (uint32_t j = cs[c]; j < ce[c]; ++j) {
+                                const float* q = spos4 + 4 * (size_t)j;
+                                const float* G = g9 + 9 * (size_t)sidx[j];
+                                const float r0 = p[0] - q[0], r1 = p[1] - q[1], r2 = p[2] - q[2];
+                                const float rg0 = G[0] * r0 + G[1] * r1 + G[2] * r2;
+                                const float rg1 = G[3] * r0 + G[4] * r1 + G[5] * r2;
+                                const float rg2 = G[6] * r0 + G[7] * r1 + G[8] * r2;
+                                const float r = len3(rg0, rg1, rg2);
+                                if (r <= h) {
+                                    const float qq = h * h - r * r;
+                                    const float det = G[0] * G[4] * G[8] - G[0] * G[7] * G[5] + G[3] * G[7] * G[2] - G[3] * G[1] * G[8] +
+                                                      G[6] * G[1] * G[5] - G[6] * G[4] * G[2];
+                                    part = part + det * mass * wpoly6 * qq * qq * qq;
+                                }
+                            }
+                            P = P + part;
+                        }
+                out[gx + gy * nx + (size_t)gz * nx * ny] = P;
+            }
+}
+
+}  // extern "C"
