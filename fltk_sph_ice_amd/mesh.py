@@ -109,3 +109,46 @@ class MCMesh:
             "edge_vrts": self.edge_vrts.download(np.float32).reshape(-1, 4),
         }
         return r
+
+
+class AnisoSurface:
+    """Surface of a particle set through the ABI in the reference's order: rxSPH_GPU::CalAnisotropicKernel
+    (rx_sph_gpu.cpp:2036-2095: grid on the positions, smoothed centres, grid on the centres, covariance,
+    decomposition, transform) then the lattice and the mesh of rxFlGLWindow::calMeshSPH_GPU
+    (rx_fltk_glcanvas.cpp:2086-2130: grid on the positions, anisotropic lattice density, marching cubes).
+
+    `sp` is the rxSimParams set with CuSetParameters; `n_lat`, `lo`, `d` the lattice; `h` the anisotropic
+    neighbourhood radius (the reference uses twice the effective radius), `lam` the smoothing weight (0.9)."""
+
+    def __init__(self, sp, dpos: DevBuf, dvel: DevBuf, n: int, n_lat, lo, d: float, lib=None, h=None, lam=0.9):
+        from .abi_driver import GridBuffers
+
+        self.lib = lib = lib or load()
+        self.sp, self.n = sp, n
+        self.dpos, self.dvel = dpos, dvel
+        self.h = float(h if h is not None else 2.0 * sp.EffectiveRadius)
+        self.lam = float(lam)
+        self.grid = GridBuffers(n, sp.NumCells, lib)
+        self.up, self.wpos, self.posw = DevBuf(16 * n, lib), DevBuf(16 * n, lib), DevBuf(16 * n, lib)
+        self.cmat, self.eig, self.rmat, self.gmat = DevBuf(36 * n, lib), DevBuf(12 * n, lib), DevBuf(36 * n, lib), DevBuf(36 * n, lib)
+        self.n_lat = tuple(int(v) for v in n_lat)
+        self.lo = tuple(float(v) for v in lo)
+        self.d = float(d)
+        self.vol = DevBuf(4 * int(np.prod(self.n_lat)), lib)
+        self.mesh = MCMesh(self.n_lat, self.lo, (self.d,) * 3, lib)
+
+    def run(self, threshold: float, timer=None) -> tuple[int, int]:
+        """One extraction; returns (vertices, triangles).  `timer(name, fn, *args)` wraps each stage call."""
+        lib, g = self.lib, self.grid
+        t = timer or (lambda _name, fn, *a: fn(*a))
+        c = t("grid", g.set_object_to_cell, self.dpos.ptr, self.dvel.ptr)
+        t("smooth", lib.CuSphCalUpdatedPosition, self.up.ptr, self.wpos.ptr, self.lam, self.h, c)
+        c = t("grid", g.set_object_to_cell, self.up.ptr, self.dvel.ptr)
+        t("covariance", lib.CuSphCalCovarianceMatrix, self.posw.ptr, self.cmat.ptr, self.h, c)
+        t("decomposition", lib.CuSphSVDecomposition, self.cmat.ptr, self.posw.ptr, self.eig.ptr, self.rmat.ptr, self.n)
+        t("transform", lib.CuSphCalTransformMatrix, self.eig.ptr, self.rmat.ptr, self.gmat.ptr, self.n)
+        c = t("grid", g.set_object_to_cell, self.dpos.ptr, self.dvel.ptr)
+        nx, ny, nz = self.n_lat
+        t("lattice", lib.CuSphGridDensityAniso, self.vol.ptr, self.gmat.ptr, 0.0, c, nx, ny, nz, *self.lo, self.d, self.d, self.d)
+        check(lib)
+        return t("mesh", self.mesh.create_mesh, self.vol, threshold)
