@@ -57,36 +57,70 @@ __device__ inline int3 cell_of(float3 p, const AnisoK& k) {
     g.z = min(max((int)floorf((p.z - k.origin.z) / k.cellw.z), 0), k.gz - 1);
     return g;
 }
-__device__ inline float len3(float3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+
+// Squared distance from p to cell (x, y, z), or +inf-like "never skip" (-1) for cells on the grid's faces (the
+// clamped hash puts particles from outside the grid there).  An interior cell holds only particles with
+// floor((q - origin) / cell) equal to its index, computed in FP32, so they lie in its nominal box widened by
+// the rounding of that formula; the widening used is 1e-3 of a cell plus 1e-6 of the grid's extent.
+__device__ inline float cell_dist2(float3 p, int x, int y, int z, const AnisoK& k) {
+    if (x <= 0 || y <= 0 || z <= 0 || x >= k.gx - 1 || y >= k.gy - 1 || z >= k.gz - 1) return -1.0f;
+    auto axis = [](float q, int c, float o, float w, int n) {
+        const float slack = 1e-3f * w + 1e-6f * (fabsf(o) + (float)n * w);
+        const float lo = o + (float)c * w - slack, hi = o + (float)(c + 1) * w + slack;
+        const float d = fmaxf(fmaxf(lo - q, q - hi), 0.0f);
+        return d * d;
+    };
+    return axis(p.x, x, k.origin.x, k.cellw.x, k.gx) + axis(p.y, y, k.origin.y, k.cellw.y, k.gy) + axis(p.z, z, k.origin.z, k.cellw.z, k.gz);
+}
 
 // Calls body(j) for the slots of each cell of the box [lo, hi], cells in z, y, x order; cell_done() after
-// each non-empty cell.
-template <class Body, class Done>
-__device__ inline void for_cells(int3 lo, int3 hi, const AnisoK& k, const uint* __restrict__ cs, const uint* __restrict__ ce,
-                                 Body&& body, Done&& cell_done) {
+// each non-empty cell.  far(cell, d2) may skip a cell whose every particle the body would reject (d2 = the
+// cell's squared distance from p, negative when unknown): a skipped cell adds nothing, as in the reference
+// where its terms are all rejected and its zero sum is added.
+// Within a cell, fetch(j) loads what body(j, data) needs; four fetches are issued before their bodies run (in
+// slot order), so a wavefront keeps several gathers in flight.
+template <class Fetch, class Body, class Done, class Far>
+__device__ inline void for_cells(float3 p, int3 lo, int3 hi, const AnisoK& k, const uint* __restrict__ cs, const uint* __restrict__ ce,
+                                 Fetch&& fetch, Body&& body, Done&& cell_done, Far&& far) {
     for (int z = lo.z; z <= hi.z; ++z)
         for (int y = lo.y; y <= hi.y; ++y)
             for (int x = lo.x; x <= hi.x; ++x) {
                 const uint h = (uint)((z * k.gy + y) * k.gx + x);
+                if (far(h, cell_dist2(p, x, y, z, k))) continue;
                 const uint s = cs[h];
                 if (s == kEmpty) continue;
                 const uint e = ce[h];
-                for (uint j = s; j < e; ++j) body(j);
+                uint j = s;
+                for (; j + 4 <= e; j += 4) {
+                    const auto a = fetch(j), b = fetch(j + 1), c = fetch(j + 2), d = fetch(j + 3);
+                    body(j, a);
+                    body(j + 1, b);
+                    body(j + 2, c);
+                    body(j + 3, d);
+                }
+                for (; j < e; ++j) body(j, fetch(j));
                 cell_done();
             }
 }
+
+// r2 above this has sqrtf(r2) >= h: the square root can be skipped for candidates beyond it
+__device__ inline float sqrt_skip2(float h) { return h * h * 1.000001f; }
 
 // rx_aniso_kernel.cu:33-63 (calWeighedAvgPositionCell), summed over the cells of p +- h
 __device__ inline float4 weighted_mean_sum(float3 p, float h, const float4* __restrict__ sp, const uint* __restrict__ cs,
                                            const uint* __restrict__ ce, const AnisoK& k) {
     const int3 lo = cell_of(make_float3(p.x - h, p.y - h, p.z - h), k);
     const int3 hi = cell_of(make_float3(p.x + h, p.y + h, p.z + h), k);
+    const float reach2 = h * h * 1.0001f;  // a cell farther than h (with margin) has no particle with r < h
+    const float far2 = sqrt_skip2(h);
     float4 tot = make_float4(0.0f, 0.0f, 0.0f, 0.0f), part = tot;
     for_cells(
-        lo, hi, k, cs, ce,
-        [&](uint j) {
-            const float4 q = sp[j];
-            const float r = len3(make_float3(p.x - q.x, p.y - q.y, p.z - q.z));
+        p, lo, hi, k, cs, ce, [&](uint j) { return sp[j]; },
+        [&](uint, const float4& q) {
+            const float3 d = make_float3(p.x - q.x, p.y - q.y, p.z - q.z);
+            const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+            if (r2 > far2) return;
+            const float r = sqrtf(r2);
             if (r < h) {
                 const float a = 1.0f - r / h;
                 const float w = a * a * a;
@@ -102,7 +136,8 @@ __device__ inline float4 weighted_mean_sum(float3 p, float h, const float4* __re
             tot.z = tot.z + part.z;
             tot.w = tot.w + part.w;
             part = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        });
+        },
+        [&](uint, float d2) { return d2 > reach2; });
     return tot;
 }
 
@@ -132,13 +167,17 @@ __global__ __launch_bounds__(kTpb) void k_aniso_cov(float4* __restrict__ posw, f
     const float3 m = make_float3(s.x / s.w, s.y / s.w, s.z / s.w);
     const int3 lo = cell_of(make_float3(p.x - h, p.y - h, p.z - h), k);
     const int3 hi = cell_of(make_float3(p.x + h, p.y + h, p.z + h), k);
+    const float reach2 = h * h * 1.0001f;
+    const float far2 = sqrt_skip2(h);
     float cv[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     float wsum = 0.0f, cnt = 0.0f, pw = 0.0f, pc = 0.0f;
     for_cells(
-        lo, hi, k, c.dCellStart, c.dCellEnd,
-        [&](uint j) {
-            const float4 q = c.dSortedPos[j];
-            const float r = len3(make_float3(p.x - q.x, p.y - q.y, p.z - q.z));
+        p, lo, hi, k, c.dCellStart, c.dCellEnd, [&](uint j) { return c.dSortedPos[j]; },
+        [&](uint, const float4& q) {
+            const float3 dd = make_float3(p.x - q.x, p.y - q.y, p.z - q.z);
+            const float r2 = dd.x * dd.x + dd.y * dd.y + dd.z * dd.z;
+            if (r2 > far2) return;
+            const float r = sqrtf(r2);
             if (r < h) {
                 const float a = 1.0f - r / h;
                 const float w = a * a * a;
@@ -156,7 +195,8 @@ __global__ __launch_bounds__(kTpb) void k_aniso_cov(float4* __restrict__ posw, f
             cnt = cnt + pc;
             pw = 0.0f;
             pc = 0.0f;
-        });
+        },
+        [&](uint, float d2) { return d2 > reach2; });
     const uint o = c.dSortedIndex[i];
 #pragma unroll
     for (int t = 0; t < 9; ++t) cmat[9 * (size_t)o + t] = cv[t] / wsum;
@@ -308,41 +348,78 @@ __device__ inline float det3(const float* g) {
            g[6] * g[4] * g[2];
 }
 
-// G of each slot in sorted order with its determinant: three float4 rows (G row r in .xyz; det in row 0's .w)
-__global__ __launch_bounds__(kTpb) void k_aniso_gather_g(const float* __restrict__ gmat, const uint* __restrict__ sidx,
-                                                         float4* __restrict__ gs, uint n) {
+// Per sorted slot j: rec[j] = (x_j, mu2_j) with mu2_j a lower bound of the squared smallest singular value of
+// G_j, and G_j's rows with det(G_j) in row 0's w.  Every r has |G_j r| >= sigma_min |r|, so a candidate with
+// |r|^2 mu2_j > (1.01 h)^2 is outside the support |G_j r| <= h also after the FP32 rounding of G_j r (its
+// relative error is below 1e-5 while sigma_min is at least 1e-3 of |G_j|; otherwise mu2_j = 0 and no candidate
+// of j is rejected early).  sigma_min^2 is the smallest eigenvalue of G^T G (FP64 Jacobi).
+__global__ __launch_bounds__(kTpb) void k_aniso_prep(const float* __restrict__ gmat, rxParticleCell c, float4* __restrict__ rec,
+                                                     float4* __restrict__ gs) {
     const uint j = blockIdx.x * kTpb + threadIdx.x;
-    if (j >= n) return;
-    const float* g = gmat + 9 * (size_t)sidx[j];
+    if (j >= c.uNumParticles) return;
+    const float* g = gmat + 9 * (size_t)c.dSortedIndex[j];
     float gl[9];
 #pragma unroll
     for (int t = 0; t < 9; ++t) gl[t] = g[t];
+    double a[3][3], v[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            a[r][q] = (double)gl[r] * gl[q] + (double)gl[3 + r] * gl[3 + q] + (double)gl[6 + r] * gl[6 + q];
+    const double tr = a[0][0] + a[1][1] + a[2][2];
+    sym_eigen3(a, v);
+    const double lmin = fmin(fmin(a[0][0], a[1][1]), a[2][2]);
+    const float mu2 = (lmin > 1e-6 * tr) ? (float)(lmin * (1.0 - 1e-6)) : 0.0f;
+    const float4 q = c.dSortedPos[j];
+    rec[j] = make_float4(q.x, q.y, q.z, mu2);
     gs[3 * (size_t)j + 0] = make_float4(gl[0], gl[1], gl[2], det3(gl));
     gs[3 * (size_t)j + 1] = make_float4(gl[3], gl[4], gl[5], 0.0f);
     gs[3 * (size_t)j + 2] = make_float4(gl[6], gl[7], gl[8], 0.0f);
 }
 
-// rx_aniso_kernel.cu:625-715 (calDensityAnisoCellG, sphCalDensityAnisoInGrid)
-__global__ __launch_bounds__(kTpb) void k_aniso_grid_density(float* __restrict__ out, const float4* __restrict__ gs, rxParticleCell c,
-                                                             AnisoK k, uint3 gnum, float3 gmin, float3 glen) {
+// Per cell: the smallest mu2 of its particles (+inf for an empty cell).
+__global__ __launch_bounds__(kTpb) void k_aniso_cell_mu(const float4* __restrict__ rec, rxParticleCell c, float* __restrict__ cmu) {
+    const uint h = blockIdx.x * kTpb + threadIdx.x;
+    if (h >= c.uNumCells) return;
+    const uint s = c.dCellStart[h];
+    float m = __builtin_inff();
+    if (s != kEmpty) {
+        const uint e = c.dCellEnd[h];
+        for (uint j = s; j < e; ++j) m = fminf(m, rec[j].w);
+    }
+    cmu[h] = m;
+}
+
+// rx_aniso_kernel.cu:625-715 (calDensityAnisoCellG, sphCalDensityAnisoInGrid).  Cells whose nearest point is
+// beyond the support of all their particles (squared distance x the cell's smallest mu2 above (1.01 h)^2) are
+// skipped, and so are candidates with |r|^2 mu2_j above it: neither can pass r <= h, so the sums are the
+// reference's term for term.
+__global__ __launch_bounds__(kTpb) void k_aniso_grid_density(float* __restrict__ out, const float4* __restrict__ rec,
+                                                             const float4* __restrict__ gs, const float* __restrict__ cmu,
+                                                             rxParticleCell c, AnisoK k, uint3 gnum, float3 gmin, float3 glen) {
     const size_t idx = (size_t)blockIdx.x * kTpb + threadIdx.x;
     const size_t total = (size_t)gnum.x * gnum.y * gnum.z;
     if (idx >= total) return;
     const uint gx = (uint)(idx % gnum.x), gy = (uint)((idx / gnum.x) % gnum.y), gz = (uint)(idx / ((size_t)gnum.x * gnum.y));
     const float3 p = make_float3(gmin.x + (float)gx * glen.x, gmin.y + (float)gy * glen.y, gmin.z + (float)gz * glen.z);
     const float h = k.h, h3 = h * 3.0f;
+    const float reach2 = (h * 1.01f) * (h * 1.01f);
     const int3 lo = cell_of(make_float3(p.x - h3, p.y - h3, p.z - h3), k);
     const int3 hi = cell_of(make_float3(p.x + h3, p.y + h3, p.z + h3), k);
     float P = 0.0f, part = 0.0f;
+    const float far2 = sqrt_skip2(h);
     for_cells(
-        lo, hi, k, c.dCellStart, c.dCellEnd,
-        [&](uint j) {
-            const float4 q = c.dSortedPos[j];
+        p, lo, hi, k, c.dCellStart, c.dCellEnd, [&](uint j) { return rec[j]; },
+        [&](uint j, const float4& q) {
             const float3 rij = make_float3(p.x - q.x, p.y - q.y, p.z - q.z);
+            if ((rij.x * rij.x + rij.y * rij.y + rij.z * rij.z) * q.w > reach2) return;
             const float4 g0 = gs[3 * (size_t)j], g1 = gs[3 * (size_t)j + 1], g2 = gs[3 * (size_t)j + 2];
             const float3 rg = make_float3(g0.x * rij.x + g0.y * rij.y + g0.z * rij.z, g1.x * rij.x + g1.y * rij.y + g1.z * rij.z,
                                           g2.x * rij.x + g2.y * rij.y + g2.z * rij.z);
-            const float r = len3(rg);
+            const float rg2 = rg.x * rg.x + rg.y * rg.y + rg.z * rg.z;
+            if (rg2 > far2) return;
+            const float r = sqrtf(rg2);
             if (r <= h) {
                 const float qq = h * h - r * r;
                 part = part + g0.w * k.mass * k.wpoly6 * qq * qq * qq;
@@ -351,7 +428,8 @@ __global__ __launch_bounds__(kTpb) void k_aniso_grid_density(float* __restrict__
         [&]() {
             P = P + part;
             part = 0.0f;
-        });
+        },
+        [&](uint cell, float d2) { return d2 >= 0.0f && d2 * cmu[cell] > reach2; });
     out[gx + gy * gnum.x + (size_t)gz * gnum.x * gnum.y] = P;
 }
 
@@ -413,14 +491,16 @@ void CuSphGridDensityAniso(float* dGridD, float* dG, float Emax, rxParticleCell 
     if (nx <= 0 || ny <= 0 || nz <= 0) return;
     const size_t total = (size_t)nx * ny * nz;
     hipStream_t s = rx_stream();
-    // slot 6: G in sorted order with det(G) (48 bytes per particle)
-    float4* gs = nullptr;
-    if (cell.uNumParticles) {
-        gs = (float4*)rx_scratch(6, 48 * (size_t)cell.uNumParticles);
-        if (!gs) return;
-        k_aniso_gather_g<<<rx_blocks(cell.uNumParticles, kTpb), kTpb, 0, s>>>(dG, cell.dSortedIndex, gs, cell.uNumParticles);
-    }
-    k_aniso_grid_density<<<rx_blocks(total, kTpb), kTpb, 0, s>>>(dGridD, gs, cell, aniso_consts(), make_uint3(nx, ny, nz),
+    // slot 6: per slot the position with mu2 (16 B) and G with det(G) (48 B); per cell the smallest mu2 (4 B)
+    const size_t np = cell.uNumParticles;
+    char* ws = (char*)rx_scratch(6, 64 * np + 4 * (size_t)cell.uNumCells + 256);
+    if (!ws) return;
+    float4* rec = (float4*)ws;
+    float4* gs = rec + np;
+    float* cmu = (float*)(gs + 3 * np);
+    if (np) k_aniso_prep<<<rx_blocks(np, kTpb), kTpb, 0, s>>>(dG, cell, rec, gs);
+    k_aniso_cell_mu<<<rx_blocks(cell.uNumCells, kTpb), kTpb, 0, s>>>(rec, cell, cmu);
+    k_aniso_grid_density<<<rx_blocks(total, kTpb), kTpb, 0, s>>>(dGridD, rec, gs, cmu, cell, aniso_consts(), make_uint3(nx, ny, nz),
                                                                  make_float3(x0, y0, z0), make_float3(dx, dy, dz));
     finish();
 }

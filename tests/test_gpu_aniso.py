@@ -16,11 +16,15 @@ from fltk_sph_ice_amd.mesh import MCMesh
 pytestmark = pytest.mark.gpu
 
 
-def _setup(lib, nx, ny, nz, jitter, seed):
+def _setup(lib, nx, ny, nz, jitter, seed, outside=0.0):
     sc = scene.pbf_dam_break(nx, ny, nz)
     pos, vel = sc.particles()
     rng = np.random.default_rng(seed)
     pos[:, :3] += rng.uniform(-jitter, jitter, (sc.n, 3)).astype(np.float32) * np.float32(sc.spacing)
+    if outside:
+        # a share of the particles moved past the grid's low faces: the clamped hash puts them in face cells
+        k = rng.random(sc.n) < outside
+        pos[k, :3] -= rng.uniform(0.05, 0.2, (int(k.sum()), 3)).astype(np.float32)
     sp = sc.params()
     lib.CuSetParameters(C.byref(sp))
     g = GridBuffers(sc.n, sp.NumCells, lib)
@@ -33,9 +37,9 @@ def _setup(lib, nx, ny, nz, jitter, seed):
     return sc, sp, c, (g, dpos, dvel), stage, geo, gsize
 
 
-@pytest.mark.parametrize("jitter", [0.0, 0.3])
-def test_stages_match_cpu_path(lib, jitter):
-    sc, sp, c, keep, (spos, sidx, cs, ce), geo, gsize = _setup(lib, 14, 12, 10, jitter, 3)
+@pytest.mark.parametrize("jitter,outside", [(0.0, 0.0), (0.3, 0.0), (0.3, 0.05)])
+def test_stages_match_cpu_path(lib, jitter, outside):
+    sc, sp, c, keep, (spos, sidx, cs, ce), geo, gsize = _setup(lib, 14, 12, 10, jitter, 3, outside)
     n = sc.n
     h, lam = 2.0 * float(sp.EffectiveRadius), 0.9  # rxSPH_GPU::CalAnisotropicKernel: h = 2 x effective radius
     print(f"aniso n={n} jitter={jitter}", flush=True)
