@@ -129,4 +129,19 @@ private:
     void drop_graphs();
     int enqueue_step(float dt);
     void collect_marks();
+    // Adaptive list sizes (RX_PBF_OPT_ADAPT_SKIN).  Behind each eager step its largest displacement, overflow
+    // and fallback counts (4 stats slots) are copied into a pinned ring with an event; before a step the host
+    // reads the records whose events have completed, without waiting, and sets the skin level and list
+    // capacity the next list build uses.
+    static constexpr int kRecs = 8;
+    float* m_hrec = nullptr;
+    hipEvent_t m_rec_ev[kRecs] = {};
+    long m_rec_head = 0, m_rec_tail = 0;
+    bool m_adapt = true;
+    float m_skin_base = 0.0f;  // level 0; level l uses (1 + l) times it
+    int m_level = 0, m_level_target = 0, m_calm = 0, m_calm_want = 0;
+    int m_maxn_target = 96;
+    void poll_records();
+    void push_record();
+    bool apply_targets(hipStream_t s);
 };

@@ -861,6 +861,16 @@ void rxPBDSPH_GPU::Finalize() {
             *p = nullptr;
         }
     m_grid.release();
+    if (m_hrec) {
+        RX_HIP(hipHostFree(m_hrec));
+        m_hrec = nullptr;
+    }
+    for (hipEvent_t& e : m_rec_ev)
+        if (e) {
+            RX_HIP(hipEventDestroy(e));
+            e = nullptr;
+        }
+    m_rec_head = m_rec_tail = 0;
     for (hipEvent_t e : m_event_pool) RX_HIP(hipEventDestroy(e));
     m_event_pool.clear();
     m_marks.clear();
@@ -873,20 +883,97 @@ void rxPBDSPH_GPU::Finalize() {
     m_have_list = false;
 }
 
+// Bytes of the skin list for n slots of capacity maxn: ceil(maxn / 8) uint4 chunks of 8 16-bit entries per slot.
+static size_t list_bytes(size_t n, int maxn) { return sizeof(uint4) * n * (size_t)((maxn + 7) / 8); }
+
 int rxPBDSPH_GPU::Initialize(const rxSphEnv& env) {
     Finalize();
     if (rxSetupSimParams(&env, &m_params) != 0) return 1;
     m_env = env;
-    m_skin = 0.15f * env.h;
-    m_maxn = 96;
+    m_skin = m_skin_base = 0.15f * env.h;
+    m_level = m_level_target = m_calm = 0;
+    m_maxn = m_maxn_target = 96;
     m_grid.setup(m_params, env.h + m_skin);
     lattice_refs();
     if (!RX_HIP(hipStreamCreateWithFlags(&m_stream, hipStreamNonBlocking))) return 1;
+    if (!RX_HIP(hipHostMalloc((void**)&m_hrec, sizeof(float) * 4 * kRecs, hipHostMallocDefault))) return 1;
+    for (hipEvent_t& e : m_rec_ev)
+        if (!RX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming))) return 1;
     return 0;
 }
 
-// Bytes of the skin list for n slots of capacity maxn: ceil(maxn / 8) uint4 chunks of 8 16-bit entries per slot.
-static size_t list_bytes(size_t n, int maxn) { return sizeof(uint4) * n * (size_t)((maxn + 7) / 8); }
+// Reads the completed records (oldest first) and updates the targets.  The wanted level is the lowest whose skin
+// is at least 2.2 times the step's largest displacement D: then d_i + D <= 2D stays within the skin and no
+// particle walks the grid.  A grid walk costs 10 to 20 list walks while each level up costs 1.4 to 1.9 times
+// the list work, so the level rises (at once, to the wanted one) only when more than 3% of the step's neighbour
+// passes walked the grid; it falls to the wanted level after 2 records in a row want a lower one.  The list
+// capacity rises by 64 when more than n/2000 lists overflowed.
+void rxPBDSPH_GPU::poll_records() {
+    while (m_rec_tail < m_rec_head) {
+        const int slot = (int)(m_rec_tail % kRecs);
+        const hipError_t e = hipEventQuery(m_rec_ev[slot]);
+        if (e == hipErrorNotReady) {
+            (void)hipGetLastError();  // "not ready" is a state, not an error to report later
+            break;
+        }
+        if (!RX_HIP(e)) break;
+        const float* r = m_hrec + 4 * slot;  // ST_DISP2, ST_OVERFLOW, ST_ITERS, ST_FALLBACK
+        const float D = sqrtf(fmaxf(r[0], 0.0f));
+        const uint over = __builtin_bit_cast(uint, r[1]), fb = __builtin_bit_cast(uint, r[3]);
+        ++m_rec_tail;
+        const float need = 2.2f * D;
+        int want = 0;
+        while (want < 3 && m_skin_base * (float)(1 + want) < need) ++want;
+        const float passes = (float)(2 * std::min(m_env.max_iter, 14) + 2);
+        if ((float)fb > 0.03f * (float)m_n * passes && want > m_level_target) {
+            m_level_target = want;
+            m_calm = 0;
+        } else if (want < m_level_target) {
+            m_calm_want = m_calm ? std::max(m_calm_want, want) : want;
+            if (++m_calm >= 2) {
+                m_level_target = m_calm_want;
+                m_calm = 0;
+            }
+        } else {
+            m_calm = 0;
+        }
+        if (over > m_n / 2000 && m_maxn_target < 512) m_maxn_target = std::min(512, m_maxn_target + 64);
+    }
+}
+
+// Copies the step's ST_DISP2..ST_FALLBACK behind its work (skipped while the ring is full: the host is more than
+// kRecs steps ahead of the device).
+void rxPBDSPH_GPU::push_record() {
+    if (!m_hrec || m_rec_head - m_rec_tail >= kRecs) return;
+    const int slot = (int)(m_rec_head % kRecs);
+    if (RX_HIP(hipMemcpyAsync(m_hrec + 4 * slot, m_stats + ST_DISP2, 4 * sizeof(float), hipMemcpyDeviceToHost, m_stream)) &&
+        RX_HIP(hipEventRecord(m_rec_ev[slot], m_stream)))
+        ++m_rec_head;
+}
+
+// Applies the targets between the prediction (which read the previous list) and the rebuild.  A new capacity
+// reallocates the list, so the passes still reading it are waited for first.  A new skin changes the cell of
+// the grid to h + skin; the cells only grow relative to level 0, so the reserved cell arrays suffice.
+bool rxPBDSPH_GPU::apply_targets(hipStream_t s) {
+    bool changed = false;
+    if (m_maxn_target != m_maxn) {
+        drop_graphs();
+        if (!RX_HIP(hipStreamSynchronize(s))) return false;
+        RX_HIP(hipFree(m_list));
+        m_list = nullptr;
+        if (!RX_HIP(hipMalloc(&m_list, list_bytes(m_n, m_maxn_target)))) return false;
+        m_maxn = m_maxn_target;
+        changed = true;
+    }
+    if (m_level_target != m_level) {
+        m_level = m_level_target;
+        m_skin = m_skin_base * (float)(1 + m_level);
+        m_grid.setup(m_params, m_env.h + m_skin);
+        changed = true;
+    }
+    return changed;
+}
+
 
 bool rxPBDSPH_GPU::alloc(uint n) {
     size_t v4 = sizeof(float4) * n, f = sizeof(float) * n, u = sizeof(uint) * n;
@@ -912,6 +999,9 @@ int rxPBDSPH_GPU::SetParticles(const float* pos4, const float* vel4, int n) {
         return 1;
     }
     drop_graphs();
+    m_rec_tail = m_rec_head;  // records of the previous particles no longer apply
+    m_level_target = 0;
+    m_calm = 0;
     if ((uint)n != m_n) {
         for (void** p : {(void**)&m_x, (void**)&m_xs, (void**)&m_xl, (void**)&m_xb, (void**)&m_v, (void**)&m_t, (void**)&m_dens,
                          (void**)&m_lamacc, (void**)&m_id, (void**)&m_idt, (void**)&m_list, (void**)&m_cnt, (void**)&m_stats,
@@ -1039,8 +1129,12 @@ int rxPBDSPH_GPU::Update(float dt, int step) {
     }
     int rc = 0;
     if (!m_use_graph || m_profile || !m_have_list) {
+        if (m_adapt) poll_records();
         rc = enqueue_step(dt);  // the first step (no list yet) and profiled steps run eagerly
+        if (m_adapt && rc == 0) push_record();
     } else {
+        m_level_target = m_level;  // replayed and captured steps keep the sizes they were recorded with
+        m_maxn_target = m_maxn;
         std::vector<unsigned char> key = graph_key(dt);
         StepGraph* hit = nullptr;
         for (StepGraph& g : m_graphs)
@@ -1112,6 +1206,7 @@ int rxPBDSPH_GPU::enqueue_step(float dt) {
     k_predict<<<nb, kTpb, 0, s>>>(m_x, m_v, m_xs, m_dens, m_stats, mode, k, m_list, m_cnt, m_grid.hash, m_grid.off, m_xb);
     end_mark(mk);
     if (m_step > 0) k_fold_run<<<1, 64, 0, s>>>(m_stats, k);
+    if (apply_targets(s)) k = consts(dt);  // list sizes for this step's build (the prediction used the old list)
 
     // ---- rebuild on x*; the step-start density follows the particles
     k_fill<float><<<1, 64, 0, s>>>(m_stats + ST_DISP2, 0.0f, 1);
@@ -1299,6 +1394,8 @@ int rxPBDSPH_GPU::GetStats(float* out, int count) {
     for (int t = 0; t < 16; ++t) h[ST_MPOS_ITER + t] = h[ST_MPOS_ITER + t] / (float)m_n;
     for (int t : {ST_RUN_ITERS, ST_RUN_STEPS, ST_RUN_FAIL}) h[t] = (float)__builtin_bit_cast(uint, h[t]);
     h[ST_DISP2] = sqrtf(h[ST_DISP2]);
+    h[60] = m_skin / m_env.h;  // host state: the list sizes in use
+    h[61] = (float)m_maxn;
     for (int i = 0; i < count && i < ST_N; ++i) out[i] = h[i];
     return 0;
 }
@@ -1454,10 +1551,14 @@ int rxPBDSPH_GPU::SetOption(int opt, float value) {
             m_list = nullptr;
             if (!RX_HIP(hipMalloc(&m_list, list_bytes(m_n, m)))) return 1;
         }
-        m_maxn = m;
+        m_maxn = m_maxn_target = m;
         m_have_list = false;
         return 0;
     }
+    case RX_PBF_OPT_ADAPT_SKIN:
+        m_adapt = value != 0.0f;
+        if (!m_adapt) m_level_target = 0;  // back to the set skin at the next step
+        return 0;
     case RX_PBF_OPT_HEAT_DT: m_heat_dt = value; return 0;
     case RX_PBF_OPT_LIST_TILE:
         if (value < 0.0f || value > 4.0f) {
@@ -1468,7 +1569,9 @@ int rxPBDSPH_GPU::SetOption(int opt, float value) {
         return 0;
     case RX_PBF_OPT_SKIN:
         if (value < 0.0f) return 1;
-        m_skin = value * m_env.h;
+        m_skin = m_skin_base = value * m_env.h;
+        m_level = m_level_target = 0;
+        m_calm = 0;
         m_grid.setup(m_params, m_env.h + m_skin);
         m_grid.reserve(m_n);
         m_have_list = false;

@@ -113,9 +113,17 @@ enum rxPbfOption {
     RX_PBF_OPT_HEAT_TD = 14,       /* thermal diffusion coefficient (HeatTransfar mTD; default 0)         */
     RX_PBF_OPT_HEAT_DT = 15,       /* heat time step (0: the step's dt)                                    */
     RX_PBF_OPT_MAX_NEIGHBORS = 16, /* neighbour-list capacity per particle, a multiple of 4 (default 96)    */
-    RX_PBF_OPT_GRAPH = 17          /* 1: after the first step, each step's launches are recorded once into a
+    RX_PBF_OPT_GRAPH = 17,         /* 1: after the first step, each step's launches are recorded once into a
                                       hipGraph per buffer configuration and replayed (for launch-bound sizes);
                                       profiled steps (RX_PBF_OPT_PROFILE) run eagerly.  0 (default): eager */
+    RX_PBF_OPT_ADAPT_SKIN = 18     /* 1 (default): size the neighbour lists from the steps already completed:
+                                      the skin grows to 2, 3 or 4 times RX_PBF_OPT_SKIN (to 2.2 times the
+                                      largest in-step displacement) when more than 3% of a step's neighbour
+                                      passes walked the grid instead of their lists, and shrinks back after 2
+                                      steps that needed less; the list capacity grows by 64 (up to 512) when
+                                      lists overflow.  The host reads each step's figures once the device has
+                                      finished it, without waiting, so the sizes follow with a lag.  Eager steps
+                                      only (replayed graphs keep their sizes). */
 };
 
 /* Kernel classes of rxPBDSPH_GPU_GetProfile. */
@@ -138,6 +146,7 @@ enum rxProfileClass {
  *   17      largest displacement since the neighbour-list build
  *   18      particles whose neighbour list overflowed
  *   19      corrections executed in the step
+ *   20      neighbour passes of particles that walked the grid instead of their list
  *   22      1 when a displacement since the list build exceeded what the capped fallback walk covers
  *           (4 cells: the passes may then miss pairs)
  *   23      mean |rho - rho0| / rho0 at the step's start positions
@@ -146,7 +155,9 @@ enum rxProfileClass {
  * Run accumulators (since SetParticles or RX_PBF_OPT_RESET_RUN), over the steps judged so far (a step is
  * judged by the prediction pass of the step after it):
  *   48      max of slot 0        49  max of slot 24      50  corrections executed
- *   51      steps judged         52  steps whose judged error exceeded TOL                         */
+ *   51      steps judged         52  steps whose judged error exceeded TOL
+ * Host state at the call:
+ *   60      list skin in use, as a fraction of h      61  list capacity in use                     */
 
 /* Fused PBF solver (rxPBDSPH_GPU, rx_sph.h:1284-1601): Initialize (rx_sph_pbd_gpu.cpp:168-300) and Update
  * (rx_sph_pbd_gpu.cpp:790-957) with min_iter = max_iter Jacobi iterations, CPU-path lambda and collisions,

@@ -29,6 +29,56 @@ def test_one_step_parity(dims):
     assert stt[18] == 0  # no list overflow
 
 
+@pytest.mark.parametrize("skin,maxn", [(0.45, 160), (0.6, 256)])
+def test_one_step_parity_with_wider_lists(skin, maxn):
+    # the list sizes that the adaptive sizing switches to: larger grid cells (h + skin), longer lists
+    sc = scene.pbf_dam_break(16, 12, 14)
+    pos, vel = sc.particles()
+    P = oracle.params_from(sc.params(), sc)
+    rpos, rvel, rdens, st = oracle.pbf_step(pos, vel, P)
+    s = PBFSolver(sc, pos, vel)
+    s.option(_abi.RX_PBF_OPT_SKIN, skin)
+    s.option(_abi.RX_PBF_OPT_MAX_NEIGHBORS, maxn)
+    s.step(1)
+    stt = s.stats()
+    assert stt[60] == pytest.approx(skin, rel=1e-6) and stt[61] == maxn and stt[18] == 0
+    gpos = s.positions()
+    scale = np.abs(rpos[:, :3]).max()
+    assert np.abs(gpos[:, :3] - rpos[:, :3]).max() <= 1e-5 * scale
+
+
+def _colliding_halves(dims=(24, 16, 16), speed=6.0):
+    sc = scene.pbf_dam_break(*dims)
+    pos, vel = sc.particles()
+    mid = np.median(pos[:, 0])
+    vel[:, 0] = np.where(pos[:, 0] < mid, speed, -speed).astype(np.float32)
+    return sc, pos, vel
+
+
+def test_list_sizes_follow_violent_motion():
+    """Two halves of a block thrown at each other: within a step the corrections move particles far past half
+    the skin, so with fixed sizes most neighbour passes walk the grid.  The adaptive sizing widens the skin
+    and the capacity from the completed steps, and the walks become rare."""
+    sc, pos, vel = _colliding_halves()
+    runs = {}
+    for adapt in (1, 0):
+        s = PBFSolver(sc, pos, vel)
+        s.option(_abi.RX_PBF_OPT_ADAPT_SKIN, adapt)
+        rec = []
+        for _ in range(24):
+            s.step(1)
+            st = s.stats()
+            rec.append((st[20], st[60], st[61], st[17] / sc.h))
+        p = s.positions()
+        assert np.isfinite(p).all()
+        runs[adapt] = np.array(rec)
+    a, f = runs[1], runs[0]
+    print("adaptive: skin/h max", a[:, 1].max(), "capacity max", a[:, 2].max(), "walks", a[4:, 0].sum(), "fixed walks", f[4:, 0].sum())
+    assert np.all(f[:, 1] == pytest.approx(0.15)) and np.all(f[:, 2] == 96)
+    assert f[4:, 0].sum() > 0.3 * 10 * sc.n * 20  # the fixed sizes walk the grid for most passes
+    assert a[:, 1].max() > 0.15 and a[4:, 0].sum() < 0.5 * f[4:, 0].sum()
+
+
 def test_multi_step_parity_and_list_reuse():
     """Ten steps: the list built on x* is reused by the next step's predict pass."""
     sc = scene.pbf_dam_break(16, 16, 16)
@@ -129,6 +179,7 @@ def test_graph_replay_matches_eager_launches(variant):
     def make(graph):
         s = PBFSolver(sc, pos, vel)
         s.option(_abi.RX_PBF_OPT_GRAPH, graph)
+        s.option(_abi.RX_PBF_OPT_ADAPT_SKIN, 0)  # replayed graphs keep their list sizes; compare like with like
         if variant == "adaptive_scaled":
             s.option(_abi.RX_PBF_OPT_MODE, _abi.RX_PBF_MODE_SCALED)
             s.option(_abi.RX_PBF_OPT_WARM_SCALE, 3.0)
