@@ -1,6 +1,6 @@
 """ctypes bindings of the CPU oracle (liboracle.so).  TEST INFRASTRUCTURE ONLY.
 
-The oracle is the checker of the HIP path: tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+The HIP path is compared with the oracle: tests/, __graft_entry__.smoke() and the cpu_baseline leg of
 bench.py use it; the product never imports it.
 """
 from __future__ import annotations

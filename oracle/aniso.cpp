@@ -1,5 +1,5 @@
 // aniso.cpp -- CPU restatement of the reference's anisotropic-kernel stages (rx_sph.cu:1126-1276,
-// rx_aniso_kernel.cu:20-718).  TEST INFRASTRUCTURE ONLY: the checker of fltk_sph_ice_amd/csrc/aniso.hip, loaded
+// rx_aniso_kernel.cu:20-718).  TEST INFRASTRUCTURE ONLY: the oracle of fltk_sph_ice_amd/csrc/aniso.hip, loaded
 // by tests/ through liboracle.so.
 //
 // Pinned by (tests/test_oracle_aniso.py): the weighted means and covariances against a brute-force numpy sum

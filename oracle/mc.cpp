@@ -1,5 +1,5 @@
 // mc.cpp -- CPU restatement of the reference's marching-cubes stages (rx_mc.cu:148-389, rx_mc_kernel.cu:319-636).
-// TEST INFRASTRUCTURE ONLY: the checker of fltk_sph_ice_amd/csrc/mc.hip, loaded by tests/ through liboracle.so.
+// TEST INFRASTRUCTURE ONLY: the oracle of fltk_sph_ice_amd/csrc/mc.hip, loaded by tests/ through liboracle.so.
 //
 // Pinned by: the case tables' digests against the reference's tables (tests/golden/mc_tables_digest.json), the
 // closed-surface invariants of a sphere and a torus (every mesh edge shared by two triangles with opposite

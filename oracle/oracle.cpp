@@ -1,5 +1,5 @@
-// oracle.cpp -- CPU restatement of the reference's particle path.  TEST INFRASTRUCTURE ONLY: this is the
-// checker of the HIP path (loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg).
+// oracle.cpp -- CPU restatement of the reference's particle path.  TEST INFRASTRUCTURE ONLY: the oracle the HIP
+// path is compared with (loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg).
 //
 // Structure mirrors the reference CPU classes: a grid (rx_nnsearch.h rxNNGrid), per-particle neighbour
 // vectors rebuilt by SetParticlesToCell (rx_sph_cpu.cpp:1365-1380), then the solver loops.  Kernel values

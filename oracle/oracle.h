@@ -1,7 +1,7 @@
 /*
  * oracle.h -- CPU restatement of the reference's particle path.  TEST INFRASTRUCTURE ONLY.
  *
- * This library is the checker of the HIP path: only the tests, __graft_entry__.smoke() and the CPU-baseline
+ * The HIP path is compared with this library: only the tests, __graft_entry__.smoke() and the CPU-baseline
  * leg of bench.py load it.  Nothing in the product links it.
  *
  * Each function follows a reference CPU routine (cited in oracle.cpp).  Storage precision is selectable:
