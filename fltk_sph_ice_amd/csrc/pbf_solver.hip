@@ -688,7 +688,7 @@ __global__ __launch_bounds__(kTpb) void k_dp(const float4* __restrict__ xl, floa
         float li = p4.w;
         float dt2 = k.dt * k.dt;
         float3 acc = f3(0, 0, 0);
-        for_neighbors<2>(
+        for_neighbors<1>(
             i, 0, k, list, cnt, bhash, off, disp2, p, xbuild, [&](uint j) { return xl[j]; },
             [&](uint, const float4& q4) {
                 float3 r = p - f3(q4);
