@@ -723,8 +723,9 @@ __global__ void k_velocity(const float4* __restrict__ xa, const float4* __restri
     v[i] = make_float4(vv.x, vv.y, vv.z, dens[i]);
 }
 
-// XSPH (rx_pbdsph_kernel.cu:1090-1162 form): reads v = (v, rho), writes vout.
-__global__ __launch_bounds__(kTpb) void k_xsph(const float4* __restrict__ xa, const float4* __restrict__ v, float4* __restrict__ vout,
+// XSPH (rx_pbdsph_kernel.cu:1090-1162 form): reads v = (v, rho), writes vout.  Held to 80 registers (six
+// wavefronts per SIMD, no spills): the pass waits on neighbour gathers, and the extra wavefront hides more of it.
+__global__ __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(6))) void k_xsph(const float4* __restrict__ xa, const float4* __restrict__ v, float4* __restrict__ vout,
                                                float* __restrict__ stats, PbfK k, const uint* __restrict__ list,
                                                const uint4* __restrict__ cnt, const uint* __restrict__ bhash,
                                                const uint* __restrict__ off, const float4* __restrict__ xb) {
