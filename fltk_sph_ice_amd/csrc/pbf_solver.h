@@ -141,6 +141,7 @@ private:
     float m_skin_base = 0.0f;  // level 0; level l uses (1 + l) times it
     int m_level = 0, m_level_target = 0, m_calm = 0, m_calm_want = 0;
     int m_maxn_target = 96;
+    int m_maxn_cap = 512;     // largest capacity the adaptive sizing asks for
     void poll_records();
     void push_record();
     bool apply_targets(hipStream_t s);

@@ -894,6 +894,7 @@ int rxPBDSPH_GPU::Initialize(const rxSphEnv& env) {
     m_skin = m_skin_base = 0.15f * env.h;
     m_level = m_level_target = m_calm = 0;
     m_maxn = m_maxn_target = 96;
+    m_maxn_cap = 512;
     m_grid.setup(m_params, env.h + m_skin);
     lattice_refs();
     if (!RX_HIP(hipStreamCreateWithFlags(&m_stream, hipStreamNonBlocking))) return 1;
@@ -938,7 +939,7 @@ void rxPBDSPH_GPU::poll_records() {
         } else {
             m_calm = 0;
         }
-        if (over > m_n / 2000 && m_maxn_target < 512) m_maxn_target = std::min(512, m_maxn_target + 64);
+        if (over > m_n / 2000 && m_maxn_target < m_maxn_cap) m_maxn_target = std::min(m_maxn_cap, m_maxn_target + 64);
     }
 }
 
@@ -958,13 +959,24 @@ void rxPBDSPH_GPU::push_record() {
 bool rxPBDSPH_GPU::apply_targets(hipStream_t s) {
     bool changed = false;
     if (m_maxn_target != m_maxn) {
-        drop_graphs();
-        if (!RX_HIP(hipStreamSynchronize(s))) return false;
-        RX_HIP(hipFree(m_list));
-        m_list = nullptr;
-        if (!RX_HIP(hipMalloc(&m_list, list_bytes(m_n, m_maxn_target)))) return false;
-        m_maxn = m_maxn_target;
-        changed = true;
+        // the larger list is allocated before the old one is released; when the device has no room for it the
+        // capacity stays (overflowed lists keep walking the grid) and no larger one is asked for again
+        uint* grown = nullptr;
+        if (hipMalloc((void**)&grown, list_bytes(m_n, m_maxn_target)) == hipSuccess) {
+            drop_graphs();
+            if (!RX_HIP(hipStreamSynchronize(s))) {
+                RX_HIP(hipFree(grown));
+                return false;
+            }
+            RX_HIP(hipFree(m_list));
+            m_list = grown;
+            m_maxn = m_maxn_target;
+            changed = true;
+        } else {
+            (void)hipGetLastError();
+            m_maxn_target = m_maxn;
+            m_maxn_cap = m_maxn;
+        }
     }
     if (m_level_target != m_level) {
         m_level = m_level_target;
@@ -1553,6 +1565,7 @@ int rxPBDSPH_GPU::SetOption(int opt, float value) {
             if (!RX_HIP(hipMalloc(&m_list, list_bytes(m_n, m)))) return 1;
         }
         m_maxn = m_maxn_target = m;
+        m_maxn_cap = std::max(512, m);
         m_have_list = false;
         return 0;
     }
