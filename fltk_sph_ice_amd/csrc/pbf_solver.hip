@@ -998,7 +998,9 @@ bool rxPBDSPH_GPU::alloc(uint n) {
               RX_HIP(hipMalloc(&m_feat, v4 * 4)) &&
               RX_HIP(hipMalloc(&m_ftmp16, v4 * 4));
     m_grid.n = n;
+    m_grid.setup(m_params, m_env.h);  // the cell-h grid of the first prediction has the most cells
     m_grid.reserve(n);
+    m_grid.setup(m_params, m_env.h + m_skin);
     return ok;
 }
 
@@ -1201,7 +1203,10 @@ int rxPBDSPH_GPU::enqueue_step(float dt) {
     // ---- predict at x_n
     int mode = 0;
     if (!m_have_list) {
-        // no list yet: grid on x_n, gather, predict with the 3x3x3 stencil of that grid
+        // no list yet: a grid of cell h on x_n (the smallest cell a 3x3x3 stencil allows: 27 cells of about
+        // 8 particles instead of about 12 at the list's cell h + skin), gather, predict with its stencil
+        m_grid.setup(m_params, m_env.h);
+        k = consts(dt);
         m_grid.build(m_x, s);
         reorder(m_x, s);
         reorder(m_v, s);
@@ -1219,6 +1224,10 @@ int rxPBDSPH_GPU::enqueue_step(float dt) {
     k_predict<<<nb, kTpb, 0, s>>>(m_x, m_v, m_xs, m_dens, m_stats, mode, k, m_list, m_cnt, m_grid.hash, m_grid.off, m_xb);
     end_mark(mk);
     if (m_step > 0) k_fold_run<<<1, 64, 0, s>>>(m_stats, k);
+    if (mode == 1) {  // back to the list's grid
+        m_grid.setup(m_params, m_env.h + m_skin);
+        k = consts(dt);
+    }
     if (apply_targets(s)) k = consts(dt);  // list sizes for this step's build (the prediction used the old list)
 
     // ---- rebuild on x*; the step-start density follows the particles
