@@ -272,6 +272,7 @@ void CuMCCalTriNum(float* dVolume, uint* dVoxBit, uint* dVoxVNum, uint* dVoxVNum
         return;
     hipStream_t s = rx_stream();
     uint* tot = (uint*)rx_scratch(4, 4 * sizeof(uint));
+    if (!tot) return;
     uint h[3];
     k_mc_classify<<<rx_blocks(num_voxels, kTpb), kTpb, 0, s>>>(dVolume, grid_size, num_voxels, threshold, dVoxBit, dVoxVNum,
                                                                   dVoxTNum, dVoxOcc);
@@ -317,6 +318,7 @@ void CuMCCalEdgeVrts(float* dVolume, float* dEdgeVrts, float* dCompactedEdgeVrts
     k_mc_edges<<<rx_blocks(total, kTpb), kTpb, 0, s>>>(dVolume, k, (float4*)dEdgeVrts, dEdgeOcc);
     rx_scan_u32(dEdgeOcc, dEdgeOccScan, total, false, s);
     uint* tot = (uint*)rx_scratch(4, 4 * sizeof(uint));
+    if (!tot) return;
     uint h[3];
     k_scan_totals<<<1, 64, 0, s>>>(dEdgeOcc, dEdgeOccScan, nullptr, nullptr, nullptr, nullptr, total, tot);
     if (!read_totals(tot, h, s)) return;

@@ -183,6 +183,7 @@ template <class T> void scan_impl(const T* in, T* out, size_t n, hipStream_t s, 
         return;
     }
     T* sums = ws ? (T*)ws : (T*)rx_scratch(8 + level, sizeof(T) * tiles);
+    if (!sums) return;  // the failed allocation is recorded as the last error
     char* next = ws ? ws + ((sizeof(T) * tiles + 255) & ~(size_t)255) : nullptr;
     k_scan_reduce<T><<<(unsigned)tiles, kScanTpb, 0, s>>>(in, sums, n);
     scan_impl<T>(sums, sums, tiles, s, level + 1, next);
@@ -211,6 +212,7 @@ void rx_scan_u32(const uint* in, uint* out, size_t n, bool write_total, hipStrea
     }
     // keep the last input element: the scan may overwrite it when out aliases in
     uint* last = w ? (uint*)w : (uint*)rx_scratch(7, sizeof(uint));
+    if (!last) return;
     if (n) k_copy_last<<<1, 64, 0, s>>>(in, last, n);
     scan_impl<uint>(in, out, n, s, 0, w ? w + 256 : nullptr);
     k_write_total<uint><<<1, 64, 0, s>>>(last, out, out + n, n);
@@ -426,6 +428,7 @@ void CuSort(unsigned int* dHash, uint* dIndex, uint numParticles) {
     size_t nb = (size_t)nc + 1;
     uint* cnt = (uint*)rx_scratch(0, sizeof(uint) * (nb + 1));
     uint* aux = (uint*)rx_scratch(1, sizeof(uint) * (size_t)n * 7 + 64);
+    if (!cnt || !aux) return;
     uint* slot = aux;
     uint* bucket = aux + n;
     uint* isdel = aux + 2 * (size_t)n;

@@ -1070,6 +1070,7 @@ void CuPbdSphDensityErrorStats(float* dDens, float rest_dens, uint nprts, float*
     hipStream_t s = rx_stream();
     int nb = (int)std::min<uint>(rx_blocks(nprts, 256), 1024);
     char* scr = (char*)rx_scratch(5, (sizeof(float) + sizeof(double)) * nb + 64);
+    if (!scr) return;
     double* psum = (double*)scr;
     float* pmax = (float*)(scr + sizeof(double) * nb);
     float* out = pmax + nb;
