@@ -491,14 +491,23 @@ void CuSphGridDensityAniso(float* dGridD, float* dG, float Emax, rxParticleCell 
     if (nx <= 0 || ny <= 0 || nz <= 0) return;
     const size_t total = (size_t)nx * ny * nz;
     hipStream_t s = rx_stream();
-    // slot 6: per slot the position with mu2 (16 B) and G with det(G) (48 B); per cell the smallest mu2 (4 B)
     const size_t np = cell.uNumParticles;
+    if (!dGridD) {
+        rx_set_error(2, "%s: null grid array", fn);
+        return;
+    }
+    if (!np) {  // no particles: every lattice point has density 0 (the cell arrays may be unset)
+        rx_fill<float>(dGridD, 0.0f, total, s);
+        finish();
+        return;
+    }
+    // slot 6: per slot the position with mu2 (16 B) and G with det(G) (48 B); per cell the smallest mu2 (4 B)
     char* ws = (char*)rx_scratch(6, 64 * np + 4 * (size_t)cell.uNumCells + 256);
     if (!ws) return;
     float4* rec = (float4*)ws;
     float4* gs = rec + np;
     float* cmu = (float*)(gs + 3 * np);
-    if (np) k_aniso_prep<<<rx_blocks(np, kTpb), kTpb, 0, s>>>(dG, cell, rec, gs);
+    k_aniso_prep<<<rx_blocks(np, kTpb), kTpb, 0, s>>>(dG, cell, rec, gs);
     k_aniso_cell_mu<<<rx_blocks(cell.uNumCells, kTpb), kTpb, 0, s>>>(rec, cell, cmu);
     k_aniso_grid_density<<<rx_blocks(total, kTpb), kTpb, 0, s>>>(dGridD, rec, gs, cmu, cell, aniso_consts(), make_uint3(nx, ny, nz),
                                                                  make_float3(x0, y0, z0), make_float3(dx, dy, dz));

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import oracle
-from fltk_sph_ice_amd import check, scene
+from fltk_sph_ice_amd import _abi, check, scene
 from fltk_sph_ice_amd.abi_driver import GridBuffers
 from fltk_sph_ice_amd.device import DevBuf
 from fltk_sph_ice_amd.mesh import MCMesh
@@ -94,3 +94,17 @@ def test_anisotropic_surface_mesh(lib):
     block_hi = np.array(sc.particles()[0][:, :3].max(0))
     # the surface wraps the block: vertices within about a particle spacing of it
     assert V.min() > -0.03 and np.all(V.max(0) < block_hi + 3 * sc.spacing)
+
+
+def test_lattice_of_no_particles_is_zero(lib):
+    # an empty fluid: the cell arrays are unset and every lattice point reads 0
+    sp = scene.pbf_dam_break(4, 4, 4).params()
+    lib.CuSetParameters(C.byref(sp))
+    c = _abi.rxParticleCell()
+    c.uNumCells = sp.NumCells
+    nlat = (9, 7, 5)
+    vol = DevBuf.from_array(np.full(int(np.prod(nlat)), 7.0, np.float32))
+    d = np.float32(0.05)
+    lib.CuSphGridDensityAniso(vol.ptr, None, 0.0, c, *nlat, 0.0, 0.0, 0.0, d, d, d)
+    check(lib)
+    assert np.array_equal(vol.download(np.float32), np.zeros(int(np.prod(nlat)), np.float32))
