@@ -75,7 +75,7 @@ struct FusedGrid {
     size_t cap_n = 0, cap_c = 0;
 
     void setup(const rxSimParams& p, float cell);
-    void reserve(uint n_particles);
+    bool reserve(uint n_particles);  // false (and every array released) when an allocation fails
     void release();
     // Builds the grid on pos (in the current slot order); afterwards hash/off describe the new order and
     // perm maps new slots to current slots.  Nothing is reordered here.

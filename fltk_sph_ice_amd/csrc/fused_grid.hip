@@ -66,29 +66,32 @@ void FusedGrid::setup(const rxSimParams& p, float cell) {
     k.ncells = (uint)k.gx * (uint)k.gy * (uint)k.gz;
 }
 
-void FusedGrid::reserve(uint np) {
+bool FusedGrid::reserve(uint np) {
+    bool ok = true;
     if (np > cap_n) {
-        if (hash) {
-            RX_HIP(hipFree(hash));
-            RX_HIP(hipFree(perm));
-            RX_HIP(hipFree(tmp_hash));
-            RX_HIP(hipFree(slot));
-            RX_HIP(hipFree(tsrc));
-        }
-        RX_HIP(hipMalloc(&hash, sizeof(uint) * np));
-        RX_HIP(hipMalloc(&perm, sizeof(uint) * np));
-        RX_HIP(hipMalloc(&tmp_hash, sizeof(uint) * np));
-        RX_HIP(hipMalloc(&slot, sizeof(uint) * np));
-        RX_HIP(hipMalloc(&tsrc, sizeof(uint) * np));
-        cap_n = np;
+        for (uint** p : {&hash, &perm, &tmp_hash, &slot, &tsrc})
+            if (*p) {
+                RX_HIP(hipFree(*p));
+                *p = nullptr;
+            }
+        cap_n = 0;
+        ok = RX_HIP(hipMalloc(&hash, sizeof(uint) * np)) && RX_HIP(hipMalloc(&perm, sizeof(uint) * np)) &&
+             RX_HIP(hipMalloc(&tmp_hash, sizeof(uint) * np)) && RX_HIP(hipMalloc(&slot, sizeof(uint) * np)) &&
+             RX_HIP(hipMalloc(&tsrc, sizeof(uint) * np));
+        if (ok) cap_n = np;
     }
-    if ((size_t)k.ncells + 1 > cap_c) {
+    if (ok && (size_t)k.ncells + 1 > cap_c) {
         if (off) RX_HIP(hipFree(off));
         if (scan_ws) RX_HIP(hipFree(scan_ws));
-        RX_HIP(hipMalloc(&off, sizeof(uint) * ((size_t)k.ncells + 2)));
-        RX_HIP(hipMalloc(&scan_ws, rx_scan_ws_bytes((size_t)k.ncells + 1)));
-        cap_c = (size_t)k.ncells + 1;
+        off = nullptr;
+        scan_ws = nullptr;
+        cap_c = 0;
+        ok = RX_HIP(hipMalloc(&off, sizeof(uint) * ((size_t)k.ncells + 2))) &&
+             RX_HIP(hipMalloc(&scan_ws, rx_scan_ws_bytes((size_t)k.ncells + 1)));
+        if (ok) cap_c = (size_t)k.ncells + 1;
     }
+    if (!ok) release();
+    return ok;
 }
 
 void FusedGrid::release() {

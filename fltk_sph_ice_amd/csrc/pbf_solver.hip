@@ -999,7 +999,7 @@ bool rxPBDSPH_GPU::alloc(uint n) {
               RX_HIP(hipMalloc(&m_ftmp16, v4 * 4));
     m_grid.n = n;
     m_grid.setup(m_params, m_env.h);  // the cell-h grid of the first prediction has the most cells
-    m_grid.reserve(n);
+    ok = ok && m_grid.reserve(n);
     m_grid.setup(m_params, m_env.h + m_skin);
     return ok;
 }
@@ -1026,7 +1026,10 @@ int rxPBDSPH_GPU::SetParticles(const float* pos4, const float* vel4, int n) {
                 *p = nullptr;
             }
         m_n = (uint)n;
-        if (!alloc(m_n)) return 1;
+        if (!alloc(m_n)) {
+            m_n = 0;  // Update refuses until a SetParticles succeeds
+            return 1;
+        }
     }
     hipStream_t s = m_stream;
     RX_HIP(hipMemcpyAsync(m_x, pos4, sizeof(float4) * n, hipMemcpyHostToDevice, s));
@@ -1567,11 +1570,12 @@ int rxPBDSPH_GPU::SetOption(int opt, float value) {
             return 1;
         }
         if (m != m_maxn && m_list) {
+            uint* nl = nullptr;  // the new list first: on failure the old one stays in use
+            if (!RX_HIP(hipMalloc(&nl, list_bytes(m_n, m)))) return 1;
             drop_graphs();
             RX_HIP(hipStreamSynchronize(m_stream));
             RX_HIP(hipFree(m_list));
-            m_list = nullptr;
-            if (!RX_HIP(hipMalloc(&m_list, list_bytes(m_n, m)))) return 1;
+            m_list = nl;
         }
         m_maxn = m_maxn_target = m;
         m_maxn_cap = std::max(512, m);
@@ -1596,8 +1600,11 @@ int rxPBDSPH_GPU::SetOption(int opt, float value) {
         m_level = m_level_target = 0;
         m_calm = 0;
         m_grid.setup(m_params, m_env.h + m_skin);
-        m_grid.reserve(m_n);
         m_have_list = false;
+        if (!m_grid.reserve(m_n)) {
+            m_n = 0;  // Update refuses until a SetParticles succeeds
+            return 1;
+        }
         return 0;
     default:
         rx_set_error(1, "rxPBDSPH_GPU::SetOption: unknown option %d", opt);

@@ -196,10 +196,11 @@ public:
             size_t v4 = sizeof(float4) * n;
             if (!(RX_HIP(hipMalloc(&m_x, v4)) && RX_HIP(hipMalloc(&m_v, v4)) && RX_HIP(hipMalloc(&m_xw, v4)) &&
                   RX_HIP(hipMalloc(&m_vw, v4)) && RX_HIP(hipMalloc(&m_dp, sizeof(float2) * n)) &&
-                  RX_HIP(hipMalloc(&m_id, sizeof(uint) * n)) && RX_HIP(hipMalloc(&m_idw, sizeof(uint) * n))))
+                  RX_HIP(hipMalloc(&m_id, sizeof(uint) * n)) && RX_HIP(hipMalloc(&m_idw, sizeof(uint) * n)) && m_grid.reserve(m_n))) {
+                m_n = 0;  // Update refuses until a SetParticles succeeds
                 return 1;
+            }
             m_grid.n = m_n;
-            m_grid.reserve(m_n);
         }
         RX_HIP(hipMemcpyAsync(m_x, pos4, sizeof(float4) * n, hipMemcpyHostToDevice, m_stream));
         if (vel4) RX_HIP(hipMemcpyAsync(m_v, vel4, sizeof(float4) * n, hipMemcpyHostToDevice, m_stream));
