@@ -418,15 +418,13 @@ __global__ void k_gather_rows(const float* __restrict__ X, const float* __restri
 class rxMLP {
 public:
     explicit rxMLP(unsigned seed) {
-        RX_HIP(hipStreamCreateWithFlags(&m_stream, hipStreamNonBlocking));
-        RX_HIP(hipMalloc(&d_p, sizeof(float) * kNumParams));
-        RX_HIP(hipMalloc(&d_g, sizeof(float) * (kNumParams + 1)));
-        RX_HIP(hipMalloc(&d_m1, sizeof(float) * kNumParams));
-        RX_HIP(hipMalloc(&d_m2, sizeof(float) * kNumParams));
-        RX_HIP(hipMalloc(&d_w1, sizeof(unsigned short) * kMlpH * kMlpIn));
-        RX_HIP(hipMalloc(&d_w2, sizeof(unsigned short) * kMlpH * kMlpH));
-        RX_HIP(hipMalloc(&d_w2t, sizeof(unsigned short) * kMlpH * kMlpH));
-        RX_HIP(hipMalloc(&d_norm, sizeof(float) * 32));
+        m_ok = RX_HIP(hipStreamCreateWithFlags(&m_stream, hipStreamNonBlocking)) &&
+               RX_HIP(hipMalloc(&d_p, sizeof(float) * kNumParams)) && RX_HIP(hipMalloc(&d_g, sizeof(float) * (kNumParams + 1))) &&
+               RX_HIP(hipMalloc(&d_m1, sizeof(float) * kNumParams)) && RX_HIP(hipMalloc(&d_m2, sizeof(float) * kNumParams)) &&
+               RX_HIP(hipMalloc(&d_w1, sizeof(unsigned short) * kMlpH * kMlpIn)) &&
+               RX_HIP(hipMalloc(&d_w2, sizeof(unsigned short) * kMlpH * kMlpH)) &&
+               RX_HIP(hipMalloc(&d_w2t, sizeof(unsigned short) * kMlpH * kMlpH)) && RX_HIP(hipMalloc(&d_norm, sizeof(float) * 32));
+        if (!m_ok) return;  // rxMLP_Create deletes the object
         // He-uniform initialisation (seeded); padded inputs 12..15 keep zero weights
         std::mt19937 g(seed);
         std::vector<float> p(kNumParams, 0.0f);
@@ -486,17 +484,19 @@ public:
     }
     void pack() { k_mlp_pack<<<rx_blocks(kMlpH * kMlpH, 256), 256, 0, m_stream>>>(d_p, d_w1, d_w2, d_w2t); }
 
-    void ensure_batch(uint b) {
-        if (b <= m_bcap) return;
+    bool ensure_batch(uint b) {
+        if (b <= m_bcap) return true;
         free_batch();
-        RX_HIP(hipMalloc(&d_z1, sizeof(float) * kMlpH * b));
-        RX_HIP(hipMalloc(&d_z2, sizeof(float) * kMlpH * b));
-        RX_HIP(hipMalloc(&d_dz1, sizeof(float) * kMlpH * b));
-        RX_HIP(hipMalloc(&d_dz2, sizeof(float) * kMlpH * b));
-        RX_HIP(hipMalloc(&d_y, sizeof(float) * b));
-        RX_HIP(hipMalloc(&d_xb, sizeof(float) * kMlpIn * b));
-        RX_HIP(hipMalloc(&d_tb, sizeof(float) * b));
+        const bool ok = RX_HIP(hipMalloc(&d_z1, sizeof(float) * kMlpH * b)) && RX_HIP(hipMalloc(&d_z2, sizeof(float) * kMlpH * b)) &&
+                        RX_HIP(hipMalloc(&d_dz1, sizeof(float) * kMlpH * b)) && RX_HIP(hipMalloc(&d_dz2, sizeof(float) * kMlpH * b)) &&
+                        RX_HIP(hipMalloc(&d_y, sizeof(float) * b)) && RX_HIP(hipMalloc(&d_xb, sizeof(float) * kMlpIn * b)) &&
+                        RX_HIP(hipMalloc(&d_tb, sizeof(float) * b));
+        if (!ok) {
+            free_batch();
+            return false;
+        }
         m_bcap = b;
+        return true;
     }
     void free_batch() {
         for (float** q : {&d_z1, &d_z2, &d_dz1, &d_dz2, &d_y, &d_xb, &d_tb})
@@ -508,8 +508,8 @@ public:
     }
 
     // Gradients of 0.5 mean (y - t)^2 over n rows (standardised target t) into d_g (+ loss at d_g[kNumParams]).
-    void gradients(const float* X, const float* T, uint n) {
-        ensure_batch(n);
+    bool gradients(const float* X, const float* T, uint n) {
+        if (!ensure_batch(n)) return false;
         hipStream_t s = m_stream;
         MlpDev v = view();
         rx_fill<float>(d_g, 0.0f, kNumParams + 1, s);
@@ -518,6 +518,7 @@ public:
         k_mlp_bwd_rows<<<blocks, 256, 0, s>>>(T, d_y, d_z1, d_z2, n, v, d_dz1, d_dz2, d_g, d_g + kNumParams);
         int rpb = 256;
         k_mlp_wgrad<<<(n + rpb - 1) / rpb, 256, 0, s>>>(X, d_z1, d_dz1, d_dz2, n, rpb, v, d_g);
+        return true;
     }
 
     void adam_step(float lr, float wd) {
@@ -532,15 +533,20 @@ public:
         hipStream_t s = m_stream;
         float *dX = nullptr, *dT = nullptr;
         int* didx = nullptr;
+        auto release = [&]() {
+            for (void* q : {(void*)dX, (void*)dT, (void*)didx})
+                if (q) RX_HIP(hipFree(q));
+        };
         if (!(RX_HIP(hipMalloc(&dX, sizeof(float) * kMlpIn * n)) && RX_HIP(hipMalloc(&dT, sizeof(float) * n)) &&
-              RX_HIP(hipMalloc(&didx, sizeof(int) * n))))
+              RX_HIP(hipMalloc(&didx, sizeof(int) * n)) && ensure_batch((uint)batch))) {
+            release();
             return 1;
+        }
         RX_HIP(hipMemcpyAsync(dX, hX, sizeof(float) * kMlpIn * n, hipMemcpyHostToDevice, s));
         RX_HIP(hipMemcpyAsync(dT, hT, sizeof(float) * n, hipMemcpyHostToDevice, s));
         std::mt19937 g(seed);
         std::vector<int> perm(n);
         for (int i = 0; i < n; ++i) perm[i] = i;
-        ensure_batch((uint)batch);
         int nbatches = n / batch;
         for (int ep = 0; ep < epochs; ++ep) {
             for (int i = n - 1; i > 0; --i) std::swap(perm[i], perm[g() % (uint)(i + 1)]);
@@ -561,9 +567,7 @@ public:
             if (hist) hist[ep] = (float)eloss;
         }
         RX_HIP(hipStreamSynchronize(s));
-        RX_HIP(hipFree(dX));
-        RX_HIP(hipFree(dT));
-        RX_HIP(hipFree(didx));
+        release();
         return RX_HIP(hipGetLastError()) ? 0 : 1;
     }
 
@@ -572,6 +576,7 @@ public:
     unsigned short *d_w1 = nullptr, *d_w2 = nullptr, *d_w2t = nullptr;
     float *d_z1 = nullptr, *d_z2 = nullptr, *d_dz1 = nullptr, *d_dz2 = nullptr, *d_y = nullptr, *d_xb = nullptr, *d_tb = nullptr;
     uint m_bcap = 0;
+    bool m_ok = false;  // every array of the constructor was allocated
     float m_out_scale = 1.0f, m_out_bias = 0.0f;
     int m_t = 0;
 };
@@ -597,7 +602,14 @@ bool rx_mlp_view(void* obj, MlpDev* out) {
 extern "C" {
 
 int rxMLP_NumParams(void) { return kNumParams; }
-void* rxMLP_Create(unsigned seed) { return new rxMLP(seed); }
+void* rxMLP_Create(unsigned seed) {
+    rxMLP* m = new rxMLP(seed);
+    if (!m->m_ok) {
+        delete m;
+        return nullptr;
+    }
+    return m;
+}
 void rxMLP_Destroy(void* h) { delete (rxMLP*)h; }
 int rxMLP_SetParams(void* h, const float* params) {
     if (!h || !params) return 1;
@@ -636,10 +648,18 @@ int rxMLP_Gradients(void* h, const float* X, const float* T, int n, float* grads
     if (!h || !X || !T || n <= 0) return 1;
     rxMLP* p = (rxMLP*)h;
     float *dX = nullptr, *dT = nullptr;
-    if (!(RX_HIP(hipMalloc(&dX, sizeof(float) * kMlpIn * n)) && RX_HIP(hipMalloc(&dT, sizeof(float) * n)))) return 1;
-    RX_HIP(hipMemcpyAsync(dX, X, sizeof(float) * kMlpIn * n, hipMemcpyHostToDevice, p->m_stream));
-    RX_HIP(hipMemcpyAsync(dT, T, sizeof(float) * n, hipMemcpyHostToDevice, p->m_stream));
-    p->gradients(dX, dT, (uint)n);
+    bool ok = RX_HIP(hipMalloc(&dX, sizeof(float) * kMlpIn * n)) && RX_HIP(hipMalloc(&dT, sizeof(float) * n));
+    if (ok) {
+        RX_HIP(hipMemcpyAsync(dX, X, sizeof(float) * kMlpIn * n, hipMemcpyHostToDevice, p->m_stream));
+        RX_HIP(hipMemcpyAsync(dT, T, sizeof(float) * n, hipMemcpyHostToDevice, p->m_stream));
+        ok = p->gradients(dX, dT, (uint)n);
+    }
+    if (!ok) {
+        RX_HIP(hipStreamSynchronize(p->m_stream));
+        for (void* q : {(void*)dX, (void*)dT})
+            if (q) RX_HIP(hipFree(q));
+        return 1;
+    }
     if (grads) RX_HIP(hipMemcpyAsync(grads, p->d_g, sizeof(float) * kNumParams, hipMemcpyDeviceToHost, p->m_stream));
     if (loss) RX_HIP(hipMemcpyAsync(loss, p->d_g + kNumParams, sizeof(float), hipMemcpyDeviceToHost, p->m_stream));
     RX_HIP(hipStreamSynchronize(p->m_stream));

@@ -46,6 +46,9 @@ class Network:
     def __init__(self, seed: int = 0):
         self.lib = load()
         self.h = self.lib.rxMLP_Create(seed)
+        if not self.h:
+            check(self.lib)
+            raise RuntimeError("rxMLP_Create failed")
         self.np = self.lib.rxMLP_NumParams()
 
     def params(self) -> np.ndarray:
