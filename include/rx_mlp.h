@@ -15,6 +15,7 @@ extern "C" {
 #endif
 
 int rxMLP_NumParams(void);
+/* Returns NULL (and records the error) when a device allocation fails. */
 void* rxMLP_Create(unsigned seed);
 void rxMLP_Destroy(void* mlp);
 int rxMLP_SetParams(void* mlp, const float* params);
