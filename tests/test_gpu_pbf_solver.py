@@ -11,7 +11,7 @@ from fltk_sph_ice_amd.solvers import PBFSolver
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("dims", [(14, 14, 14), (20, 9, 11)])
+@pytest.mark.parametrize("dims", [(14, 14, 14), (20, 9, 11), (1, 1, 1), (3, 2, 1)])
 def test_one_step_parity(dims):
     sc = scene.pbf_dam_break(*dims)
     pos, vel = sc.particles()
