@@ -26,6 +26,23 @@ def test_steps_parity(graph):
     assert np.abs(gv[:, :3] - rv[:, :3]).max() <= 1e-3 * np.abs(rv[:, :3]).max()
 
 
+@pytest.mark.parametrize("dims", [(1, 1, 1), (3, 2, 1)])
+def test_tiny_fluids_parity(dims):
+    sc = scene.wcsph_dam_break(*dims)
+    sc.viscosity = 3.5 / 998.29
+    pos, vel = sc.particles()
+    P = oracle.params_from(sc.params(), sc)
+    rp, rv = pos, vel
+    for _ in range(3):
+        rp, rv, rd = oracle.sph_step(rp, rv, P, sc.dt)
+    s = SPHSolver(sc, pos, vel, graph=False)
+    s.step(3)
+    gp, gv, gd = s.array(_abi.RX_ARRAY_POSITION), s.array(_abi.RX_ARRAY_VELOCITY), s.array(_abi.RX_ARRAY_DENSITY)
+    np.testing.assert_allclose(gd, rd, rtol=1e-5)
+    assert np.abs(gp[:, :3] - rp[:, :3]).max() <= 1e-5
+    assert np.abs(gv[:, :3] - rv[:, :3]).max() <= 1e-3 * np.abs(rv[:, :3]).max()
+
+
 def test_graph_and_eager_agree_bitwise():
     sc = scene.wcsph_dam_break(20, 12, 9)
     pos, vel = sc.particles()
