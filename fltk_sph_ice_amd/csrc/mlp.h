@@ -28,5 +28,12 @@ struct MlpDev {
 void rx_mlp_infer_to_w(const float* feat, float4* xl, uint n, const MlpDev& m, float w_scale, hipStream_t s);
 // Inference into a plain array: y[i] = y_i * out_scale + out_bias.
 void rx_mlp_infer(const float* feat, float* y, uint n, const MlpDev& m, hipStream_t s);
-// Copies the device view out of an rxMLP object (returns false for a null object).
+// Copies the device view out of an rxMLP object (returns false for a null object).  The view borrows the
+// object's memory: it is valid while the object lives and reflects later SetParams / SetNormalization calls.
 bool rx_mlp_view(void* mlp_object, MlpDev* out);
+// Bytes of an inference copy of a network (weights, biases and standardisation; no training state).
+size_t rx_mlp_copy_bytes();
+// Copies the inference weights of an rxMLP object into dst (device memory of rx_mlp_copy_bytes() bytes) on
+// stream s, after the object's pending work has finished, and points *out at the copy: the copy stays valid
+// after the object is changed or destroyed.
+bool rx_mlp_copy(void* mlp_object, void* dst, hipStream_t s, MlpDev* out);

@@ -599,6 +599,46 @@ bool rx_mlp_view(void* obj, MlpDev* out) {
     return true;
 }
 
+// Inference copy: bf16 W1 (2 KB) and W2 (8 KB), then FP32 b1, b2, w3, b3 and the standardisation constants.
+namespace {
+constexpr size_t kCopyW2 = sizeof(unsigned short) * kMlpH * kMlpIn;
+constexpr size_t kCopyF = kCopyW2 + sizeof(unsigned short) * kMlpH * kMlpH;
+constexpr size_t kCopyB1 = kCopyF, kCopyB2 = kCopyB1 + sizeof(float) * kMlpH;  // b2, w3, b3 follow contiguously
+constexpr size_t kCopyNorm = kCopyB2 + sizeof(float) * (2 * kMlpH + 1);
+constexpr size_t kCopyBytes = kCopyNorm + sizeof(float) * 32;
+}  // namespace
+size_t rx_mlp_copy_bytes() { return kCopyBytes; }
+
+bool rx_mlp_copy(void* obj, void* dst, hipStream_t s, MlpDev* out) {
+    if (!obj || !dst) return false;
+    rxMLP* m = (rxMLP*)obj;
+    // the packing kernel of the last SetParams runs on the network's own stream: finish it before copying
+    if (!RX_HIP(hipStreamSynchronize(m->m_stream))) return false;
+    const MlpDev v = m->view();
+    unsigned char* d = (unsigned char*)dst;
+    const bool ok =
+        RX_HIP(hipMemcpyAsync(d, v.w1, kCopyW2, hipMemcpyDeviceToDevice, s)) &&
+        RX_HIP(hipMemcpyAsync(d + kCopyW2, v.w2, kCopyF - kCopyW2, hipMemcpyDeviceToDevice, s)) &&
+        RX_HIP(hipMemcpyAsync(d + kCopyB1, v.b1, sizeof(float) * kMlpH, hipMemcpyDeviceToDevice, s)) &&
+        RX_HIP(hipMemcpyAsync(d + kCopyB2, v.b2, sizeof(float) * (2 * kMlpH + 1), hipMemcpyDeviceToDevice, s)) &&
+        RX_HIP(hipMemcpyAsync(d + kCopyNorm, v.in_mu, sizeof(float) * 32, hipMemcpyDeviceToDevice, s));
+    if (!ok) return false;
+    MlpDev c;
+    c.w1 = (const unsigned short*)d;
+    c.w2 = (const unsigned short*)(d + kCopyW2);
+    c.w2t = nullptr;  // backward only
+    c.b1 = (const float*)(d + kCopyB1);
+    c.b2 = (const float*)(d + kCopyB2);
+    c.w3 = c.b2 + kMlpH;
+    c.b3 = c.w3 + kMlpH;
+    c.in_mu = (const float*)(d + kCopyNorm);
+    c.in_isd = c.in_mu + 16;
+    c.out_scale = v.out_scale;
+    c.out_bias = v.out_bias;
+    *out = c;
+    return true;
+}
+
 extern "C" {
 
 int rxMLP_NumParams(void) { return kNumParams; }

@@ -21,7 +21,7 @@ public:
     int GetStats(float* out, int count);
     int SetOption(int opt, float value);
     int Sync();
-    int SetNetwork(void* mlp_object);  // weights of an rxMLP object (copied)
+    int SetNetwork(void* mlp_object);  // inference weights of an rxMLP object (copied; the object may go)
     int SetThermal(const float* temps, const float* heats, const int* phase);
     void FeatureScales(float* sd_ref, float* nref) const {
         if (sd_ref) *sd_ref = m_sd_ref;
@@ -69,9 +69,12 @@ private:
     bool m_adaptive = false;
     float m_warm_scale = 1.0f;
     int m_judge = 0;
+    bool m_grad_mass = false;  // RX_PBF_OPT_GRAD_MASS
+    float m_gamma = 0.0f;      // RX_PBF_OPT_MEMORY
     int m_list_tile = 0;  // LDS tiling variant of the list pass (RX_PBF_OPT_LIST_TILE)
     float m_sd_ref = 1.0f, m_nref = 1.0f;
-    MlpDev m_mlp{};
+    MlpDev m_mlp{};            // points into m_mlp_mem: the solver's own copy of the network
+    void* m_mlp_mem = nullptr;
     bool m_mlp_ready = false;
     int m_step = 0;
     // device state, all in cell order
@@ -134,7 +137,9 @@ private:
     // reads the records whose events have completed, without waiting, and sets the skin level and list
     // capacity the next list build uses.
     static constexpr int kRecs = 8;
+    static constexpr int kRecLen = 6;  // stats slots copied per record (ST_DISP2 .. ST_BAD)
     float* m_hrec = nullptr;
+    long m_bad_steps = 0, m_bad_reported = 0;  // completed steps whose fallback reach was capped, and those reported
     hipEvent_t m_rec_ev[kRecs] = {};
     long m_rec_head = 0, m_rec_tail = 0;
     bool m_adapt = true;

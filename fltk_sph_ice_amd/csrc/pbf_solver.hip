@@ -31,13 +31,17 @@ struct PbfK {
     float skin;       // list cutoff minus h
     float wide2;      // largest displacement^2 the capped fallback walk still covers
     uint* fallback;   // counts the passes of particles that walked the grid instead of their list
+    uint* bad;        // stats slots ST_BAD and ST_RUN_BAD: set when a walk's reach was capped (pairs missed)
     int maxn;
     float eps;
+    float gfac;       // constraint-gradient factor: spiky gradient coefficient / rho0, times m in the mass mode
     float dt, inv_dt;
     float xsph_c;
     float tol;        // adaptive solve: stop once the judged density error is at most tol
     int judge;        // 0: max (rho - rho0)+ / rho0; 1: its mean over the particles
     float lam0_scale; // factor on the first multiplier (scaled mode; 1 otherwise)
+    const float* prev;// the multiplier each particle's previous step applied in total (slot order), or null
+    float gamma;      // share of min(prev, 0) added to the first multiplier (RX_PBF_OPT_MEMORY)
     // feature normalisation of the learned warm start
     float sd_ref;     // lambda denominator of an interior particle of the rest lattice
     float inv_nref;   // 1 / neighbour count of the rest lattice
@@ -70,6 +74,7 @@ enum Stat {
     ST_RUN_ITERS = 50,       //   corrections executed (uint)
     ST_RUN_STEPS = 51,       //   judged steps (uint)
     ST_RUN_FAIL = 52,        //   judged steps whose judged error exceeded tol (uint)
+    ST_RUN_BAD = 53,         //   1 once any pass since the reset had its fallback reach capped (uint)
     ST_N = 64
 };
 
@@ -287,7 +292,12 @@ __device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint
         // axis once L exceeds a cell, not the 5 of a cube of radius 2 around the cell)
         const uint64_t wave = __ballot(1);
         if (__lane_id() == (uint)__builtin_ctzll(wave)) atomicAdd(k.fallback, (uint)__builtin_popcountll(wave));
-        const float L = fminf((k.s.h + di + sqrtf(disp2)) * 1.0001f, kMaxStencil * k.g.cell);
+        const float Lwant = (k.s.h + di + sqrtf(disp2)) * 1.0001f;
+        const float L = fminf(Lwant, kMaxStencil * k.g.cell);
+        if (Lwant > L) {  // the capped walk may miss pairs: flag the step and the run (never silently)
+            atomicOr(k.bad, 1u);
+            atomicOr(k.bad + (ST_RUN_BAD - ST_BAD), 1u);
+        }
         const float3 c = f3(xb[i]);
         stencil_box(fg_pos(c - f3(L, L, L), k.g), fg_pos(c + f3(L, L, L), k.g), k.g, off, fetch, body);
     }
@@ -308,7 +318,6 @@ __global__ __launch_bounds__(kTpb) void k_predict(const float4* __restrict__ x, 
     bool valid;
     uint i = slot_of(k, valid);
     float disp2 = stats[ST_DISP2];
-    if (mode == 0 && disp2 > k.wide2 && threadIdx.x == 0) atomicOr((unsigned int*)&stats[ST_BAD], 1u);
     float err = 0.0f, aerr = 0.0f;
     if (valid) {
         float3 p = f3(x[i]), vi = f3(v[i]);
@@ -405,7 +414,7 @@ __device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, const 
                 if (r2 > 0.0f) {
                     float ri = rinv_of(r2);
                     float q = k.s.h - r2 * ri;
-                    g = (k.s.gwspiky * k.s.inv_rho0 * q * q * ri) * r;
+                    g = (k.gfac * q * q * ri) * r;
                     gs += g;
                     sd += dot3(g, g);
                 }
@@ -443,7 +452,8 @@ __device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, const 
     sd += sgs;
     float C = d * k.s.inv_rho0 - 1.0f;
     float lam = -C / (sd + k.eps);
-    xl[i] = make_float4(p.x, p.y, p.z, lam * k.lam0_scale);
+    const float mem = k.prev ? fminf(k.prev[i], 0.0f) : 0.0f;  // pressure part of the previous step's multiplier
+    xl[i] = make_float4(p.x, p.y, p.z, lam * k.lam0_scale + k.gamma * mem);
     xbuild[i] = p4;
     dens[i] = d;
     float err = fmaxf(C, 0.0f);
@@ -648,7 +658,7 @@ __global__ __launch_bounds__(kTpb) void k_lambda(const float4* __restrict__ xa, 
                     if (r2 > 0.0f) {
                         float ri = rinv_of(r2);
                         float q = k.s.h - r2 * ri;
-                        float3 g = (k.s.gwspiky * k.s.inv_rho0 * q * q * ri) * r;
+                        float3 g = (k.gfac * q * q * ri) * r;
                         gs += g;
                         sd += dot3(g, g);
                     }
@@ -702,7 +712,7 @@ __global__ __launch_bounds__(kTpb) void k_dp(const float4* __restrict__ xl, floa
                         sc = -k.s.ap_k * (k.s.ap_n == 4.0f ? pow4(ww) : powf(ww, k.s.ap_n)) * dt2;
                     }
                     float q = k.s.h - r2 * ri;
-                    acc += ((li + q4.w + sc) * (k.s.gwspiky * k.s.inv_rho0) * q * q * ri) * r;
+                    acc += ((li + q4.w + sc) * k.gfac * q * q * ri) * r;
                 }
             });
         float3 xn = clamp_wall(p + acc, k.s);
@@ -856,11 +866,12 @@ void rxPBDSPH_GPU::Finalize() {
     drop_graphs();
     for (void** p : {(void**)&m_x, (void**)&m_xs, (void**)&m_xl, (void**)&m_xb, (void**)&m_v, (void**)&m_t, (void**)&m_dens,
                      (void**)&m_lamacc, (void**)&m_id, (void**)&m_idt, (void**)&m_list, (void**)&m_cnt, (void**)&m_stats,
-                     (void**)&m_ftmp, (void**)&m_feat, (void**)&m_ftmp16, (void**)&m_therm, (void**)&m_therm_t})
+                     (void**)&m_ftmp, (void**)&m_feat, (void**)&m_ftmp16, (void**)&m_therm, (void**)&m_therm_t, (void**)&m_mlp_mem})
         if (*p) {
             RX_HIP(hipFree(*p));
             *p = nullptr;
         }
+    m_mlp_ready = false;
     m_grid.release();
     if (m_hrec) {
         RX_HIP(hipHostFree(m_hrec));
@@ -898,7 +909,7 @@ int rxPBDSPH_GPU::Initialize(const rxSphEnv& env) {
     m_grid.setup(m_params, env.h + m_skin);
     lattice_refs();
     if (!RX_HIP(hipStreamCreateWithFlags(&m_stream, hipStreamNonBlocking))) return 1;
-    if (!RX_HIP(hipHostMalloc((void**)&m_hrec, sizeof(float) * 4 * kRecs, hipHostMallocDefault))) return 1;
+    if (!RX_HIP(hipHostMalloc((void**)&m_hrec, sizeof(float) * kRecLen * kRecs, hipHostMallocDefault))) return 1;
     for (hipEvent_t& e : m_rec_ev)
         if (!RX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming))) return 1;
     return 0;
@@ -919,10 +930,17 @@ void rxPBDSPH_GPU::poll_records() {
             break;
         }
         if (!RX_HIP(e)) break;
-        const float* r = m_hrec + 4 * slot;  // ST_DISP2, ST_OVERFLOW, ST_ITERS, ST_FALLBACK
+        const float* r = m_hrec + kRecLen * slot;  // ST_DISP2, ST_OVERFLOW, ST_ITERS, ST_FALLBACK, ST_ERR_FINAL, ST_BAD
         const float D = sqrtf(fmaxf(r[0], 0.0f));
         const uint over = __builtin_bit_cast(uint, r[1]), fb = __builtin_bit_cast(uint, r[3]);
         ++m_rec_tail;
+        if (__builtin_bit_cast(uint, r[5])) {
+            // a pass of that step walked less than the reach it needed: start the next step from a fresh grid and
+            // let Update report it
+            ++m_bad_steps;
+            m_have_list = false;
+        }
+        if (!m_adapt) continue;
         const float need = 2.2f * D;
         int want = 0;
         while (want < 3 && m_skin_base * (float)(1 + want) < need) ++want;
@@ -948,7 +966,7 @@ void rxPBDSPH_GPU::poll_records() {
 void rxPBDSPH_GPU::push_record() {
     if (!m_hrec || m_rec_head - m_rec_tail >= kRecs) return;
     const int slot = (int)(m_rec_head % kRecs);
-    if (RX_HIP(hipMemcpyAsync(m_hrec + 4 * slot, m_stats + ST_DISP2, 4 * sizeof(float), hipMemcpyDeviceToHost, m_stream)) &&
+    if (RX_HIP(hipMemcpyAsync(m_hrec + kRecLen * slot, m_stats + ST_DISP2, kRecLen * sizeof(float), hipMemcpyDeviceToHost, m_stream)) &&
         RX_HIP(hipEventRecord(m_rec_ev[slot], m_stream)))
         ++m_rec_head;
 }
@@ -1015,6 +1033,7 @@ int rxPBDSPH_GPU::SetParticles(const float* pos4, const float* vel4, int n) {
     }
     drop_graphs();
     m_rec_tail = m_rec_head;  // records of the previous particles no longer apply
+    m_bad_steps = m_bad_reported = 0;
     m_level_target = 0;
     m_calm = 0;
     if ((uint)n != m_n) {
@@ -1048,6 +1067,7 @@ int rxPBDSPH_GPU::SetParticles(const float* pos4, const float* vel4, int n) {
 void rxPBDSPH_GPU::lattice_refs() {
     double h = m_env.h, l = m_env.spacing > 0.0f ? m_env.spacing : 0.5 * h;
     double gw = -45.0 / (3.14159265358979323846 * pow(h, 6.0)), r0 = m_params.Density;
+    const double gm = m_grad_mass ? (double)m_params.Mass : 1.0;
     int nmax = (int)ceil(h / l) + 1;
     double sd = 0.0;
     int cnt = 0;
@@ -1058,7 +1078,7 @@ void rxPBDSPH_GPU::lattice_refs() {
                 if (r > h) continue;
                 ++cnt;
                 if (r > 0.0) {
-                    double g = gw * (h - r) * (h - r) / r0;  // |grad W| / rho0
+                    double g = gw * (h - r) * (h - r) / r0 * gm;  // |grad W| / rho0 (times m in the mass mode)
                     sd += g * g;
                 }
             }
@@ -1081,8 +1101,10 @@ PbfK rxPBDSPH_GPU::consts(float dt) const {
     float wide = 0.5f * ((float)kMaxStencil * m_grid.k.cell - m_env.h);  // largest fallback reach minus h, per particle
     k.wide2 = wide * wide;
     k.fallback = (uint*)(m_stats + ST_FALLBACK);
+    k.bad = (uint*)(m_stats + ST_BAD);
     k.maxn = m_maxn;
     k.eps = m_env.epsilon;
+    k.gfac = k.s.gwspiky * k.s.inv_rho0 * (m_grad_mass ? k.s.mass : 1.0f);
     k.dt = dt;
     k.inv_dt = 1.0f / dt;
     k.xsph_c = m_env.xsph_c;
@@ -1146,10 +1168,17 @@ int rxPBDSPH_GPU::Update(float dt, int step) {
         return 1;
     }
     int rc = 0;
+    poll_records();
+    if (m_bad_steps > m_bad_reported) {
+        m_bad_reported = m_bad_steps;
+        rx_set_error(3, "rxPBDSPH_GPU::Update: an earlier step moved particles beyond the %d-cell reach of the "
+                        "neighbour walk, so that step missed pairs; the neighbour list is rebuilt from this step on",
+                     kMaxStencil);
+        return 3;
+    }
     if (!m_use_graph || m_profile || !m_have_list) {
-        if (m_adapt) poll_records();
         rc = enqueue_step(dt);  // the first step (no list yet) and profiled steps run eagerly
-        if (m_adapt && rc == 0) push_record();
+        if (rc == 0) push_record();
     } else {
         m_level_target = m_level;  // replayed and captured steps keep the sizes they were recorded with
         m_maxn_target = m_maxn;
@@ -1162,6 +1191,7 @@ int rxPBDSPH_GPU::Update(float dt, int step) {
             load_ptrs(hit->after);
             m_have_list = true;
             ++m_step;
+            push_record();
         } else {
             // capture this step's launches, then run it from the graph
             if (m_graphs.size() >= 12) drop_graphs();
@@ -1183,6 +1213,7 @@ int rxPBDSPH_GPU::Update(float dt, int step) {
                 m_graphs.push_back(StepGraph{key, save_ptrs(), exec});
                 if (!RX_HIP(hipGraphLaunch(exec, m_stream))) return 1;
             }
+            push_record();
         }
     }
     if (rc) return rc;
@@ -1246,8 +1277,12 @@ int rxPBDSPH_GPU::enqueue_step(float dt) {
         fg_gather<float4>(m_therm_t, m_therm, m_grid.perm, n, s);
         std::swap(m_therm, m_therm_t);
     }
+    const bool memory = m_gamma > 0.0f;
+    if (memory) fg_gather<float>(m_ftmp, m_lamacc, m_grid.perm, n, s);  // the previous step's total multiplier
     rx_fill<float>(m_lamacc, 0.0f, n, s);
     end_mark(mk);
+    k.prev = memory && m_step > 0 ? m_ftmp : nullptr;
+    k.gamma = m_gamma;
 
     // ---- list + lambda_0 (the network's multiplier in learned mode), then up to K corrections
     const bool feats = (m_mode == RX_PBF_MODE_LEARNED) || m_record_features;
@@ -1417,7 +1452,7 @@ int rxPBDSPH_GPU::GetStats(float* out, int count) {
     h[ST_MEAN_ABS_START] = h[ST_MEAN_ABS_START] / (float)m_n;
     h[ST_MPOS_START] = h[ST_MPOS_START] / (float)m_n;
     for (int t = 0; t < 16; ++t) h[ST_MPOS_ITER + t] = h[ST_MPOS_ITER + t] / (float)m_n;
-    for (int t : {ST_RUN_ITERS, ST_RUN_STEPS, ST_RUN_FAIL}) h[t] = (float)__builtin_bit_cast(uint, h[t]);
+    for (int t : {ST_RUN_ITERS, ST_RUN_STEPS, ST_RUN_FAIL, ST_RUN_BAD}) h[t] = (float)__builtin_bit_cast(uint, h[t]);
     h[ST_DISP2] = sqrtf(h[ST_DISP2]);
     h[60] = m_skin / m_env.h;  // host state: the list sizes in use
     h[61] = (float)m_maxn;
@@ -1427,11 +1462,23 @@ int rxPBDSPH_GPU::GetStats(float* out, int count) {
 
 int rxPBDSPH_GPU::Sync() { return RX_HIP(hipStreamSynchronize(m_stream)) ? 0 : 1; }
 
+// The solver keeps its own copy of the network's inference weights (made on the solver's stream once the
+// network's pending work is done), so the network may be changed or destroyed afterwards.
 int rxPBDSPH_GPU::SetNetwork(void* mlp) {
     drop_graphs();
-    if (!rx_mlp_view(mlp, &m_mlp)) {
+    m_mlp_ready = false;
+    if (!mlp) {
         rx_set_error(1, "rxPBDSPH_GPU::SetNetwork: null network");
-        m_mlp_ready = false;
+        return 1;
+    }
+    if (!m_stream) {
+        rx_set_error(1, "rxPBDSPH_GPU::SetNetwork: solver not initialised");
+        return 1;
+    }
+    if (!m_mlp_mem && !RX_HIP(hipMalloc(&m_mlp_mem, rx_mlp_copy_bytes()))) return 1;
+    if (!RX_HIP(hipStreamSynchronize(m_stream))) return 1;  // no pending inference reads the old copy
+    if (!rx_mlp_copy(mlp, m_mlp_mem, m_stream, &m_mlp)) {
+        rx_set_error(1, "rxPBDSPH_GPU::SetNetwork: copying the network failed");
         return 1;
     }
     m_mlp_ready = true;
@@ -1504,8 +1551,10 @@ int rxPBDSPH_GPU_SetParticles(void* h, const float* pos4, const float* vel4, int
 }
 int rxPBDSPH_GPU_Update(void* h, float dt, int steps) {
     if (!h) return 1;
-    for (int i = 0; i < steps; ++i)
-        if (((rxPBDSPH_GPU*)h)->Update(dt, i) != 0) return 1;
+    for (int i = 0; i < steps; ++i) {
+        const int rc = ((rxPBDSPH_GPU*)h)->Update(dt, i);
+        if (rc != 0) return rc;
+    }
     return 0;
 }
 int rxPBDSPH_GPU_GetArray(void* h, int type, float* host, int count) { return h ? ((rxPBDSPH_GPU*)h)->GetArray(type, host, count) : 1; }
@@ -1587,6 +1636,18 @@ int rxPBDSPH_GPU::SetOption(int opt, float value) {
         if (!m_adapt) m_level_target = 0;  // back to the set skin at the next step
         return 0;
     case RX_PBF_OPT_HEAT_DT: m_heat_dt = value; return 0;
+    case RX_PBF_OPT_MEMORY:
+        if (value < 0.0f || value > 1.0f) {
+            rx_set_error(1, "rxPBDSPH_GPU::SetOption: memory share must be in [0, 1]");
+            return 1;
+        }
+        m_gamma = value;
+        return 0;
+    case RX_PBF_OPT_GRAD_MASS:
+        m_grad_mass = value != 0.0f;
+        lattice_refs();  // the feature scales follow the gradient
+        drop_graphs();
+        return 0;
     case RX_PBF_OPT_LIST_TILE:
         if (value < 0.0f || value > 4.0f) {
             rx_set_error(1, "rxPBDSPH_GPU::SetOption: list tile variant must be 0..4");

@@ -719,6 +719,7 @@ public:
     }
     void drop_graph() {
         if (m_graph) {
+            if (m_stream) RX_HIP(hipStreamSynchronize(m_stream));  // no launch of the graph is still pending
             RX_HIP(hipGraphExecDestroy(m_graph));
             m_graph = nullptr;
         }

@@ -289,6 +289,7 @@ public:
 private:
     void drop_graph() {
         if (m_graph_exec) {
+            if (m_stream) RX_HIP(hipStreamSynchronize(m_stream));  // no launch of the graph is still pending
             RX_HIP(hipGraphExecDestroy(m_graph_exec));
             m_graph_exec = nullptr;
         }

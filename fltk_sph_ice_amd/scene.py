@@ -42,6 +42,7 @@ class Scene:
     ap_n: float = 4.0
     ap_q: float = 0.2
     xsph_c: float = 0.0
+    grad_mass: int = 0       # PBF: 1 = constraint gradients carry the mass (RX_PBF_OPT_GRAD_MASS)
     mass: float = 0.0        # 0: solved from the lattice sum at `spacing` so that the lattice has rest density
     extra: dict = field(default_factory=dict)
 

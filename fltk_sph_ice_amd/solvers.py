@@ -22,6 +22,8 @@ class PBFSolver:
         if not self.h:
             check(self.lib)
             raise RuntimeError("rxPBDSPH_GPU_Create failed")
+        if getattr(scene, "grad_mass", 0):
+            self.option(_abi.RX_PBF_OPT_GRAD_MASS, 1.0)
         if pos4 is None:
             pos4, vel4 = scene.particles()
         self.set_particles(pos4, vel4)

@@ -116,7 +116,7 @@ enum rxPbfOption {
     RX_PBF_OPT_GRAPH = 17,         /* 1: after the first step, each step's launches are recorded once into a
                                       hipGraph per buffer configuration and replayed (for launch-bound sizes);
                                       profiled steps (RX_PBF_OPT_PROFILE) run eagerly.  0 (default): eager */
-    RX_PBF_OPT_ADAPT_SKIN = 18     /* 1 (default): size the neighbour lists from the steps already completed:
+    RX_PBF_OPT_ADAPT_SKIN = 18,    /* 1 (default): size the neighbour lists from the steps already completed:
                                       the skin grows to 2, 3 or 4 times RX_PBF_OPT_SKIN (to 2.2 times the
                                       largest in-step displacement) when more than 3% of a step's neighbour
                                       passes walked the grid instead of their lists, and shrinks back after 2
@@ -124,6 +124,14 @@ enum rxPbfOption {
                                       lists overflow.  The host reads each step's figures once the device has
                                       finished it, without waiting, so the sizes follow with a lag.  Eager steps
                                       only (replayed graphs keep their sizes). */
+    RX_PBF_OPT_GRAD_MASS = 19,     /* 1: the constraint gradients carry the particle mass, grad C_i =
+                                      (m / rho0) sum_j grad W_ij, in the multiplier's denominator and in the
+                                      position correction (Macklin & Mueller 2013, section 3, with rho = sum m W).
+                                      0 (default): the reference's grad W / rho0 (rx_sph_pbd.cpp:782-786,
+                                      rx_sph_pbd.cpp:970-973), which scales each correction by m */
+    RX_PBF_OPT_MEMORY = 20         /* gamma in [0, 1] (default 0): the first correction of a step adds gamma times
+                                      the pressure part, min(L, 0), of the multiplier L that the particle's
+                                      previous step applied in total (a leaky warm start from the last step) */
 };
 
 /* Kernel classes of rxPBDSPH_GPU_GetProfile. */
@@ -147,8 +155,9 @@ enum rxProfileClass {
  *   18      particles whose neighbour list overflowed
  *   19      corrections executed in the step
  *   20      neighbour passes of particles that walked the grid instead of their list
- *   22      1 when a displacement since the list build exceeded what the capped fallback walk covers
- *           (4 cells: the passes may then miss pairs)
+ *   22      1 when a pass of the step needed a fallback walk beyond its 4-cell cap (the pass then missed
+ *           pairs).  Update returns 3 once the host has seen such a completed step, and the next step starts
+ *           from a fresh grid (the list is rebuilt)
  *   23      mean |rho - rho0| / rho0 at the step's start positions
  *   24      mean (rho - rho0)+ / rho0 at the step's start positions
  *   32 + k  the same at the start of Jacobi iteration k (k < 16)
@@ -156,6 +165,7 @@ enum rxProfileClass {
  * judged by the prediction pass of the step after it):
  *   48      max of slot 0        49  max of slot 24      50  corrections executed
  *   51      steps judged         52  steps whose judged error exceeded TOL
+ *   53      1 once any pass since the reset needed a walk beyond the cap (slot 22 of any step)
  * Host state at the call:
  *   60      list skin in use, as a fraction of h      61  list capacity in use                     */
 
@@ -172,7 +182,9 @@ int rxPBDSPH_GPU_Sync(void* solver);
 int rxPBDSPH_GPU_SetOption(void* solver, int option, float value);
 /* Milliseconds and launch counts per rxProfileClass since the last call (n entries each). */
 int rxPBDSPH_GPU_GetProfile(void* solver, float* ms, int* count, int n);
-/* Network of the learned warm start (an rxMLP object, see rx_mlp.h); its weights are copied. */
+/* Network of the learned warm start (an rxMLP object, see rx_mlp.h).  The solver copies the network's inference
+ * weights, biases, standardisation and output scaling at this call (after the network's pending work); later
+ * changes to the network, or destroying it, do not affect the solver until SetNetwork is called again. */
 int rxPBDSPH_GPU_SetNetwork(void* solver, void* mlp);
 /* Thermal state (host arrays in original order; heats and phase may be NULL; temps NULL removes it): after
  * every step the temperatures diffuse over the neighbours within h at the final positions

@@ -21,7 +21,7 @@ class OrcParams(C.Structure):
         ("bmin", C.c_double * 3), ("bmax", C.c_double * 3), ("origin", C.c_double * 3), ("cell", C.c_double * 3),
         ("gsize", C.c_int * 3), ("eps", C.c_double), ("use_ap", C.c_int), ("ap_k", C.c_double),
         ("ap_n", C.c_double), ("ap_q", C.c_double), ("xsph_c", C.c_double), ("iterations", C.c_int),
-        ("dt", C.c_double),
+        ("dt", C.c_double), ("grad_mass", C.c_int),
     ]
 
 
@@ -70,6 +70,11 @@ _SIGS = {
     "orc_pbf_position_correction_f": (None, [P, P, I, P, F, P]),
     "orc_pbf_xsph_f": (None, [P, P, P, I, P, F, P]),
     "orc_pbf_step_f": (None, [P, P, I, P, P, P, P]),
+    "orc_sph_force_scale_f": (None, [P, P, P, P, I, P, P]),
+    "orc_pbf_ext_force_scale_f": (None, [P, P, P, I, P, P]),
+    "orc_boundary_force_scale_f": (None, [P, I, P, P, I, P, P, P, P]),
+    "orc_pbf_lambda_scale_f": (None, [P, I, P, P, I, P, P, P]),
+    "orc_pbf_dp_scale_f": (None, [P, P, I, P, P, P, I, P, F, P]),
     "orc_pbf_step_d": (None, [P, P, I, P, P, P]),
     "orc_jacobi_eigen3": (I, [P, P, D, I]),
     "orc_polar_decomposition": (None, [P, P, P]),
@@ -141,6 +146,7 @@ def params_from(sp, scene=None) -> OrcParams:
         o.eps = scene.epsilon
         o.xsph_c = scene.xsph_c
         o.iterations = scene.iterations
+        o.grad_mass = int(getattr(scene, "grad_mass", 0))
     return o
 
 
@@ -255,3 +261,61 @@ def aniso_pipeline(spos4, sidx, cs, ce, geo, gsize, lam: float, h: float, h_eff:
                                    ptr(r["gmat"]), nx, ny, nz, *map(float, lo), *map(float, d), ptr(out))
         r["grid"] = out
     return r
+
+
+# ---------------------------------------------------------------------------------------------- per-particle scales
+# The magnitudes FP32 results are judged against: for a sum of terms that nearly cancel (pressure forces in a
+# uniform lattice, the position correction) the rounding of any evaluation order is bounded by a small multiple
+# of sum_j |term_ij|, not by |sum|.
+def _nb_args(b4, vol, lam_b):
+    if b4 is None:
+        return None, None, None, 0
+    return ptr(b4), ptr(vol), (ptr(lam_b) if lam_b is not None else None), len(b4)
+
+
+def sph_force_scale(pos4, vel4, dens, pres, P: OrcParams) -> np.ndarray:
+    n = pos4.shape[0]
+    out = np.zeros(n, np.float64)
+    load().orc_sph_force_scale_f(ptr(pos4), ptr(vel4), ptr(dens), ptr(pres), n, C.byref(P), ptr(out))
+    return out
+
+
+def pbf_ext_force_scale(pos4, vel4, dens, P: OrcParams) -> np.ndarray:
+    n = pos4.shape[0]
+    out = np.zeros(n, np.float64)
+    load().orc_pbf_ext_force_scale_f(ptr(pos4), ptr(vel4), ptr(dens), n, C.byref(P), ptr(out))
+    return out
+
+
+def boundary_force_scale(pos4, b4, vol, dens, pres, P: OrcParams, base=None) -> np.ndarray:
+    """sum_b |boundary term| added to `base` (the fluid scale) or to zeros."""
+    n = pos4.shape[0]
+    out = np.zeros(n, np.float64) if base is None else np.array(base, np.float64)
+    load().orc_boundary_force_scale_f(ptr(pos4), n, ptr(b4), ptr(vol), len(b4), C.byref(P), ptr(dens), ptr(pres), ptr(out))
+    return out
+
+
+def pbf_lambda_scale(pos4, P: OrcParams, b4=None, vol=None):
+    """(rho_i / rho0 + |C_i|) / (sd_i + eps) per particle (and per boundary particle when b4 is given)."""
+    n = pos4.shape[0]
+    out = np.zeros(n, np.float64)
+    outb = np.zeros(len(b4) if b4 is not None else 1, np.float64)
+    pb, pv, _, nb = _nb_args(b4, vol, None)
+    load().orc_pbf_lambda_scale_f(ptr(pos4), n, pb, pv, nb, C.byref(P), ptr(out), ptr(outb))
+    return (out, outb) if b4 is not None else out
+
+
+def pbf_dp_scale(pos4, lam, P: OrcParams, dt, b4=None, vol=None, lam_b=None) -> np.ndarray:
+    n = pos4.shape[0]
+    out = np.zeros(n, np.float64)
+    pb, pv, pl, nb = _nb_args(b4, vol, lam_b)
+    load().orc_pbf_dp_scale_f(ptr(pos4), ptr(lam), n, pb, pv, pl, nb, C.byref(P), np.float32(dt), ptr(out))
+    return out
+
+
+def rel_err_vs_scale(got, ref, scale) -> float:
+    """max_i |got_i - ref_i| / scale_i (vectors: the Euclidean norm of the difference)."""
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    d = np.abs(got - ref) if got.ndim == 1 else np.linalg.norm(got - ref, axis=1)
+    return float((d / np.maximum(scale, 1e-300)).max())

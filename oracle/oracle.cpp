@@ -308,8 +308,10 @@ template <class T> void sph_density(const T* p4, int n, const std::vector<std::v
 }
 
 // rxSPH::calForce (rx_sph_cpu.cpp:558-656); result is an acceleration.
+// scale (may be null): per particle sum_j |pressure term_ij| + |viscosity term_ij| + |g|, the magnitude the
+// summed acceleration is judged against (its terms nearly cancel in a uniform lattice).
 template <class T> void sph_force(const T* p4, const T* v4, const T* dens, T* pres, int n,
-                                  const std::vector<std::vector<Neigh>>& nb, const OrcParams& P, T* acc) {
+                                  const std::vector<std::vector<Neigh>>& nb, const OrcParams& P, T* acc, double* scale = nullptr) {
     double h = R<T>(P.h), m = R<T>(P.mass), ag = R<T>(coef(1, h, 2)), al = R<T>(coef(2, h, 3));
     double r0 = R<T>(P.rest_dens), mu = R<T>(P.viscosity), k = R<T>(P.gas_k);
     for (int i = 0; i < n; ++i) pres[i] = (T)(k * ((double)dens[i] - r0));
@@ -317,6 +319,7 @@ template <class T> void sph_force(const T* p4, const T* v4, const T* dens, T* pr
         V3 pos0 = ld(p4, i), vel0 = ld(v4, i);
         double prsi = R<T>((double)pres[i] / ((double)dens[i] * (double)dens[i]));
         V3 Fp{0, 0, 0}, Fev{0, 0, 0};
+        double sc = 0.0;
         for (const Neigh& e : nb[i]) {
             int j = e.idx;
             if (j == i) continue;
@@ -324,11 +327,16 @@ template <class T> void sph_force(const T* p4, const T* v4, const T* dens, T* pr
             V3 vji = ld(v4, j) - vel0;
             double r = norm(rij);
             double prsj = R<T>((double)pres[j] / ((double)dens[j] * (double)dens[j]));
-            Fp = Fp + (m * (prsi + prsj) * GW_spiky_s(r, h, ag)) * rij;
-            Fev = Fev + (m / (double)dens[j] * LW_visc(r, h, al)) * vji;
+            V3 tp = (m * (prsi + prsj) * GW_spiky_s(r, h, ag)) * rij;
+            V3 tv = (m / (double)dens[j] * LW_visc(r, h, al)) * vji;
+            Fp = Fp + tp;
+            Fev = Fev + tv;
+            sc += norm(tp) + std::fabs(mu) * norm(tv);
         }
         V3 f = (-1.0) * Fp + mu * Fev;
-        f = f + V3{P.gravity[0], P.gravity[1], P.gravity[2]};
+        V3 g{P.gravity[0], P.gravity[1], P.gravity[2]};
+        f = f + g;
+        if (scale) scale[i] = sc + norm(g);
         st(acc, i, f);
         acc[4 * i + 3] = 0;
     }
@@ -344,21 +352,28 @@ template <class T> void sph_step(T* pos, T* vel, int n, const OrcParams& P, doub
 
 // ------------------------------------------------------------------------------------------ PBF
 // rxPBDSPH::calForceExtAndVisc (rx_sph_pbd.cpp:651-689): viscosity divided by rho_i, plus gravity.
+// scale (may be null): per particle mu sum_j |term_ij| + |g|.
 template <class T> void pbf_ext_force(const T* p4, const T* v4, const T* dens, int n,
-                                      const std::vector<std::vector<Neigh>>& nb, const OrcParams& P, T* acc) {
+                                      const std::vector<std::vector<Neigh>>& nb, const OrcParams& P, T* acc,
+                                      double* scale = nullptr) {
     double h = R<T>(P.h), m = R<T>(P.mass), al = R<T>(coef(2, h, 3)), mu = R<T>(P.viscosity);
     for (int i = 0; i < n; ++i) {
         V3 pos0 = ld(p4, i), vel0 = ld(v4, i);
         V3 Fev{0, 0, 0};
+        double sc = 0.0;
         for (const Neigh& e : nb[i]) {
             int j = e.idx;
             if (j == i) continue;
             V3 rij = pos0 - ld(p4, j);
             V3 vji = ld(v4, j) - vel0;
             double r = norm(rij);
-            Fev = Fev + (m / (double)dens[i] * LW_visc(r, h, al)) * vji;
+            V3 t = (m / (double)dens[i] * LW_visc(r, h, al)) * vji;
+            Fev = Fev + t;
+            sc += norm(t);
         }
-        V3 f = mu * Fev + V3{P.gravity[0], P.gravity[1], P.gravity[2]};
+        V3 g{P.gravity[0], P.gravity[1], P.gravity[2]};
+        V3 f = mu * Fev + g;
+        if (scale) scale[i] = std::fabs(mu) * sc + norm(g);
         st(acc, i, f);
         acc[4 * i + 3] = 0;
     }
@@ -409,8 +424,9 @@ template <class T> T boundary_density(V3 pos0, const Boundary<T>& B, const OrcPa
 // Boundary pressure term as the GPU kernel computes it (rx_sph_kernel.cu:662-727): an acceleration,
 // sum_b -rho0 V_b (p_i / rho_i^2) grad W_spiky for 1e-4 < r <= h.  (The CPU loop, rx_sph_cpu.cpp:617-651,
 // divides this sum by the mass.)
+// scale (may be null): sum_b |term_ib| is added to scale[i].
 template <class T> void boundary_force(const T* p4, int n, const Boundary<T>& B, const OrcParams& P, const T* dens, const T* pres,
-                                       T* acc) {
+                                       T* acc, double* scale = nullptr) {
     double h = R<T>(P.h), ag = R<T>(coef(1, h, 2)), r0 = R<T>(P.rest_dens);
     std::vector<int> nbr;
     for (int i = 0; i < n; ++i) {
@@ -422,7 +438,9 @@ template <class T> void boundary_force(const T* p4, int n, const Boundary<T>& B,
             V3 rij = pos0 - ld(B.b4, j);
             double r = norm(rij);
             if (r <= 0.0001) continue;
-            bp = bp + (-r0 * (double)B.vol[j] * prsi * GW_spiky_s(r, h, ag)) * rij;
+            V3 t = (-r0 * (double)B.vol[j] * prsi * GW_spiky_s(r, h, ag)) * rij;
+            bp = bp + t;
+            if (scale) scale[i] += norm(t);
         }
         for (int k = 0; k < 3; ++k) acc[4 * i + k] = (T)((double)acc[4 * i + k] + (k == 0 ? bp.x : k == 1 ? bp.y : bp.z));
     }
@@ -431,10 +449,15 @@ template <class T> void boundary_force(const T* p4, int n, const Boundary<T>& B,
 // rxPBDSPH::calScalingFactor (rx_sph_pbd.cpp:699-822), fluid-only branch.
 // With boundary particles (RX_USE_BOUNDARY_PARTICLE) the density and the denominator gain the boundary terms
 // (rx_sph_pbd.cpp:729-748, 790-812).
+// scale (may be null): per particle (rho_i / rho0 + |C_i|) / (sd_i + eps), the magnitude that bounds the rounding
+// of lambda (C = rho/rho0 - 1 cancels).
+// P.grad_mass != 0: every constraint gradient carries the particle mass, grad C_i = (m / rho0) sum grad W
+// (Macklin & Mueller 2013, section 3, with rho_i = sum m W); the reference's gradient has no mass factor.
 template <class T> void pbf_lambda(const T* p4, int n, const std::vector<std::vector<Neigh>>& nb, const OrcParams& P,
-                                   T* dens, T* lambda, const Boundary<T>* B = nullptr) {
+                                   T* dens, T* lambda, const Boundary<T>* B = nullptr, double* scale = nullptr) {
     double h = R<T>(P.h), m = R<T>(P.mass), aw = R<T>(coef(0, h, 1)), ag = R<T>(coef(1, h, 2));
     double r0 = R<T>(P.rest_dens), eps = R<T>(P.eps);
+    const double gm = P.grad_mass ? m : 1.0;
     std::vector<int> bn;
     for (int i = 0; i < n; ++i) {
         V3 pos0 = ld(p4, i);
@@ -454,12 +477,12 @@ template <class T> void pbf_lambda(const T* p4, int n, const std::vector<std::ve
                 for (const Neigh& f : nb[i]) {
                     V3 rij = pos0 - ld(p4, f.idx);
                     double ri = norm(rij);
-                    dp = dp + (GW_spiky_s(ri, h, ag) / r0) * rij;
+                    dp = dp + (gm * GW_spiky_s(ri, h, ag) / r0) * rij;
                 }
             } else {
                 V3 rik = pos0 - ld(p4, k);
                 double r = norm(rik);
-                dp = (-GW_spiky_s(r, h, ag) / r0) * rik;
+                dp = (-gm * GW_spiky_s(r, h, ag) / r0) * rik;
             }
             sd = (T)((double)sd + dot(dp, dp));
         }
@@ -467,19 +490,22 @@ template <class T> void pbf_lambda(const T* p4, int n, const std::vector<std::ve
             for (int j : bn) {
                 V3 rij = pos0 - ld(B->b4, j);
                 double r = norm(rij);
-                V3 dpb = ((r0 * (double)B->vol[j] / m) * GW_spiky_s(r, h, ag) / r0) * rij;
+                V3 dpb = (gm * (r0 * (double)B->vol[j] / m) * GW_spiky_s(r, h, ag) / r0) * rij;
                 sd = (T)((double)sd + dot(dpb, dpb));
             }
         }
         lambda[i] = (T)(-(double)C / ((double)sd + eps));
+        if (scale) scale[i] = ((double)d / r0 + std::fabs((double)C)) / ((double)sd + eps);
     }
 }
 
 // rxPBDSPH::calScalingFactor, boundary-particle loop (rx_sph_pbd.cpp:825-906): lambda_b from the fluid and
 // boundary particles within h of b.
-template <class T> void pbf_boundary_lambda(const T* p4, int n, const Boundary<T>& B, const OrcParams& P, T* lambda_b) {
+template <class T> void pbf_boundary_lambda(const T* p4, int n, const Boundary<T>& B, const OrcParams& P, T* lambda_b,
+                                            double* scale = nullptr) {
     double h = R<T>(P.h), m = R<T>(P.mass), aw = R<T>(coef(0, h, 1)), ag = R<T>(coef(1, h, 2));
     double r0 = R<T>(P.rest_dens), eps = R<T>(P.eps), h2 = R<T>(h * h);
+    const double gm = P.grad_mass ? m : 1.0;
     std::vector<int> fn, bn;
     for (int i = 0; i < B.nb; ++i) {
         V3 pos0 = ld(B.b4, i);
@@ -496,15 +522,16 @@ template <class T> void pbf_boundary_lambda(const T* p4, int n, const Boundary<T
         T sd = 0;
         for (int k : fn) {
             V3 rik = pos0 - ld(p4, k);
-            V3 dp = (GW_spiky_s(norm(rik), h, ag) / r0) * rik;
+            V3 dp = (gm * GW_spiky_s(norm(rik), h, ag) / r0) * rik;
             sd = (T)((double)sd + dot(dp, dp));
         }
         for (int j : bn) {
             V3 rij = pos0 - ld(B.b4, j);
-            V3 dpb = ((r0 * (double)B.vol[j] / m) * GW_spiky_s(norm(rij), h, ag) / r0) * rij;
+            V3 dpb = (gm * (r0 * (double)B.vol[j] / m) * GW_spiky_s(norm(rij), h, ag) / r0) * rij;
             sd = (T)((double)sd + dot(dpb, dpb));
         }
         lambda_b[i] = (T)(-(double)C / ((double)sd + eps));
+        if (scale) scale[i] = ((double)brho / r0 + std::fabs((double)C)) / ((double)sd + eps);
     }
 }
 
@@ -512,15 +539,19 @@ template <class T> void pbf_boundary_lambda(const T* p4, int n, const Boundary<T
 // the reference (rx_sph_pbd.cpp:970).
 // With boundary particles: plus sum_b (lambda_i + lambda_b + s_corr) grad W / rho0, s_corr's W scaled by
 // rho0 V_b / m (rx_sph_pbd.cpp:978-1018).
+// scale (may be null): per particle sum_j |term_ij| (boundary terms included).
 template <class T> void pbf_dp(const T* p4, const T* lambda, int n, const std::vector<std::vector<Neigh>>& nb,
-                               const OrcParams& P, double dt, T* dp4, const Boundary<T>* B = nullptr, const T* lambda_b = nullptr) {
+                               const OrcParams& P, double dt, T* dp4, const Boundary<T>* B = nullptr, const T* lambda_b = nullptr,
+                               double* scale = nullptr) {
     double h = R<T>(P.h), aw = R<T>(coef(0, h, 1)), ag = R<T>(coef(1, h, 2)), r0 = R<T>(P.rest_dens);
+    const double gm = P.grad_mass ? R<T>(P.mass) : 1.0;
     double k = R<T>(P.ap_k), nn = R<T>(P.ap_n), dq = R<T>(P.ap_q * h);
     double wq = R<T>(W_poly6(dq, h, aw));
     std::vector<int> bn;
     for (int i = 0; i < n; ++i) {
         V3 pos0 = ld(p4, i);
         V3 dpij{0, 0, 0};
+        double sc = 0.0;
         for (const Neigh& e : nb[i]) {
             int j = e.idx;
             V3 rij = pos0 - ld(p4, j);
@@ -531,7 +562,9 @@ template <class T> void pbf_dp(const T* p4, const T* lambda, int n, const std::v
                 double ww = R<T>(W_poly6(r, h, aw) / wq);
                 scorr = R<T>(-k * std::pow(ww, nn) * dt * dt);
             }
-            dpij = dpij + (((double)lambda[i] + (double)lambda[j] + scorr) * GW_spiky_s(r, h, ag) / r0) * rij;
+            V3 t = (gm * ((double)lambda[i] + (double)lambda[j] + scorr) * GW_spiky_s(r, h, ag) / r0) * rij;
+            dpij = dpij + t;
+            sc += norm(t);
         }
         if (B) {
             double m = R<T>(P.mass);
@@ -546,10 +579,13 @@ template <class T> void pbf_dp(const T* p4, const T* lambda, int n, const std::v
                     double ww = R<T>((r0 * (double)B->vol[j] / m) * W_poly6(r, h, aw) / wq);
                     scorr = R<T>(-k * std::pow(ww, nn) * dt * dt);
                 }
-                dpb = dpb + (((double)lambda[i] + (double)lambda_b[j] + scorr) * GW_spiky_s(r, h, ag) / r0) * rij;
+                V3 t = (gm * ((double)lambda[i] + (double)lambda_b[j] + scorr) * GW_spiky_s(r, h, ag) / r0) * rij;
+                dpb = dpb + t;
+                sc += norm(t);
             }
             dpij = dpij + dpb;
         }
+        if (scale) scale[i] = sc;
         st(dp4, i, dpij);
         dp4[4 * i + 3] = 0;
     }
@@ -1036,6 +1072,43 @@ void orc_grid_density_f(const float* pos4, int n, const OrcParams* P, int nx, in
                 }
                 out[(size_t)i + (size_t)j * nx + (size_t)k * nx * ny] = d;
             }
+}
+
+// Per-particle magnitudes the FP32 results are judged against (sum over the terms of |term|; see the templates).
+void orc_sph_force_scale_f(const float* pos4, const float* vel4, const float* dens, const float* pres, int n, const OrcParams* P,
+                           double* scale) {
+    std::vector<std::vector<Neigh>> nb;
+    neighbour_lists(pos4, n, *P, nb);
+    std::vector<float> pr(pres, pres + n), acc(4 * (size_t)n);
+    sph_force(pos4, vel4, dens, pr.data(), n, nb, *P, acc.data(), scale);
+}
+void orc_pbf_ext_force_scale_f(const float* pos4, const float* vel4, const float* dens, int n, const OrcParams* P, double* scale) {
+    std::vector<std::vector<Neigh>> nb;
+    neighbour_lists(pos4, n, *P, nb);
+    std::vector<float> acc(4 * (size_t)n);
+    pbf_ext_force(pos4, vel4, dens, n, nb, *P, acc.data(), scale);
+}
+void orc_boundary_force_scale_f(const float* pos4, int n, const float* b4, const float* vol, int nb, const OrcParams* P,
+                                const float* dens, const float* pres, double* scale) {
+    std::vector<float> acc(4 * (size_t)n);
+    boundary_force(pos4, n, Boundary<float>{b4, vol, nb}, *P, dens, pres, acc.data(), scale);
+}
+void orc_pbf_lambda_scale_f(const float* pos4, int n, const float* b4, const float* vol, int nb, const OrcParams* P, double* scale,
+                            double* scale_b) {
+    std::vector<std::vector<Neigh>> nbl;
+    neighbour_lists(pos4, n, *P, nbl);
+    std::vector<float> d(n), l(n), lb(nb > 0 ? nb : 1);
+    Boundary<float> B{b4, vol, nb};
+    pbf_lambda(pos4, n, nbl, *P, d.data(), l.data(), nb > 0 ? &B : nullptr, scale);
+    if (nb > 0 && scale_b) pbf_boundary_lambda(pos4, n, B, *P, lb.data(), scale_b);
+}
+void orc_pbf_dp_scale_f(const float* pos4, const float* lambda, int n, const float* b4, const float* vol, const float* lambda_b, int nb,
+                        const OrcParams* P, float dt, double* scale) {
+    std::vector<std::vector<Neigh>> nbl;
+    neighbour_lists(pos4, n, *P, nbl);
+    std::vector<float> dp(4 * (size_t)n);
+    Boundary<float> B{b4, vol, nb};
+    pbf_dp(pos4, lambda, n, nbl, *P, (double)dt, dp.data(), nb > 0 ? &B : nullptr, lambda_b, scale);
 }
 
 void orc_pbf_step_f(float* pos4, float* vel4, int n, const OrcParams* P, float* dens, float* lambda_sum, OrcPbfStats* st) {

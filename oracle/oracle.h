@@ -40,6 +40,8 @@ typedef struct OrcParams {
     double xsph_c;        /* XSPH coefficient (0 = off)                           */
     int iterations;       /* Jacobi iterations per step                           */
     double dt;
+    int grad_mass;        /* PBF: 1 = constraint gradients carry the mass (m / rho0) sum grad W;
+                             0 = the reference's grad W / rho0                    */
 } OrcParams;
 
 /* Per-step PBF statistics. */
@@ -97,6 +99,18 @@ void orc_pbf_scaling_factor_wb_f(const float* pos4, int n, const float* b4, cons
                                  float* dens, float* lambda, float* lambda_b);
 void orc_pbf_position_correction_wb_f(const float* pos4, const float* lambda, int n, const float* b4, const float* vol,
                                       const float* lambda_b, int nb, const OrcParams* P, float dt, float* dp4);
+/* Per-particle magnitudes (double) the FP32 results are judged against: the sum over the summed terms of |term|
+ * (forces, the position correction), plus |g| where gravity is added; for lambda (rho_i / rho0 + |C_i|) / (sd_i + eps).
+ * Boundary arguments may be 0 / NULL (fluid only); orc_boundary_force_scale_f adds the boundary terms to scale. */
+void orc_sph_force_scale_f(const float* pos4, const float* vel4, const float* dens, const float* pres, int n, const OrcParams* P,
+                           double* scale);
+void orc_pbf_ext_force_scale_f(const float* pos4, const float* vel4, const float* dens, int n, const OrcParams* P, double* scale);
+void orc_boundary_force_scale_f(const float* pos4, int n, const float* b4, const float* vol, int nb, const OrcParams* P,
+                                const float* dens, const float* pres, double* scale);
+void orc_pbf_lambda_scale_f(const float* pos4, int n, const float* b4, const float* vol, int nb, const OrcParams* P, double* scale,
+                            double* scale_b);
+void orc_pbf_dp_scale_f(const float* pos4, const float* lambda, int n, const float* b4, const float* vol, const float* lambda_b, int nb,
+                        const OrcParams* P, float dt, double* scale);
 /* One PBF step of the reference CPU path with a fixed number of Jacobi iterations, collisions and the
  * optional XSPH pass; lambda_sum (may be NULL) receives sum_k lambda_k; dens the last lambda-pass density. */
 void orc_pbf_step_f(float* pos4, float* vel4, int n, const OrcParams* P, float* dens, float* lambda_sum, OrcPbfStats* st);

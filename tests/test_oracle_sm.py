@@ -172,3 +172,33 @@ def test_masked_rigid_motion_is_a_fixed_point(orc, beta):
     np.testing.assert_allclose(p[a, :3], pos[a, :3], atol=2e-5)
     # inactive particles keep their prediction (zero velocity, no gravity here: unchanged)
     np.testing.assert_array_equal(p[~a, :3], pos[~a, :3])
+
+
+@pytest.mark.parametrize(
+    "p0,v0,expect_v",
+    [
+        # crosses +x: x reflects with restitution 0.9 about the start point, y and z stay at the start point
+        ((0.99, 0.5, 0.5), (3.0, 0.2, -0.1), (-0.9 * 3.0, 0.0, 0.0)),
+        # crosses the floor: y reflects (gravity included in the reflected step), x and z stay
+        ((0.5, 0.01, 0.5), (0.1, -2.0, 0.3), (0.0, -0.9 * (-2.0 - 9.8 * 0.01), 0.0)),
+        # crosses x and y (edge): the x branch runs first and resets y to the start point, which is inside,
+        # so only x reflects
+        ((0.99, 0.01, 0.5), (3.0, -2.0, 0.0), (-0.9 * 3.0, 0.0, 0.0)),
+        # stays inside: free flight under gravity
+        ((0.5, 0.5, 0.5), (0.1, 0.2, 0.3), (0.1, 0.2 - 9.8 * 0.01, 0.3)),
+    ],
+)
+def test_single_particle_wall_response(orc, p0, v0, expect_v):
+    # OrientedPrtObj::UpdateCluster_SM (ElasticObject_OP.cpp:572-608) with a one-member cluster (no matching:
+    # ShapeMatching returns early for one vertex).  Box [0, 1]^3, dt = 0.01, g = -9.8 y.
+    S = _params(alpha=1.0, beta=0.0, iters=1, g=(0.0, -9.8, 0.0), dt=0.01)
+    for k in range(3):
+        S.bmin[k], S.bmax[k] = 0.0, 1.0
+    p = np.array([[*p0, 0.0]], np.float64)
+    v = np.array([[*v0, 0.0]], np.float64)
+    rest = p.copy()
+    cs, mem = np.array([0, 1], np.int32), np.array([0], np.int32)
+    orc.orc_sm_step_d(oracle.ptr(p), oracle.ptr(v), 1, oracle.ptr(cs), oracle.ptr(mem), 1, oracle.ptr(rest), C.byref(S))
+    np.testing.assert_allclose(v[0, :3], expect_v, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(p[0, :3], np.array(p0) + 0.01 * np.array(expect_v), rtol=1e-12, atol=1e-14)
+    assert np.all((p[0, :3] >= 0.0) & (p[0, :3] <= 1.0))
