@@ -283,6 +283,10 @@ SIGNATURES: dict[str, tuple] = {
 CXX_SYMBOLS = {
     "LaunchShapeMatchingGPU": ("_Z22LaunchShapeMatchingGPUiPfS_S_S_S_S_S_PiS0_f",
                                (None, [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, flt])),
+    "LaunchShapeMatchingIterationGPU": ("_Z31LaunchShapeMatchingIterationGPUiPfS_S_S_S_S_S_S_S_PiS0_f",
+                                        (None, [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, flt])),
+    "LaunchCalcAverageGPU": ("_Z20LaunchCalcAverageGPUiPfS_S_S_S_S_PiS0_S0_iii",
+                             (None, [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, i32])),
     "LaunchInterPolationGPU": ("_Z22LaunchInterPolationGPUiPfS_S_S_", (None, [i32, vp, vp, vp, vp])),
     "LaunchHeatTransferGPU": ("_Z21LaunchHeatTransferGPUPfS_S_PKiS1_PKfPiS4_S1_fffffi",
                               (None, [vp, vp, vp, vp, vp, vp, vp, vp, vp, flt, flt, flt, flt, flt, i32])),

@@ -78,6 +78,24 @@ int rxIceSM_KernelTimes(void* sm, float* cluster_ms, float* average_ms);
  * Gravity and walls come from the last CuSetParameters (Gravity, BoundaryMin/Max). */
 void LaunchShapeMatchingGPU(int prtNum, float* prtPos, float* prtVel, float* orgPos, float* curPos, float* orgCm, float* curCm,
                             float* vel, int* pIndxes, int* d_IndxSet, float dt);
+/* Ice_SM.h:38-53 (LaunchShapeMatchingIterationGPU): one iteration pass per cluster as
+ * OrientedPrtObj::UpdateCluster_SM_Itr (ElasticObject_OP.cpp:722-887): the members take the averaged solid
+ * positions sldPos (3 floats per particle, from LaunchCalcAverageGPU), the cluster is matched rigidly (Apq's third
+ * column negated when det Apq < 0 and the cluster has at least 10 members), and the goal positions, clamped to the
+ * walls of the last CuSetParameters, are written to curPos; curCm receives the cluster's centre of mass.  vel is
+ * not written (the reference's iteration pass leaves it); prtPos, prtVel, sldVel and dt are not read.  Same slot
+ * layout as LaunchShapeMatchingGPU; prtNum is the number of clusters. */
+void LaunchShapeMatchingIterationGPU(int prtNum, float* prtPos, float* prtVel, float* sldPos, float* sldVel, float* orgPos, float* curPos,
+                                     float* orgCm, float* curCm, float* vel, int* pIndxes, int* d_IndxSet, float dt);
+/* IceObject.h:370-386 (LaunchCalcAverageGPU): per particle p < prtNum, the mean position and velocity of its
+ * cluster members, ConvoNormal::StepConvolution (Ice_Convolution_Normal.cpp:32-67) in the reference's PtoC layout
+ * (IceStructure.cu:63-126): entry j < min(PtoCIndx[p], PtoCMax) names cluster PtoC[base + j] and member number
+ * PtoC[base + PtoCMax + j], base = PtoCMax * PtoCParamSize * p, -1 = empty; the member's slot is
+ * smIndxSet[2 * cluster] + member number.  smPrtPos/smPrtVel hold 3 floats per slot, sldPrtPos/sldPrtVel 3 per
+ * particle.  A particle in no cluster keeps its solid values (the reference kernel wrote uninitialised sums);
+ * sphPrtPos, sphPrtVel and PNumMax are not read (as in the reference kernel). */
+void LaunchCalcAverageGPU(int prtNum, float* sldPrtPos, float* sldPrtVel, float* sphPrtPos, float* sphPrtVel, float* smPrtPos, float* smPrtVel,
+                          int* smIndxSet, int* PtoCIndx, int* PtoC, int PNumMax, int PtoCMax, int PtoCParamSize);
 /* IceObject.h:388-395 (LaunchInterPolationGPU): copies the solid positions/velocities into the fluid arrays
  * (blend coefficient 1, IceObject.cpp:876-897). sldPrtPos/Vel hold 3 floats per particle, sphPrtPos/Vel 4. */
 void LaunchInterPolationGPU(int prtNum, float* sldPrtPos, float* sldPrtVel, float* sphPrtPos, float* sphPrtVel);

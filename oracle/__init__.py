@@ -81,6 +81,9 @@ _SIGS = {
     "orc_sm_step_f": (None, [P, P, I, P, P, I, P, P]),
     "orc_sm_step_d": (None, [P, P, I, P, P, I, P, P]),
     "orc_sm_step_masked_f": (None, [P, P, I, P, P, I, P, P, P]),
+    "orc_legacy_cluster_f": (None, [I, P, P, P, P, P, P, P, P, P, P, F, P, I]),
+    "orc_legacy_cluster_d": (None, [I, P, P, P, P, P, P, P, P, P, P, D, P, I]),
+    "orc_legacy_average_f": (None, [I, P, P, P, P, P, P, P, I, I]),
     "orc_heat_diffuse_f": (None, [P, P, P, I, P, F, F, P]),
     "orc_heat_transfer_gpu_f": (None, [P, P, P, P, P, P, P, P, P, F, F, F, F, F, I]),
     "orc_heat_cpu_f": (None, [P, P, P, I, P, F, F]),

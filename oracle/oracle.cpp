@@ -896,6 +896,110 @@ template <class T> void sm_step(T* pos, T* vel, int n, const int* cs, const int*
     }
 }
 
+// ------------------------------------------------------------------------------------------ legacy ice launches
+// The reference's per-cluster slot layout (Ice_SM.h:23-53, IceObject.h:370-395): member slots of cluster c are
+// indxSet[2c] .. indxSet[2c+1] (inclusive), pIdx[slot] = particle, orgPos/curPos/vel 3 values per slot,
+// orgCm/curCm 3 per cluster, particle arrays prtPos/prtVel 4 per particle, sldPos 3 per particle.  Unit masses.
+//
+// itr = false: OrientedPrtObj::UpdateCluster_SM (ElasticObject_OP.cpp:551-720): prediction x + (v + g dt) dt
+//   with the wall response, rigid matching (alpha = 1), goal written to curPos, vel = (goal - x) / dt.
+// itr = true: OrientedPrtObj::UpdateCluster_SM_Itr (ElasticObject_OP.cpp:722-887): curPos = sldPos of the
+//   member, rigid matching with Apq's third column negated when det Apq < 0 and the cluster has at least 10
+//   members, goal written to curPos and clamped to the walls; vel untouched.
+// A cluster of one member keeps its first positions (both return before the matching).
+template <class T> void legacy_cluster(int nc, const T* prtPos, const T* prtVel, const T* sldPos, const T* orgPos, T* curPos,
+                                       const T* orgCm, T* curCm, T* vel, const int* pIdx, const int* indxSet, double dt,
+                                       const double* g, const double* lo, const double* hi, double res, bool itr) {
+    for (int c = 0; c < nc; ++c) {
+        const int s = indxSet[2 * c], e = indxSet[2 * c + 1];
+        for (int i = s; i <= e; ++i) {
+            const int p = pIdx[i];
+            if (itr) {
+                for (int a = 0; a < 3; ++a) curPos[3 * i + a] = sldPos[3 * p + a];
+                continue;
+            }
+            double x[3], v[3], cp[3];
+            for (int a = 0; a < 3; ++a) {
+                x[a] = prtPos[4 * p + a];
+                v[a] = prtVel[4 * p + a];
+                cp[a] = R<T>(x[a] + (v[a] + g[a] * dt) * dt);
+            }
+            for (int a = 0; a < 3; ++a)
+                if (cp[a] < lo[a] || cp[a] > hi[a]) {
+                    cp[a] = R<T>(x[a] - (v[a] + g[a] * dt) * dt * res);
+                    for (int o = 0; o < 3; ++o)
+                        if (o != a) cp[o] = x[o];
+                }
+            for (int a = 0; a < 3; ++a) curPos[3 * i + a] = (T)std::min(std::max(cp[a], lo[a]), hi[a]);
+        }
+        const int k = e - s + 1;
+        if (k <= 1) continue;
+        double cm[3] = {0, 0, 0};
+        for (int i = s; i <= e; ++i)
+            for (int a = 0; a < 3; ++a) cm[a] += (double)curPos[3 * i + a];
+        for (int a = 0; a < 3; ++a) cm[a] /= k;
+        double apq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int i = s; i <= e; ++i) {
+            double pp[3], qq[3];
+            for (int a = 0; a < 3; ++a) {
+                pp[a] = (double)curPos[3 * i + a] - cm[a];
+                qq[a] = (double)orgPos[3 * i + a] - (double)orgCm[3 * c + a];
+            }
+            for (int r = 0; r < 3; ++r)
+                for (int cc = 0; cc < 3; ++cc) apq[r * 3 + cc] += pp[r] * qq[cc];
+        }
+        if (itr && k >= 10 && det3(apq) < 0.0)
+            for (int r = 0; r < 3; ++r) apq[r * 3 + 2] = -apq[r * 3 + 2];
+        double Rm[9];
+        polar(apq, Rm, nullptr);
+        for (int i = s; i <= e; ++i) {
+            double qq[3];
+            for (int a = 0; a < 3; ++a) qq[a] = (double)orgPos[3 * i + a] - (double)orgCm[3 * c + a];
+            for (int a = 0; a < 3; ++a) {
+                double gp = Rm[a * 3] * qq[0] + Rm[a * 3 + 1] * qq[1] + Rm[a * 3 + 2] * qq[2] + cm[a];
+                if (itr) gp = std::min(std::max(R<T>(gp), lo[a]), hi[a]);
+                curPos[3 * i + a] = (T)gp;
+            }
+        }
+        for (int a = 0; a < 3; ++a) curCm[3 * c + a] = (T)cm[a];
+        if (!itr)
+            for (int i = s; i <= e; ++i) {
+                const int p = pIdx[i];
+                for (int a = 0; a < 3; ++a) vel[3 * i + a] = (T)(((double)curPos[3 * i + a] - (double)prtPos[4 * p + a]) / dt);
+            }
+    }
+}
+
+// ConvoNormal::StepConvolution (Ice_Convolution_Normal.cpp:32-67) in the PtoC layout of LaunchCalcAverageGPU
+// (IceStructure.cu:63-126): particle p belongs to PtoCIndx[p] entries; entry j names cluster
+// PtoC[base + j] and member number PtoC[base + PtoCMax + j] (base = PtoCMax * PtoCParamSize * p, -1 = none).
+// The mean member position and velocity over the clusters holding p become sldPos / sldVel of p; a particle in
+// no cluster keeps its values.
+template <class T> void legacy_average(int n, T* sldPos, T* sldVel, const T* smPos, const T* smVel, const int* indxSet,
+                                       const int* PtoCIndx, const int* PtoC, int PtoCMax, int PtoCParamSize) {
+    for (int p = 0; p < n; ++p) {
+        const long base = (long)PtoCMax * PtoCParamSize * p;
+        const int cnt = std::min(PtoCIndx[p], PtoCMax);
+        double sp[3] = {0, 0, 0}, sv[3] = {0, 0, 0};
+        int k = 0;
+        for (int j = 0; j < cnt; ++j) {
+            const int c = PtoC[base + j], o = PtoC[base + PtoCMax + j];
+            if (c == -1 || o == -1) continue;
+            const long slot = (long)indxSet[2 * c] + o;
+            for (int a = 0; a < 3; ++a) {
+                sp[a] += (double)smPos[3 * slot + a];
+                sv[a] += (double)smVel[3 * slot + a];
+            }
+            ++k;
+        }
+        if (k == 0) continue;
+        for (int a = 0; a < 3; ++a) {
+            sldPos[3 * p + a] = (T)(sp[a] / k);
+            sldVel[3 * p + a] = (T)(sv[a] / k);
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------------------ MLP
 inline double silu(double z) { return z / (1.0 + std::exp(-z)); }
 inline double dsilu(double z) { double s = 1.0 / (1.0 + std::exp(-z)); return s * (1.0 + z * (1.0 - s)); }
@@ -1130,6 +1234,22 @@ void orc_sm_step_masked_f(float* pos4, float* vel4, int n, const int* cs, const 
 }
 void orc_sm_step_d(double* pos4, double* vel4, int n, const int* cs, const int* mem, int nc, const double* rest, const OrcSmParams* S) {
     sm_step(pos4, vel4, n, cs, mem, nc, rest, *S);
+}
+void orc_legacy_cluster_f(int nc, const float* prtPos4, const float* prtVel4, const float* sldPos3, const float* orgPos, float* curPos,
+                          const float* orgCm, float* curCm, float* vel, const int* pIdx, const int* indxSet, float dt,
+                          const OrcSmParams* S, int itr) {
+    legacy_cluster(nc, prtPos4, prtVel4, sldPos3, orgPos, curPos, orgCm, curCm, vel, pIdx, indxSet, (double)dt, S->gravity, S->bmin,
+                   S->bmax, S->restitution, itr != 0);
+}
+void orc_legacy_cluster_d(int nc, const double* prtPos4, const double* prtVel4, const double* sldPos3, const double* orgPos,
+                          double* curPos, const double* orgCm, double* curCm, double* vel, const int* pIdx, const int* indxSet,
+                          double dt, const OrcSmParams* S, int itr) {
+    legacy_cluster(nc, prtPos4, prtVel4, sldPos3, orgPos, curPos, orgCm, curCm, vel, pIdx, indxSet, dt, S->gravity, S->bmin, S->bmax,
+                   S->restitution, itr != 0);
+}
+void orc_legacy_average_f(int n, float* sldPos, float* sldVel, const float* smPos, const float* smVel, const int* indxSet,
+                          const int* PtoCIndx, const int* PtoC, int PtoCMax, int PtoCParamSize) {
+    legacy_average(n, sldPos, sldVel, smPos, smVel, indxSet, PtoCIndx, PtoC, PtoCMax, PtoCParamSize);
 }
 
 // SPH heat diffusion over the r <= h neighbours: T_i' = T_i + dt * kappa * sum_j (m/rho_j)(T_j - T_i) LW_visc.

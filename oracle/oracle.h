@@ -139,6 +139,20 @@ void orc_sm_step_d(double* pos4, double* vel4, int n, const int* cl_start, const
 void orc_sm_step_masked_f(float* pos4, float* vel4, int n, const int* cl_start, const int* members, int nclusters,
                           const float* rest_pos4, const OrcSmParams* S, const int* active);
 
+/* Legacy ice launches in the reference's slot layout (Ice_SM.h:23-53, IceObject.h:370-395).
+ * orc_legacy_cluster_*: itr = 0 is OrientedPrtObj::UpdateCluster_SM (LaunchShapeMatchingGPU: prediction, wall
+ * response, rigid matching, vel), itr = 1 is UpdateCluster_SM_Itr (LaunchShapeMatchingIterationGPU: members take
+ * sldPos (3 per particle), rigid matching with the reflection fix for >= 10 members, walls); S gives gravity,
+ * walls and restitution.  orc_legacy_average_f: the per-particle mean over its clusters (LaunchCalcAverageGPU). */
+void orc_legacy_cluster_f(int nc, const float* prtPos4, const float* prtVel4, const float* sldPos3, const float* orgPos, float* curPos,
+                          const float* orgCm, float* curCm, float* vel, const int* pIdx, const int* indxSet, float dt,
+                          const OrcSmParams* S, int itr);
+void orc_legacy_cluster_d(int nc, const double* prtPos4, const double* prtVel4, const double* sldPos3, const double* orgPos,
+                          double* curPos, const double* orgCm, double* curCm, double* vel, const int* pIdx, const int* indxSet,
+                          double dt, const OrcSmParams* S, int itr);
+void orc_legacy_average_f(int n, float* sldPos, float* sldVel, const float* smPos, const float* smVel, const int* indxSet,
+                          const int* PtoCIndx, const int* PtoC, int PtoCMax, int PtoCParamSize);
+
 /* ---- heat (HeatTransfar.cpp / HeatTransfer.cu, SPH diffusion) */
 void orc_heat_diffuse_f(const float* pos4, const float* dens, const float* temp, int n, const OrcParams* P,
                         float diffusivity, float dt, float* temp_out);
