@@ -1,10 +1,10 @@
 // fused_grid.h -- the neighbour grid of the fused solvers.
 //
-// Particle state is kept permanently in cell order.  Each rebuild hashes the current positions, counts
-// particles per cell with wave-aggregated atomics, scans the counts into cell offsets (off[c] = first slot of
-// cell c, off[G] = N, so empty cells need no sentinel and a row of cells is one slot range), places every
-// particle, and then orders each cell by the previous slot index, which makes the permutation stable and
-// deterministic.  `perm[new] = old` then drives a gather of every state array.
+// Particle state is kept permanently in cell order.  Each rebuild hashes the current positions and radix-sorts
+// (cell, previous slot) pairs, which makes the permutation stable (ties in previous-slot order) and
+// deterministic in linear time; the cell offsets (off[c] = first slot of cell c, off[G] = N, so empty cells need
+// no sentinel and a row of cells is one slot range) are found by binary search in the sorted cells.
+// `perm[new] = old` drives the gather of every state array (optionally done by the sort's last pass).
 #pragma once
 
 #include "rx_common.h"
@@ -68,18 +68,18 @@ struct FusedGrid {
     uint* hash = nullptr;    // cell of each slot, in the new (sorted) order after build
     uint* off = nullptr;     // ncells + 1 offsets
     uint* perm = nullptr;    // new slot -> old slot
-    uint* tmp_hash = nullptr;
-    uint* slot = nullptr;
+    uint* tmp_hash = nullptr;  // cells in the current order
+    uint* slot = nullptr;      // sort ping-pong pair
     uint* tsrc = nullptr;
-    void* scan_ws = nullptr;  // own scan workspace, so that a rebuild can be captured into a graph
+    void* radix_ws = nullptr;  // own sort workspace, so that a rebuild can be captured into a graph
     size_t cap_n = 0, cap_c = 0;
 
     void setup(const rxSimParams& p, float cell);
     bool reserve(uint n_particles);  // false (and every array released) when an allocation fails
     void release();
     // Builds the grid on pos (in the current slot order); afterwards hash/off describe the new order and
-    // perm maps new slots to current slots.  Nothing is reordered here.
-    void build(const float4* pos, hipStream_t s);
+    // perm maps new slots to current slots.  gather (may be null) names state arrays to reorder on the way.
+    void build(const float4* pos, hipStream_t s, const struct RadixGather* gather = nullptr);
 };
 
 template <class T> __global__ void k_gather(T* __restrict__ dst, const T* __restrict__ src, const uint* __restrict__ perm, uint n) {

@@ -70,7 +70,6 @@ private:
     float m_warm_scale = 1.0f;
     int m_judge = 0;
     bool m_grad_mass = false;  // RX_PBF_OPT_GRAD_MASS
-    float m_gamma = 0.0f;      // RX_PBF_OPT_MEMORY
     int m_list_tile = 0;  // LDS tiling variant of the list pass (RX_PBF_OPT_LIST_TILE)
     float m_sd_ref = 1.0f, m_nref = 1.0f;
     MlpDev m_mlp{};            // points into m_mlp_mem: the solver's own copy of the network

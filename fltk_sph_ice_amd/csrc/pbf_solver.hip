@@ -18,6 +18,7 @@
 
 #include "fused_grid.h"
 #include "pbf_solver.h"
+#include "radix.h"
 #include "mlp.h"
 #include "../../include/rx_cu_funcs.h"
 #include "../../include/rx_sph_host.h"
@@ -40,8 +41,6 @@ struct PbfK {
     float tol;        // adaptive solve: stop once the judged density error is at most tol
     int judge;        // 0: max (rho - rho0)+ / rho0; 1: its mean over the particles
     float lam0_scale; // factor on the first multiplier (scaled mode; 1 otherwise)
-    const float* prev;// the multiplier each particle's previous step applied in total (slot order), or null
-    float gamma;      // share of min(prev, 0) added to the first multiplier (RX_PBF_OPT_MEMORY)
     // feature normalisation of the learned warm start
     float sd_ref;     // lambda denominator of an interior particle of the rest lattice
     float inv_nref;   // 1 / neighbour count of the rest lattice
@@ -452,8 +451,7 @@ __device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, const 
     sd += sgs;
     float C = d * k.s.inv_rho0 - 1.0f;
     float lam = -C / (sd + k.eps);
-    const float mem = k.prev ? fminf(k.prev[i], 0.0f) : 0.0f;  // pressure part of the previous step's multiplier
-    xl[i] = make_float4(p.x, p.y, p.z, lam * k.lam0_scale + k.gamma * mem);
+    xl[i] = make_float4(p.x, p.y, p.z, lam * k.lam0_scale);
     xbuild[i] = p4;
     dens[i] = d;
     float err = fmaxf(C, 0.0f);
@@ -1267,22 +1265,25 @@ int rxPBDSPH_GPU::enqueue_step(float dt) {
     // ---- rebuild on x*; the step-start density follows the particles
     k_fill<float><<<1, 64, 0, s>>>(m_stats + ST_DISP2, 0.0f, 1);
     mk = begin_mark(RX_PROF_GRID);
-    m_grid.build(m_xs, s);
-    reorder(m_xs, s);
-    reorder(m_x, s);
-    reorder(m_v, s);
-    fg_gather<uint>(m_idt, m_id, m_grid.perm, n, s);
-    std::swap(m_id, m_idt);
-    if (m_therm) {
-        fg_gather<float4>(m_therm_t, m_therm, m_grid.perm, n, s);
-        std::swap(m_therm, m_therm_t);
+    {
+        // the sort's last pass gathers the whole state into the new order (xl and xb are dead until the list
+        // pass rewrites them, so they take the gathered positions and velocities) and clears the multiplier sums
+        RadixGather gs;
+        gs.n4 = m_therm ? 4 : 3;
+        gs.src4[0] = m_xs, gs.dst4[0] = m_t;
+        gs.src4[1] = m_x, gs.dst4[1] = m_xl;
+        gs.src4[2] = m_v, gs.dst4[2] = m_xb;
+        gs.src4[3] = m_therm, gs.dst4[3] = m_therm_t;
+        gs.srcu = m_id, gs.dstu = m_idt;
+        gs.zero = m_lamacc;
+        m_grid.build(m_xs, s, &gs);
     }
-    const bool memory = m_gamma > 0.0f;
-    if (memory) fg_gather<float>(m_ftmp, m_lamacc, m_grid.perm, n, s);  // the previous step's total multiplier
-    rx_fill<float>(m_lamacc, 0.0f, n, s);
+    std::swap(m_xs, m_t);
+    std::swap(m_x, m_xl);
+    std::swap(m_v, m_xb);
+    std::swap(m_id, m_idt);
+    if (m_therm) std::swap(m_therm, m_therm_t);
     end_mark(mk);
-    k.prev = memory && m_step > 0 ? m_ftmp : nullptr;
-    k.gamma = m_gamma;
 
     // ---- list + lambda_0 (the network's multiplier in learned mode), then up to K corrections
     const bool feats = (m_mode == RX_PBF_MODE_LEARNED) || m_record_features;
@@ -1636,13 +1637,6 @@ int rxPBDSPH_GPU::SetOption(int opt, float value) {
         if (!m_adapt) m_level_target = 0;  // back to the set skin at the next step
         return 0;
     case RX_PBF_OPT_HEAT_DT: m_heat_dt = value; return 0;
-    case RX_PBF_OPT_MEMORY:
-        if (value < 0.0f || value > 1.0f) {
-            rx_set_error(1, "rxPBDSPH_GPU::SetOption: memory share must be in [0, 1]");
-            return 1;
-        }
-        m_gamma = value;
-        return 0;
     case RX_PBF_OPT_GRAD_MASS:
         m_grad_mass = value != 0.0f;
         lattice_refs();  // the feature scales follow the gradient

@@ -107,3 +107,15 @@ template <class T> inline void rx_fill(T* p, T v, size_t n, hipStream_t s) {
 void rx_scan_u32(const uint* in, uint* out, size_t n, bool write_total, hipStream_t s, void* ws = nullptr);
 size_t rx_scan_ws_bytes(size_t n);
 void rx_scan_f32(const float* in, float* out, size_t n, hipStream_t s);
+
+// Stable LSD radix sort (8-bit digits) of n (key, value) pairs by the low nbits bits of the key: reads (k0, v0)
+// (v0 null: the values are the positions 0 .. n-1), ping-pongs through (k1, v1) and (k2, v2) and leaves the
+// sorted pairs in (k1, v1).  The first pass writes (k1, v1) when rx_radix_passes(nbits) is odd and (k2, v2)
+// when it is even; the other pair may be the input pair itself (it is written only after the first pass has
+// read the input).  ws: rx_radix_ws_bytes(n) bytes of device workspace (no allocation happens here, so the
+// launches can be captured into a graph).
+int rx_radix_passes(int nbits);
+size_t rx_radix_ws_bytes(size_t n);
+void rx_radix_sort_pairs(const uint* k0, const uint* v0, uint* k1, uint* v1, uint* k2, uint* v2, size_t n, int nbits, void* ws,
+                         hipStream_t s);
+inline int rx_bits_for(uint max_key) { return max_key ? 32 - __builtin_clz(max_key) : 1; }

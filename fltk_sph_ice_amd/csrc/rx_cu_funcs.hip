@@ -240,55 +240,26 @@ __global__ void k_calc_hash_b(uint* __restrict__ hash, uint* __restrict__ index,
     index[i] = (uint)i;
 }
 
-// ---- stable counting sort by bucket (cells 0..ncells-1, bucket ncells = the deleted-particle marker)
-__global__ void k_sort_count(const uint* __restrict__ key, uint n, uint ncells, uint* __restrict__ cnt,
-                             uint* __restrict__ slot, uint* __restrict__ bucket, uint* __restrict__ is_del, int* __restrict__ bad) {
+// ---- CuSort: keys are mapped to buckets (cells 0..ncells-1; bucket ncells = the deleted-particle marker and any
+// other out-of-range key), the (bucket, index) pairs are radix-sorted (stable), and the buckets are mapped back.
+__global__ void k_sort_bucket(const uint* __restrict__ key, uint n, uint ncells, uint* __restrict__ bucket, int* __restrict__ bad) {
     uint i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint h = key[i];
-    uint b;
-    if (h < ncells) b = h;
-    else {
+    const uint h = key[i];
+    uint b = h;
+    if (h >= ncells) {
         b = ncells;
         if (h != kDeletedHash) atomicOr(bad, 1);
     }
     bucket[i] = b;
-    is_del[i] = (b == ncells) ? 1u : 0u;
-    slot[i] = atomicAdd(&cnt[b], 1u);
 }
-
-__global__ void k_sort_scatter(const uint* __restrict__ key, const uint* __restrict__ val, const uint* __restrict__ bucket,
-                               const uint* __restrict__ slot, const uint* __restrict__ off, uint n,
-                               uint* __restrict__ tkey, uint* __restrict__ tval, uint* __restrict__ tsrc) {
+// (oval may be sv: each element is read before it is written)
+__global__ void k_sort_unbucket(const uint* __restrict__ sb, const uint* sv, uint n, uint ncells, uint* __restrict__ okey, uint* oval) {
     uint i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint p = off[bucket[i]] + slot[i];
-    tkey[p] = key[i];
-    tval[p] = val[i];
-    tsrc[p] = i;
-}
-
-// The atomic slots order a bucket arbitrarily; the final place of each element is the count of elements
-// of the same bucket with a smaller input position (stability).  The deleted-marker bucket uses the
-// prefix count of markers instead (del_rank), which stays linear for any number of deleted particles.
-__global__ void k_sort_rank(const uint* __restrict__ tkey, const uint* __restrict__ tval, const uint* __restrict__ tsrc,
-                            const uint* __restrict__ off, const uint* __restrict__ del_rank, uint n, uint ncells,
-                            uint* __restrict__ okey, uint* __restrict__ oval) {
-    uint p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    uint h = tkey[p];
-    uint src = tsrc[p];
-    uint dst;
-    if (h < ncells) {
-        uint s = off[h], e = off[h + 1];
-        uint r = 0;
-        for (uint q = s; q < e; ++q) r += (tsrc[q] < src) ? 1u : 0u;
-        dst = s + r;
-    } else {
-        dst = off[ncells] + del_rank[src];
-    }
-    okey[dst] = h;
-    oval[dst] = tval[p];
+    const uint b = sb[i];
+    okey[i] = b == ncells ? kDeletedHash : b;
+    oval[i] = sv[i];
 }
 
 // rx_sph_kernel.cu:108-153 (reorderDataAndFindCellStartD): cell bounds from the sorted keys (empty cells were
@@ -415,7 +386,30 @@ void CuScanf(float* output, float* input, unsigned int numElements) {
     RX_HIP(hipStreamSynchronize(rx_stream()));
 }
 
-// Stable counting sort (replaces thrust::sort_by_key of rx_cu_funcs.cu:255-260).
+// Stable sort of (hash, index) pairs (replaces thrust::sort_by_key of rx_cu_funcs.cu:255-260): ties keep their
+// input order.  Scratch: 3 n uints (the buckets and one ping-pong pair; the buckets and dIndex are the other
+// pair) + the radix workspace.
+size_t rx_sort_scratch_bytes(size_t n) { return ((sizeof(uint) * 3 * n + 255) & ~(size_t)255) + rx_radix_ws_bytes(n); }
+
+// The launches of CuSort on stream s with every buffer supplied (scratch: rx_sort_scratch_bytes(n) bytes, 256-byte
+// aligned).  Sets the device int `bad` when a key is neither a cell hash nor the deleted marker.
+void rx_sort_launch(uint* dHash, uint* dIndex, uint n, uint nc, void* scratch, int* bad, hipStream_t s) {
+    uint* bucket = (uint*)scratch;
+    uint* kx = bucket + (size_t)n;
+    uint* vx = bucket + 2 * (size_t)n;
+    void* rws = (char*)scratch + ((sizeof(uint) * 3 * (size_t)n + 255) & ~(size_t)255);
+    const unsigned blocks = rx_blocks(n, 256);
+    const int nbits = rx_bits_for(nc);
+    k_sort_bucket<<<blocks, 256, 0, s>>>(dHash, n, nc, bucket, bad);
+    if (rx_radix_passes(nbits) & 1) {  // the first pass writes (kx, vx), where the result ends
+        rx_radix_sort_pairs(bucket, dIndex, kx, vx, bucket, dIndex, n, nbits, rws, s);
+        k_sort_unbucket<<<blocks, 256, 0, s>>>(kx, vx, n, nc, dHash, dIndex);
+    } else {  // the first pass writes (kx, vx), the last one (bucket, dIndex)
+        rx_radix_sort_pairs(bucket, dIndex, bucket, dIndex, kx, vx, n, nbits, rws, s);
+        k_sort_unbucket<<<blocks, 256, 0, s>>>(bucket, dIndex, n, nc, dHash, dIndex);
+    }
+}
+
 void CuSort(unsigned int* dHash, uint* dIndex, uint numParticles) {
     if (numParticles == 0) return;
     if (!g_params_set) {
@@ -424,27 +418,12 @@ void CuSort(unsigned int* dHash, uint* dIndex, uint numParticles) {
     }
     hipStream_t s = rx_stream();
     // keys are cell hashes of the fluid grid or of the boundary-particle grid (CuCalcHashB)
-    uint n = numParticles, nc = std::max(g_params.NumCells, g_params.NumCellsB);
-    size_t nb = (size_t)nc + 1;
-    uint* cnt = (uint*)rx_scratch(0, sizeof(uint) * (nb + 1));
-    uint* aux = (uint*)rx_scratch(1, sizeof(uint) * (size_t)n * 7 + 64);
-    if (!cnt || !aux) return;
-    uint* slot = aux;
-    uint* bucket = aux + n;
-    uint* isdel = aux + 2 * (size_t)n;
-    uint* tkey = aux + 3 * (size_t)n;
-    uint* tval = aux + 4 * (size_t)n;
-    uint* tsrc = aux + 5 * (size_t)n;
-    uint* delrank = aux + 6 * (size_t)n;
-    int* bad = (int*)(aux + 7 * (size_t)n);
-    rx_fill<uint>(cnt, 0u, nb + 1, s);
+    const uint n = numParticles, nc = std::max(g_params.NumCells, g_params.NumCellsB);
+    void* scratch = rx_scratch(1, rx_sort_scratch_bytes(n) + 256);
+    int* bad = (int*)rx_scratch(0, 256);
+    if (!scratch || !bad) return;
     rx_fill<int>(bad, 0, 1, s);
-    unsigned blocks = rx_blocks(n, 256);
-    k_sort_count<<<blocks, 256, 0, s>>>(dHash, n, nc, cnt, slot, bucket, isdel, bad);
-    rx_scan_u32(cnt, cnt, nb, true, s);  // cnt now holds bucket offsets, cnt[nb] = n
-    rx_scan_u32(isdel, delrank, n, false, s);
-    k_sort_scatter<<<blocks, 256, 0, s>>>(dHash, dIndex, bucket, slot, cnt, n, tkey, tval, tsrc);
-    k_sort_rank<<<blocks, 256, 0, s>>>(tkey, tval, tsrc, cnt, delrank, n, nc, dHash, dIndex);
+    rx_sort_launch(dHash, dIndex, n, nc, scratch, bad, s);
     int hbad = 0;
     RX_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
     RX_HIP(hipStreamSynchronize(s));
@@ -453,8 +432,8 @@ void CuSort(unsigned int* dHash, uint* dIndex, uint numParticles) {
 
 void CuSPHInit(int max_particles) {
     if (max_particles > 0) {
-        rx_scratch(0, sizeof(uint) * 1024);
-        rx_scratch(1, sizeof(uint) * (size_t)max_particles * 7 + 64);
+        rx_scratch(0, 256);
+        rx_scratch(1, rx_sort_scratch_bytes((size_t)max_particles) + 256);
     }
 }
 void CuSPHClean(void) { rx_scratch_free_all(); }

@@ -140,3 +140,58 @@ def test_scan_uint_and_float(lib, n):
     check(lib)
     reff = np.concatenate([[0.0], np.cumsum(f.astype(np.float64))[:-1]])
     assert np.allclose(do.download(np.float32), reff, rtol=1e-5, atol=1e-3)
+
+
+@pytest.mark.parametrize("n", [1 << 20, 4097, 4096, 1])
+def test_sort_all_keys_in_one_bucket(lib, n):
+    # the stable sort is linear whatever the key distribution: every key equal keeps the input order exactly
+    sc = scene.wcsph_dam_break(2, 2, 2)
+    sp = _setup(lib, sc)
+    keys = np.full(n, 3, np.uint32)
+    vals = np.random.default_rng(1).permutation(n).astype(np.uint32)
+    dk, dv = DevBuf.from_array(keys), DevBuf.from_array(vals)
+    lib.CuSort(dk.ptr, dv.ptr, n)
+    check(lib)
+    assert np.array_equal(dk.download(np.uint32), keys)
+    assert np.array_equal(dv.download(np.uint32), vals)
+
+
+def test_sort_border_cell_with_a_tenth_of_the_particles(lib):
+    # 10% of 1M particles clamped into the corner cell of the grid (particles outside the world), the rest spread
+    sc = scene.wcsph_dam_break(8, 8, 8)
+    sp = _setup(lib, sc)
+    n = 1 << 20
+    rng = np.random.default_rng(5)
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, :3] = rng.uniform(0.0, 1.0, (n, 3)).astype(np.float32)
+    out = rng.choice(n, n // 10, replace=False)
+    pos[out, :3] = rng.uniform(-3.0, -1.0, (len(out), 3)).astype(np.float32)  # all hash to cell 0
+    g = GridBuffers(n, sp.NumCells, lib)
+    dpos = DevBuf.from_array(pos)
+    lib.CuCalcHash(g.hash.ptr, g.index.ptr, dpos.ptr, None, n)
+    lib.CuSort(g.hash.ptr, g.index.ptr, n)
+    check(lib)
+    ref_h, ref_hs, ref_ids, _, _ = _oracle_grid(pos, sp)
+    assert (ref_h == 0).sum() >= n // 10
+    assert np.array_equal(g.hash.download(np.uint32), ref_hs)
+    assert np.array_equal(g.index.download(np.uint32), ref_ids)
+
+
+@pytest.mark.parametrize("nbits_keys", [(1000, 200), (5000, 300), (200000, 1 << 21), (70001, (1 << 24) - 1)])
+def test_sort_random_keys_matches_stable_argsort(lib, nbits_keys):
+    # one-, two- and three-pass key widths (bucket count = NumCells of a grid with that many cells)
+    n, ncells = nbits_keys
+    sc = scene.wcsph_dam_break(2, 2, 2)
+    sp = sc.params()
+    sp.NumCells = ncells
+    lib.CuSetParameters(C.byref(sp))
+    rng = np.random.default_rng(n)
+    keys = rng.integers(0, ncells, n).astype(np.uint32)
+    keys[::97] = 2147483647  # deleted-particle markers sort last
+    vals = np.arange(n, dtype=np.uint32)
+    dk, dv = DevBuf.from_array(keys), DevBuf.from_array(vals)
+    lib.CuSort(dk.ptr, dv.ptr, n)
+    check(lib)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(dk.download(np.uint32), keys[order])
+    assert np.array_equal(dv.download(np.uint32), vals[order])

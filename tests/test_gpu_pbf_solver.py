@@ -231,3 +231,24 @@ def test_scene_file_run_and_particle_file_restart(tmp_path):
     assert a.stats()[22] == 0 and b.stats()[22] == 0
     a.close()
     b.close()
+
+
+def test_one_step_parity_with_a_crowded_border_cell():
+    # a tenth of the particles squeezed into the corner cell of the box (about 20 times the rest density):
+    # one grid cell holds hundreds of particles (the stable sort ranks them linearly; their lists overflow
+    # and the passes walk the grid) and the step still matches the oracle
+    sc = scene.pbf_dam_break(20, 12, 10)
+    pos, vel = sc.particles()
+    rng = np.random.default_rng(12)
+    crowd = rng.choice(sc.n, sc.n // 10, replace=False)
+    pos[crowd, :3] = rng.uniform(0.03, 0.09, (len(crowd), 3)).astype(np.float32)
+    P = oracle.params_from(sc.params(), sc)
+    rpos, rvel, rdens, st = oracle.pbf_step(pos, vel, P)
+    s = PBFSolver(sc, pos, vel)
+    s.step(1)
+    assert s.stats()[18] > 0  # the crowded lists overflowed
+    gpos = s.positions()
+    scale = np.abs(rpos[:, :3]).max()
+    err = np.abs(gpos[:, :3] - rpos[:, :3]).max() / scale
+    print(f"crowded corner cell: {len(crowd)} particles, max position error {err:.2e} of the box scale")
+    assert err <= 1e-5
