@@ -118,6 +118,7 @@ SIGNATURES: dict[str, tuple] = {
     "CuUnmapGLBufferObject": (None, [vp]),
     "CuScan": (None, [vp, vp, u32]),
     "CuSort": (None, [vp, vp, u32]),
+    "CuSortNoAlloc": (None, [vp, vp, u32]),
     "CuScanf": (None, [vp, vp, u32]),
     "CuSPHInit": (None, [i32]),
     "CuSPHClean": (None, []),

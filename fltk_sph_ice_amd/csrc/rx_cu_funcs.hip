@@ -302,11 +302,14 @@ void CuInit(int argc, char** argv) {
     CuSetDevice(id);
 }
 
+static void* rx_noalloc_scratch();
+
 void CuSetDevice(int id) {
     int count = 0;
     if (!RX_HIP(hipGetDeviceCount(&count))) return;
     if (id < 0 || id >= count) id = 0;
-    RX_HIP(hipSetDevice(id));
+    if (!RX_HIP(hipSetDevice(id))) return;
+    rx_noalloc_scratch();  // set-up of the allocation-free sort's static scratch on this device
 }
 
 void CuDeviceProp(void) {
@@ -428,6 +431,62 @@ void CuSort(unsigned int* dHash, uint* dIndex, uint numParticles) {
     RX_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s));
     RX_HIP(hipStreamSynchronize(s));
     if (hbad) rx_set_error(4, "CuSort: keys other than cell hashes (< %u) or 0x7FFFFFFF were sorted as 0x7FFFFFFF", nc);
+}
+
+// ---- the allocation-free sort: its scratch is a static device array (one copy per GPU, held until the process
+// ends), whose address on the current device is looked up once per device.
+namespace {
+constexpr size_t kNoAllocMaxN = (size_t)1 << 21;
+constexpr size_t kNoAllocFlag = 256;  // the out-of-range flag, then the sort's scratch
+constexpr size_t kNoAllocBytes = kNoAllocFlag + ((3 * sizeof(uint) * kNoAllocMaxN + 255) & ~(size_t)255) + sizeof(uint) * 256 * 256;
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices];
+}  // namespace
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// Device address of `scratch` on the current device (looked up on the first use per device).
+static void* rx_noalloc_scratch() {
+    int dev = 0;
+    if (!RX_HIP(hipGetDevice(&dev))) return nullptr;
+    if (dev < 0 || dev >= kMaxDevices) {
+        rx_set_error(1, "CuSortNoAlloc: device %d is beyond the %d devices the library tracks", dev, kMaxDevices);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            rx_set_error((int)e, "CuSortNoAlloc: hipGetSymbolAddress of the static scratch failed: %s", hipGetErrorString(e));
+            return nullptr;
+        }
+        if (rx_sort_scratch_bytes(kNoAllocMaxN) + kNoAllocFlag > kNoAllocBytes) {  // guards the size formula above
+            rx_set_error(1, "CuSortNoAlloc: static scratch smaller than the sort needs");
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+void CuSortNoAlloc(unsigned int* dHash, uint* dIndex, uint numParticles) {
+    if (numParticles == 0) return;
+    if ((size_t)numParticles > kNoAllocMaxN) {
+        rx_set_error(1, "CuSortNoAlloc: %u keys exceed the static scratch (at most %zu)", numParticles, kNoAllocMaxN);
+        return;
+    }
+    if (!g_params_set) {
+        rx_set_error(3, "CuSortNoAlloc: CuSetParameters must be called first (the bucket count is NumCells)");
+        return;
+    }
+    char* base = (char*)rx_noalloc_scratch();
+    if (!base) return;
+    hipStream_t s = rx_stream();
+    const uint nc = std::max(g_params.NumCells, g_params.NumCellsB);
+    rx_fill<int>((int*)base, 0, 1, s);  // a fill kernel, not a memset (the flag starts at zero on every call)
+    rx_sort_launch(dHash, dIndex, numParticles, nc, base + kNoAllocFlag, (int*)base, s);
+    RX_HIP(hipGetLastError());
+    RX_HIP(hipStreamSynchronize(s));
 }
 
 void CuSPHInit(int max_particles) {

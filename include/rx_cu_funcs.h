@@ -52,6 +52,13 @@ void CuScan(unsigned int* output, unsigned int* input, unsigned int numElements)
  * (fluid or boundary-particle grid), or the 0x7FFFFFFF marker that CuCalcHash writes for attr == -1
  * particles; any other key sets an error. */
 void CuSort(unsigned int* dHash, uint* dIndex, uint numParticles);
+/* No counterpart in the reference: CuSort for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same sort and result as CuSort, with its scratch in a
+ * static device array of about 24 MB per GPU (held until the process ends) instead of library-allocated memory;
+ * at most 2^21 (2,097,152) keys per call, larger counts are rejected with error 1 before any work.  Every call on
+ * a GPU shares that GPU's array, so calls that overlap on one GPU race.  Keys that are neither cell hashes nor
+ * 0x7FFFFFFF sort as 0x7FFFFFFF as in CuSort, but are not reported (reporting would need a host copy). */
+void CuSortNoAlloc(unsigned int* dHash, uint* dIndex, uint numParticles);
 /* Exclusive prefix sum of numElements floats (rx_cu_funcs.cuh:48). */
 void CuScanf(float* output, float* input, unsigned int numElements);
 
