@@ -124,14 +124,11 @@ enum rxPbfOption {
                                       lists overflow.  The host reads each step's figures once the device has
                                       finished it, without waiting, so the sizes follow with a lag.  Eager steps
                                       only (replayed graphs keep their sizes). */
-    RX_PBF_OPT_GRAD_MASS = 19,     /* 1: the constraint gradients carry the particle mass, grad C_i =
+    RX_PBF_OPT_GRAD_MASS = 19      /* 1: the constraint gradients carry the particle mass, grad C_i =
                                       (m / rho0) sum_j grad W_ij, in the multiplier's denominator and in the
                                       position correction (Macklin & Mueller 2013, section 3, with rho = sum m W).
                                       0 (default): the reference's grad W / rho0 (rx_sph_pbd.cpp:782-786,
                                       rx_sph_pbd.cpp:970-973), which scales each correction by m */
-    RX_PBF_OPT_MEMORY = 20         /* gamma in [0, 1] (default 0): the first correction of a step adds gamma times
-                                      the pressure part, min(L, 0), of the multiplier L that the particle's
-                                      previous step applied in total (a leaky warm start from the last step) */
 };
 
 /* Kernel classes of rxPBDSPH_GPU_GetProfile. */
