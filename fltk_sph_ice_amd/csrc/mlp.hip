@@ -33,9 +33,10 @@ __device__ inline unsigned short f2bf(float f) {  // round to nearest even
     return (unsigned short)(u >> 16);
 }
 __device__ inline float bf2f(unsigned short b) { return __uint_as_float((uint)b << 16); }
-// SiLU with the hardware exp2 and reciprocal (one v_exp_f32, one v_rcp_f32): the activations are rounded to
-// bf16 for the next layer, far coarser than the few-ulp error of the fast division.
-__device__ inline float silu(float z) { return __fdividef(z, 1.0f + __expf(-z)); }
+using mlpd::silu;
+using mlpd::mfma;
+using mlpd::ld_bf4;
+using mlpd::pack4_hw;
 __device__ inline float dsilu(float z) {
     float s = __fdividef(1.0f, 1.0f + __expf(-z));
     return s * (1.0f + z * (1.0f - s));
@@ -49,16 +50,6 @@ __device__ inline bf16x4 pack4(float a, float b, float c, float d) {
     r[3] = (short)f2bf(d);
     return r;
 }
-__device__ inline bf16x4 ld_bf4(const unsigned short* p) {  // 4 consecutive bf16 (8-byte aligned)
-    uint2 u = *(const uint2*)p;
-    bf16x4 r;
-    r[0] = (short)(u.x & 0xffffu);
-    r[1] = (short)(u.x >> 16);
-    r[2] = (short)(u.y & 0xffffu);
-    r[3] = (short)(u.y >> 16);
-    return r;
-}
-__device__ inline f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
 
 // Standardised input fragment of 16 rows starting at r0 (A layout).
 __device__ inline bf16x4 load_x(const float* __restrict__ X, uint r0, uint n, const MlpDev& m) {
@@ -126,17 +117,6 @@ __device__ inline f32x4 forward16(bf16x4 a1, const MlpDev& m, unsigned short* ld
     return p;
 }
 
-// Four floats to bf16 with the hardware's round-to-nearest-even conversion (two v_cvt_pk_bf16_f32); the same
-// rounding as f2bf.
-typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
-typedef float f32x2_t __attribute__((ext_vector_type(2)));
-typedef unsigned u32x2_t __attribute__((ext_vector_type(2)));
-__device__ inline bf16x4 pack4_hw(float a, float b, float c, float d) {
-    const f32x2_t lo = {a, b}, hi = {c, d};
-    const u32x2_t u = {__builtin_bit_cast(unsigned, __builtin_convertvector(lo, bf16x2_t)),
-                       __builtin_bit_cast(unsigned, __builtin_convertvector(hi, bf16x2_t))};
-    return __builtin_bit_cast(bf16x4, u);
-}
 
 // Inference in transposed form: with the weights as the A operand and the activations as the B operand,
 // H^T = W X^T leaves lane l holding H[row l % 16][units 4 (l / 16) .. + 3] of a 16-unit tile, which is the B

@@ -368,12 +368,16 @@ __global__ void k_fold_run(float* __restrict__ stats, PbfK k) {
 // z-plane pl (0, 1, 2 = z - 1, z, z + 1) inside the grid it calls plane(pl, base, span) with the first slot of the
 // plane's first row and the slots up to the end of its last row, then f(j, xs[j], base - (pl << 14)) for the
 // slots j of its rows in ascending order.
+// xrow (may be null): the features are also written, standardised as bf16, to this LDS row of the network
+// staged in snet (the fused inference).
 template <class Cands>
 __device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, const float4* __restrict__ xs, float4* __restrict__ xl,
                                         float4* __restrict__ xbuild,
                                         float* __restrict__ dens, uint* __restrict__ list, uint4* __restrict__ cnt,
                                         float* __restrict__ stats, const PbfK& k, float* __restrict__ feat,
-                                        const float4* __restrict__ v, float& emax, float& esum) {
+                                        const float4* __restrict__ v, float& emax, float& esum, unsigned short* xrow = nullptr,
+                                        const MlpShared* snet = nullptr) {
+    const bool want_feat = feat || xrow;
     float3 p = f3(p4);
     float d = 0.0f, sd = 0.0f;
     float3 gs = f3(0, 0, 0);
@@ -417,7 +421,7 @@ __device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, const 
                     gs += g;
                     sd += dot3(g, g);
                 }
-                if (feat) {
+                if (want_feat) {
                     float cj = q4.w * k.s.inv_rho0 - 1.0f;
                     nn += 1.0f;
                     wsum += w;
@@ -457,7 +461,7 @@ __device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, const 
     float err = fmaxf(C, 0.0f);
     emax = fmaxf(emax, err);
     esum += err;
-    if (feat) {
+    if (want_feat) {
         float wall = fminf(fminf(fminf(p.x - k.s.wall_lo.x, k.s.wall_hi.x - p.x), fminf(p.y - k.s.wall_lo.y, k.s.wall_hi.y - p.y)),
                            fminf(p.z - k.s.wall_lo.z, k.s.wall_hi.z - p.z));
         float f[12];
@@ -473,11 +477,14 @@ __device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, const 
         f[9] = (p4.w - d) / d;  // div v dt from the continuity equation: -(rho* - rho_start) / rho*
         f[10] = fminf(fmaxf(wall * k.inv_h, 0.0f), 1.0f) - 0.5f;
         f[11] = v[i].y * k.dt * k.inv_spacing;
-        float4* fo = (float4*)(feat + (size_t)i * 16);
-        fo[0] = make_float4(f[0], f[1], f[2], f[3]);
-        fo[1] = make_float4(f[4], f[5], f[6], f[7]);
-        fo[2] = make_float4(f[8], f[9], f[10], f[11]);
-        fo[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (feat) {
+            float4* fo = (float4*)(feat + (size_t)i * 16);
+            fo[0] = make_float4(f[0], f[1], f[2], f[3]);
+            fo[1] = make_float4(f[4], f[5], f[6], f[7]);
+            fo[2] = make_float4(f[8], f[9], f[10], f[11]);
+            fo[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        if (xrow) mlp_write_row(f, xrow, *snet);
     }
 }
 
@@ -514,18 +521,34 @@ __device__ inline void build_candidates(uint i, const PbfK& k, const float4* __r
 
 // One thread per slot, in slot order (the default): the 27-cell walk reads candidates from global memory; the
 // state is in cell order, so a wavefront's candidates are few distinct cells and mostly L1/L2 hits.
-__global__ __launch_bounds__(kTpb) void k_list_lambda_slots(const float4* __restrict__ xs, float4* __restrict__ xl,
+// kNet (learned warm start): the warm-start network runs on the features while they are in registers (bf16 MFMA,
+// the network staged in LDS) and its multiplier, times w_scale, replaces lambda_0 as the first multiplier.
+template <bool kNet>
+__global__ __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(6))) void k_list_lambda_slots(const float4* __restrict__ xs, float4* __restrict__ xl,
                                                             float4* __restrict__ xbuild, float* __restrict__ dens, uint* __restrict__ list,
                                                             uint4* __restrict__ cnt, float* __restrict__ stats, PbfK k,
                                                             const uint* __restrict__ bhash, const uint* __restrict__ off,
-                                                            float* __restrict__ feat, const float4* __restrict__ v) {
+                                                            float* __restrict__ feat, const float4* __restrict__ v, MlpDev net,
+                                                            float w_scale) {
+    __shared__ MlpShared s_net[1];
+    __shared__ __align__(16) unsigned short s_rows[kNet ? kTpb * kMlpIn : 1];
+    if (kNet) {
+        mlp_stage(net, s_net[0]);
+        __syncthreads();
+    }
     bool valid;
     uint i = slot_of(k, valid);
     float emax = 0.0f, esum = 0.0f;
+    unsigned short* xrow = kNet ? s_rows + threadIdx.x * kMlpIn : nullptr;
     if (valid) {
         const float4 p4 = xs[i];
         auto cands = [&](auto&& plane, auto&& f) { build_candidates(i, k, xs, bhash, off, plane, f); };
-        list_lambda_slot(i, p4, cands, xs, xl, xbuild, dens, list, cnt, stats, k, feat, v, emax, esum);
+        list_lambda_slot(i, p4, cands, xs, xl, xbuild, dens, list, cnt, stats, k, feat, v, emax, esum, xrow, s_net);
+    }
+    if (kNet) {  // every lane of the wavefront takes part (rows of lanes past the end are zero; their outputs unused)
+        if (!valid) ((uint4*)xrow)[0] = ((uint4*)xrow)[1] = make_uint4(0u, 0u, 0u, 0u);
+        const float y = mlp_predict_rows(s_rows + (threadIdx.x & ~63u) * kMlpIn, s_net[0], net);
+        if (valid) xl[i].w = y * w_scale;
     }
     block_max_atomic(emax, &stats[ST_ERR_ITER + 0]);
     block_sum_atomic(esum, &stats[ST_MPOS_ITER + 0]);
@@ -1286,8 +1309,12 @@ int rxPBDSPH_GPU::enqueue_step(float dt) {
     end_mark(mk);
 
     // ---- list + lambda_0 (the network's multiplier in learned mode), then up to K corrections
-    const bool feats = (m_mode == RX_PBF_MODE_LEARNED) || m_record_features;
-    mk = begin_mark(feats ? RX_PROF_FEATURES : RX_PROF_LIST_LAMBDA);
+    const bool learned = m_mode == RX_PBF_MODE_LEARNED;
+    // the default list pass runs the network itself; the LDS-tiled variants write the features for a separate
+    // inference launch
+    const bool fused_net = learned && m_list_tile == 0;
+    const bool feats = (learned && !fused_net) || m_record_features;
+    mk = begin_mark(learned || m_record_features ? RX_PROF_FEATURES : RX_PROF_LIST_LAMBDA);
     {
         float* f = feats ? m_feat : nullptr;
         const FGridK& gk = m_grid.k;
@@ -1309,11 +1336,16 @@ int rxPBDSPH_GPU::enqueue_step(float dt) {
                                                                           m_grid.hash, m_grid.off, f, m_v);
             break;
         default:
-            k_list_lambda_slots<<<nb, kTpb, 0, s>>>(m_xs, m_xl, m_xb, m_dens, m_list, m_cnt, m_stats, k, m_grid.hash, m_grid.off, f, m_v);
+            if (fused_net)
+                k_list_lambda_slots<true><<<nb, kTpb, 0, s>>>(m_xs, m_xl, m_xb, m_dens, m_list, m_cnt, m_stats, k, m_grid.hash, m_grid.off, f,
+                                                              m_v, m_mlp, m_warm_scale / k.sd_ref);
+            else
+                k_list_lambda_slots<false><<<nb, kTpb, 0, s>>>(m_xs, m_xl, m_xb, m_dens, m_list, m_cnt, m_stats, k, m_grid.hash, m_grid.off,
+                                                               f, m_v, MlpDev{}, 0.0f);
         }
     }
     end_mark(mk);
-    if (m_mode == RX_PBF_MODE_LEARNED) {
+    if (learned && !fused_net) {
         mk = begin_mark(RX_PROF_MLP);
         rx_mlp_infer_to_w(m_feat, m_xl, n, m_mlp, m_warm_scale / k.sd_ref, s);
         end_mark(mk);

@@ -70,3 +70,26 @@ def test_learned_start_beats_plain_at_equal_corrections(net):
     matched = next((K for K in (4, 6, 8, 12) if res[f"plain{K}"]["mean"] <= res["learned4"]["mean"]), None)
     print(f"plain corrections needed to match learned4: {matched}")
     assert matched is None or matched > 4
+
+
+def test_fused_inference_equals_the_network_forward():
+    # The list pass runs the network on each particle's features while they are on chip; with one correction
+    # per step the multiplier a step applies is that output times warm_scale / sd_ref.  The recorded features
+    # through rxMLP_Forward (the standalone inference kernel) must give the same multipliers.
+    rng = np.random.default_rng(4)
+    sc = scene.pbf_dam_break(24, 20, 16)
+    net = learned.Network(seed=3)
+    net.set_normalization(rng.normal(0, 0.1, 12).astype(np.float32), rng.uniform(0.5, 2.0, 12).astype(np.float32), 0.7, -0.2)
+    s = learned.learned_solver(sc, net, max_iters=1, tol=0.0, warm_scale=0.5)
+    s.option(_abi.RX_PBF_OPT_RECORD_FEATURES, 1)
+    s.step(2)
+    F = np.zeros((s.n, learned.NIN), np.float32)
+    assert s.lib.rxPBDSPH_GPU_GetArray(s.h, _abi.RX_ARRAY_FEATURES, F.ctypes.data, 16 * s.n) == 0
+    lam = s.array(_abi.RX_ARRAY_LAMBDA_SUM)
+    sd_ref, _ = learned.feature_scales(s)
+    expect = net.forward(F).astype(np.float64) * np.float32(0.5 / sd_ref)
+    err = np.abs(lam - expect).max() / np.abs(expect).max()
+    print(f"fused vs standalone inference: max relative difference {err:.2e}")
+    assert err <= 1e-6
+    s.close()
+    net.close()
