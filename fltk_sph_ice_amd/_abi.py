@@ -246,6 +246,7 @@ SIGNATURES: dict[str, tuple] = {
     "rxCoupled_SetOption": (i32, [vp, i32, flt]),
     "rxCoupled_SetIceTiles": (i32, [vp, i32, vp]),
     "rxCoupled_GetProfile": (i32, [vp, vp, vp, i32]),
+    "rxCoupled_SetNetwork": (i32, [vp, vp]),
     "rxSPH_GPU_Create": (vp, [C.POINTER(rxSphEnv)]),
     "rxSPH_GPU_Destroy": (None, [vp]),
     "rxSPH_GPU_SetParticles": (i32, [vp, vp, vp, i32]),

@@ -107,7 +107,8 @@ public:
             return 1;
         }
         for (int t = 0; t < steps; ++t) {
-            if (m_pbf->Update(dt, 1)) return 1;
+            const int rc = m_pbf->Update(dt, 1);
+            if (rc) return rc;
             rxPBDSPH_GPU::DeviceView d = m_pbf->device_view();
             uint n = m_pbf->size();
             unsigned nb = rx_blocks(n, 256);
@@ -163,6 +164,7 @@ int rxCoupled_GetArray(void* h, int type, float* host, int count) { return h ? (
 int rxCoupled_GetStats(void* h, float* out, int count) { return h ? ((rxCoupledGPU*)h)->m_pbf->GetStats(out, count) : 1; }
 int rxCoupled_SetOption(void* h, int option, float value) { return h ? ((rxCoupledGPU*)h)->m_pbf->SetOption(option, value) : 1; }
 int rxCoupled_GetProfile(void* h, float* ms, int* count, int n) { return h ? ((rxCoupledGPU*)h)->m_pbf->GetProfile(ms, count, n) : 1; }
+int rxCoupled_SetNetwork(void* h, void* mlp) { return h ? ((rxCoupledGPU*)h)->m_pbf->SetNetwork(mlp) : 1; }
 int rxCoupled_SetIceTiles(void* h, int autotune, int* out_tiles) {
     if (!h) return 1;
     rxCoupledGPU* c = (rxCoupledGPU*)h;

@@ -263,6 +263,16 @@ class Coupled:
         if self.lib.rxCoupled_SetOption(self.h, opt, value):
             check(self.lib)
 
+    def set_network(self, net, warm_scale: float = 0.5, max_iters: int = 4, tol: float = 0.01):
+        """The fluid's learned warm start (the network's weights are copied)."""
+        if self.lib.rxCoupled_SetNetwork(self.h, net.h):
+            check(self.lib)
+        self.option(_abi.RX_PBF_OPT_WARM_SCALE, warm_scale)
+        self.option(_abi.RX_PBF_OPT_MODE, _abi.RX_PBF_MODE_LEARNED)
+        self.option(_abi.RX_PBF_OPT_ITERATIONS, max_iters)
+        self.option(_abi.RX_PBF_OPT_ADAPTIVE, 1)
+        self.option(_abi.RX_PBF_OPT_TOL, tol)
+
     def autotune(self):
         t = np.zeros(5, np.int32)
         if self.lib.rxCoupled_SetIceTiles(self.h, 1, t.ctypes.data):

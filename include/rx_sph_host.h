@@ -222,6 +222,10 @@ int rxCoupled_GetStats(void* coupled, float* out, int count);
 int rxCoupled_SetOption(void* coupled, int option, float value);
 int rxCoupled_SetIceTiles(void* coupled, int autotune, int* out_tiles);
 int rxCoupled_GetProfile(void* coupled, float* ms, int* count, int n);
+/* Network of the fluid's learned warm start (as rxPBDSPH_GPU_SetNetwork; weights copied): with
+ * rxCoupled_SetOption(RX_PBF_OPT_MODE, RX_PBF_MODE_LEARNED) every PBF step of the coupled scene starts from the
+ * network's multiplier (the ice particles take part in the fluid solve as in the plain mode). */
+int rxCoupled_SetNetwork(void* coupled, void* mlp);
 
 #ifdef __cplusplus
 }

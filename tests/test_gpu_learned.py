@@ -93,3 +93,68 @@ def test_fused_inference_equals_the_network_forward():
     assert err <= 1e-6
     s.close()
     net.close()
+
+
+def _judged(s, steps):
+    s.option(_abi.RX_PBF_OPT_RESET_RUN, 1)
+    s.step(steps)
+    corr = float(s.stats()[50]) / steps
+    s.step(1)  # its prediction pass judges the last step
+    st = s.stats()
+    return {"max": float(st[48]), "mean": float(st[49]), "corrections": corr, "bad": float(st[53]), "walks": float(st[20])}
+
+
+def test_learned_solve_at_one_million_particles(net):
+    # C4 at its size: the 128 x 128 x 64 dam break (1,048,576 particles), 20 judged steps after 5 warm-up steps,
+    # learned (up to 4 corrections, 1% max-error judge) beside plain PBF at 4 corrections from the same state
+    sc = scene.pbf_1m()
+    pos, vel = sc.particles()
+    assert sc.n == 1 << 20
+    res = {}
+    lrn = learned.learned_solver(sc, net, 4, 0.01, pos=pos, vel=vel, warm_scale=0.5)
+    lrn.step(5)
+    res["learned"] = _judged(lrn, 20)
+    p = lrn.positions()
+    assert np.isfinite(p).all()
+    lrn.close()
+    pl = PBFSolver(sc, pos, vel)
+    pl.option(_abi.RX_PBF_OPT_ITERATIONS, 4)
+    pl.step(5)
+    res["plain"] = _judged(pl, 20)
+    pl.close()
+    for k, v in res.items():
+        print(f"1M {k:8s} worst max {v['max']:.4f} worst mean {v['mean']:.4f} corrections/step {v['corrections']:.2f}")
+    assert res["learned"]["bad"] == 0 and res["plain"]["bad"] == 0  # no neighbour walk beyond its reach
+    assert res["learned"]["corrections"] == pytest.approx(4.0)  # the 1% judge is never met here: all 4 run
+    assert res["learned"]["mean"] <= res["plain"]["mean"] and res["learned"]["max"] <= res["plain"]["max"]
+
+
+def test_coupled_scene_with_the_learned_fluid(net):
+    # C5's fluid with the learned warm start: the coupled step runs, melting proceeds, and the judged density
+    # error stays at or below the plain solve's at equal corrections
+    from fltk_sph_ice_amd.solvers import Coupled
+
+    res = {}
+    for mode in ("plain", "learned"):
+        cs = scene.coupled_melt((24, 16, 16), (12, 16, 12), seed=4)
+        sp = cs.fluid.spacing
+        cs.ice_origin = (cs.fluid.block[0] * sp + 1.5 * sp, cs.ice_origin[1], cs.ice_origin[2])
+        c = Coupled(cs)
+        if mode == "learned":
+            c.set_network(net, warm_scale=0.5, max_iters=4, tol=0.01)
+        else:
+            c.option(_abi.RX_PBF_OPT_ITERATIONS, 4)
+        c.step(5)
+        c.option(_abi.RX_PBF_OPT_RESET_RUN, 1)
+        c.step(30)
+        c.step(1)
+        st = c.stats()
+        p = c.array(_abi.RX_ARRAY_POSITION)
+        ph = c.array(_abi.RX_ARRAY_PHASE)
+        c.close()
+        assert np.isfinite(p).all()
+        assert st[53] == 0
+        res[mode] = (float(st[48]), float(st[49]), int((ph[cs.n_fluid:] != -2).sum()))
+        print(f"coupled {mode:8s} worst max {st[48]:.4f} worst mean {st[49]:.4f} ice particles no longer frozen {res[mode][2]}")
+    assert res["learned"][2] > 0  # heat crossed into the ice in learned mode too
+    assert res["learned"][1] <= res["plain"][1]
