@@ -96,6 +96,7 @@ private:
     uint* m_list = nullptr;     // skin neighbour list: uint4 chunks of 8 16-bit entries, [chunk][slot]
     uint4* m_cnt = nullptr;     // list head per slot: 3 plane bases + count (bit 31: use the grid instead)
     float* m_stats = nullptr;
+    float4* m_movers = nullptr;  // particles that moved more than skin/2 since the list build (bounded list)
     // profiling
     bool m_profile = false;
     unsigned m_profile_mask = ~0u;  // rxProfileClass bits timed while profiling
@@ -136,7 +137,7 @@ private:
     // reads the records whose events have completed, without waiting, and sets the skin level and list
     // capacity the next list build uses.
     static constexpr int kRecs = 8;
-    static constexpr int kRecLen = 6;  // stats slots copied per record (ST_DISP2 .. ST_BAD)
+    static constexpr int kRecLen = 14;  // stats slots copied per record (ST_DISP2 .. ST_CROSS + 3)
     float* m_hrec = nullptr;
     long m_bad_steps = 0, m_bad_reported = 0;  // completed steps whose fallback reach was capped, and those reported
     hipEvent_t m_rec_ev[kRecs] = {};

@@ -8,9 +8,11 @@
 //   velocity     v = (x* - x_n) / dt
 //   XSPH         v += c sum (m/rho_j)(v_j - v_i) W (list)
 // Every list pass reads the largest displacement since the list was built (kept on the device); while it is
-// within skin/2 the list holds every pair within h, otherwise the pass walks a 5x5x5 cell stencil of the
-// build-time grid instead, so the result never depends on the list being fresh.  No host synchronisation
-// happens inside a step.
+// within skin/2 the list holds every pair within h.  Past that, the few particles that moved more than skin/2
+// ("movers", listed by the correction pass) walk the build-time grid, and every other particle walks its list
+// plus the nearby movers its list lacks; with too many movers every particle bounds displacements by the
+// largest one.  The result never depends on the list being fresh.  No host synchronisation happens inside a
+// step.
 #include <stdio.h>
 #include <string.h>
 
@@ -33,6 +35,8 @@ struct PbfK {
     float wide2;      // largest displacement^2 the capped fallback walk still covers
     uint* fallback;   // counts the passes of particles that walked the grid instead of their list
     uint* bad;        // stats slots ST_BAD and ST_RUN_BAD: set when a walk's reach was capped (pairs missed)
+    float4* movers;   // (build position, slot bits) of the particles that moved more than skin/2 since the build
+    float cross2[4];  // (skin of level l / 2)^2: the crossings the skin sizing counts
     int maxn;
     float eps;
     float gfac;       // constraint-gradient factor: spiky gradient coefficient / rho0, times m in the mass mode
@@ -53,6 +57,8 @@ namespace {
 
 constexpr int kTpb = 256;
 constexpr int kMaxStencil = 4;  // largest cell radius of a fallback walk
+constexpr uint kMoverCap = 2048;  // movers listed per list build; beyond it every pass bounds displacements by the largest
+constexpr uint kMoverNear = 64;  // movers one workgroup stages; beyond it its particles bound displacements by the largest
 
 // Per-step statistics slots (float/uint in one array).
 enum Stat {
@@ -66,6 +72,9 @@ enum Stat {
     ST_BAD = 22,             // displacement beyond the wide stencil's reach
     ST_MEAN_ABS_START = 23,  // sum |rho - rho0| / rho0 at the step start (divide by N)
     ST_MPOS_START = 24,      // sum (rho - rho0)+ / rho0 at the step start (divide by N)
+    ST_MOVERS = 25,          // movers listed since the list build (uint; kept until the next build)
+    ST_MOVERS_PUB = 26,      // ST_MOVERS as of the last lambda pass: the movers a correction pass may read
+    ST_CROSS = 27,           // .. 27 + l (l < 4): corrections that carried a particle past half the skin of level l
     ST_MPOS_ITER = 32,       // .. 32 + k: the same at the start of Jacobi iteration k (k < 16)
     ST_RUN = 48,             // run accumulators from here on, kept across steps:
     ST_RUN_MAX = 48,         //   max over the judged steps of the max error
@@ -142,14 +151,6 @@ __device__ inline float3 clamp_wall(float3 x, const SphK& s) {
     return x;
 }
 
-// Neighbour candidates of slot i for a pass that needs every j with |x_i - x_j| <= h at the current positions.
-// mode 0: the skin list; mode 1: the 3x3x3 stencil of a grid built on the current positions (cell >= h).
-// A pair within h now was within L = h + d_i + D of each other at the list build (d = displacement since the
-// build, D = its maximum over all particles), so the list of i holds every such j while d_i + D <= skin.
-// Otherwise (or when the list overflowed) the pass walks the build-time grid over ceil(L / cell) cells in
-// every direction, which holds every such j; the radius is capped at kMaxStencil cells (displacements beyond
-// that raise the stats flag checked by the prediction pass).  `p` is the current position of i, `xb` the
-// build positions.
 // Walks the cells within `rad` of cell c (rows of contiguous slots), fetching four candidates ahead of their use:
 // the loads of a group are issued together, so a wavefront keeps several gathers in flight.
 // The box of cells [lo, hi] (inclusive, inside the grid).
@@ -179,15 +180,6 @@ __device__ inline void stencil_batched(int3 c, const FGridK& g, const uint* __re
                 make_int3(min(c.x + rad, g.gx - 1), min(c.y + rad, g.gy - 1), min(c.z + rad, g.gz - 1)), g, off, fetch, body);
 }
 
-// Neighbour candidates of slot i for a pass that needs every j with |x_i - x_j| <= h at the current positions;
-// `fetch(j)` loads what the pass needs of j, `body(j, data)` consumes it.
-// mode 0: the skin list; mode 1: the 3x3x3 stencil of a grid built on the current positions (cell >= h).
-// A pair within h now was within L = h + d_i + D of each other at the list build (d = displacement since the
-// build, D = its maximum over all particles), so the list of i holds every such j while d_i + D <= skin.
-// Otherwise (or when the list overflowed) the pass walks the build-time grid over ceil(L / cell) cells in
-// every direction, which holds every such j; the radius is capped at kMaxStencil cells (displacements beyond
-// that raise the stats flag checked by the prediction pass).  `p` is the current position of i, `xb` the
-// build positions.
 // List format.  The candidates of slot i come from three z-planes of cells (z - 1, z, z + 1 around its cell); within
 // a plane they lie in [b_p, e_p), b_p = the first slot of the plane's first row.  An entry is 16 bits: the plane
 // p in bits 14-15 and j - b_p in bits 0-13.  Eight entries make a uint4 chunk; chunk q of slot i is at
@@ -268,24 +260,127 @@ __device__ inline void list_walk(const uint4* __restrict__ l4, const uint4& hd, 
     }
 }
 
+// Squared length with a fixed rounding: the list build and the mover test below both decide "j is in the list
+// of i" with it, so they agree bit for bit.
+__device__ inline float len2_fixed(float3 r) { return __builtin_fmaf(r.z, r.z, __builtin_fmaf(r.y, r.y, r.x * r.x)); }
+
+// Movers: particles whose displacement since the list build exceeded skin/2 at some correction this step.  The
+// correction pass lists each once (build position, slot) the first time it crosses; they stay listed until the
+// next build.  The movers a workgroup's particles may meet are staged in LDS: n = -1 when the workgroup bounds
+// every displacement by the largest one instead (too many movers in the step, or near the workgroup).
+struct NearMovers {
+    const float4* x;   // build position, slot bits
+    const uint* cell;  // build cell hash
+    int n;
+};
+
+// Call from every thread of the workgroup (the result is the same for all).  A particle i that has moved at most
+// skin/2 meets a mover j only if |xb_i - xb_j| <= h + skin/2 + D, so the movers within that reach of the box of
+// the workgroup's build positions are staged.  `published`: read the movers as of the last lambda pass (a
+// correction pass appends to the list while it runs).
+__device__ inline NearMovers stage_movers(uint i, bool valid, const PbfK& k, float disp2, const float* __restrict__ stats,
+                                          const float4* __restrict__ xb, const uint* __restrict__ bhash, bool published) {
+    __shared__ float4 s_x[kMoverNear];
+    __shared__ uint s_cell[kMoverNear];
+    __shared__ float s_box[6][kTpb / 64];
+    __shared__ uint s_n;
+    NearMovers nm{s_x, s_cell, 0};
+    if (!(disp2 > k.half_skin2)) return nm;  // no particle has moved more than skin/2: the lists hold every pair
+    const uint nmov = __float_as_uint(stats[published ? ST_MOVERS_PUB : ST_MOVERS]);
+    if (nmov > kMoverCap) {
+        nm.n = -1;
+        return nm;
+    }
+    if (nmov == 0) return nm;
+    float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    if (valid) {
+        const float4 q = xb[i];
+        b[0] = b[3] = q.x, b[1] = b[4] = q.y, b[2] = b[5] = q.z;
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            b[c] = fminf(b[c], __shfl_xor(b[c], d, 64));
+            b[c + 3] = fmaxf(b[c + 3], __shfl_xor(b[c + 3], d, 64));
+        }
+    }
+    if ((threadIdx.x & 63) == 0)
+        for (int c = 0; c < 6; ++c) s_box[c][threadIdx.x >> 6] = b[c];
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+        for (int c = 0; c < 3; ++c) {
+            b[c] = fminf(b[c], s_box[c][w]);
+            b[c + 3] = fmaxf(b[c + 3], s_box[c + 3][w]);
+        }
+    const float R = k.s.h + 0.5f * k.skin + sqrtf(disp2);
+    for (uint m = threadIdx.x; m < nmov; m += blockDim.x) {
+        const float4 mv = k.movers[m];
+        if (mv.x >= b[0] - R && mv.x <= b[3] + R && mv.y >= b[1] - R && mv.y <= b[4] + R && mv.z >= b[2] - R && mv.z <= b[5] + R) {
+            const uint t = atomicAdd(&s_n, 1u);
+            if (t < kMoverNear) {
+                s_x[t] = mv;
+                s_cell[t] = bhash[min(__float_as_uint(mv.w), k.n - 1u)];
+            }
+        }
+    }
+    __syncthreads();
+    nm.n = s_n > kMoverNear ? -1 : (int)s_n;
+    return nm;
+}
+
+// Neighbour candidates of slot i for a pass that needs every j with |x_i - x_j| <= h at the current positions;
+// `fetch(j)` loads what the pass needs of j, `body(j, data)` consumes it.  `p` is the current position of i,
+// `xb` the build positions, d = displacement since the build, D = its largest value (disp2 = D^2).
+// mode 1: the 3x3x3 stencil of a grid built on the current positions (cell >= h).
+// mode 0: the skin list, which holds every j with |xb_i - xb_j| <= h + skin.  A pair within h now was within
+// h + d_i + d_j at the build, so the list of i alone holds every such j while d_i + D <= skin (always so while
+// D <= skin/2).  With movers staged (nm.n >= 0) a particle with d_i <= skin/2 walks its list, which holds every
+// j that has stayed within skin/2, and then the staged movers its list does not hold (the build's own test on
+// the build positions and cells decides that, so no pair is visited twice).  Every other particle (and every
+// overflowed list) walks the build-time grid over the cells within L = h + d_i + D of its build position, which
+// hold every such j; the radius is capped at kMaxStencil cells (beyond it the step and the run are flagged).
 template <int G = 1, class Fetch, class Body>
 __device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint* __restrict__ list, const uint4* __restrict__ cnt,
                                      const uint* __restrict__ bhash, const uint* __restrict__ off, float disp2, float3 p,
-                                     const float4* __restrict__ xb, Fetch&& fetch, Body&& body) {
+                                     const float4* __restrict__ xb, const NearMovers& nm, Fetch&& fetch, Body&& body) {
     if (mode == 1) {
         stencil_batched(fg_unhash(bhash[i], k.g), k.g, off, fetch, body, 1);
         return;
     }
     const uint4 hd = cnt[i];
     bool ok = hd.w <= (uint)k.maxn;
+    bool movers = false;
     float di = 0.0f;
     if (disp2 > k.half_skin2) {
-        float3 dd = p - f3(xb[i]);
-        di = sqrtf(dot3(dd, dd));
-        ok = ok && di + sqrtf(disp2) <= k.skin;
+        const float di2 = len2_fixed(p - f3(xb[i]));
+        di = sqrtf(di2);
+        if (nm.n >= 0) {
+            ok = ok && di2 <= k.half_skin2;
+            movers = ok && nm.n > 0;
+        } else {
+            ok = ok && di + sqrtf(disp2) <= k.skin;
+        }
     }
     if (ok) {
         list_walk<G>((const uint4*)list + i, hd, k.n, fetch, body);
+        if (movers) {
+            const float3 bi = f3(xb[i]);
+            const int3 ci = fg_unhash(bhash[i], k.g);
+            const float R = k.s.h + di + sqrtf(disp2);
+            for (int m = 0; m < nm.n; ++m) {
+                const float4 mv = nm.x[m];
+                const float r2 = len2_fixed(bi - f3(mv));
+                if (r2 > R * R) continue;  // out of reach now
+                const int3 cj = fg_unhash(nm.cell[m], k.g);
+                const bool listed = abs(cj.x - ci.x) <= 1 && abs(cj.y - ci.y) <= 1 && abs(cj.z - ci.z) <= 1 && r2 <= k.rc2;
+                if (!listed) {
+                    const uint j = __float_as_uint(mv.w);
+                    body(j, fetch(j));
+                }
+            }
+        }
     } else {
         // every j within L of i's build position: the cells that the box [x_b - L, x_b + L] touches (3 or 4 per
         // axis once L exceeds a cell, not the 5 of a cube of radius 2 around the cell)
@@ -317,13 +412,14 @@ __global__ __launch_bounds__(kTpb) void k_predict(const float4* __restrict__ x, 
     bool valid;
     uint i = slot_of(k, valid);
     float disp2 = stats[ST_DISP2];
+    const NearMovers nm = mode == 0 ? stage_movers(i, valid, k, disp2, stats, xb, bhash, false) : NearMovers{nullptr, nullptr, 0};
     float err = 0.0f, aerr = 0.0f;
     if (valid) {
         float3 p = f3(x[i]), vi = f3(v[i]);
         float d = 0.0f;
         float3 fv = f3(0, 0, 0);
         for_neighbors<1>(
-            i, mode, k, list, cnt, bhash, off, disp2, p, xb, [&](uint j) { return XV{x[j], v[j]}; },
+            i, mode, k, list, cnt, bhash, off, disp2, p, xb, nm, [&](uint j) { return XV{x[j], v[j]}; },
             [&](uint, const XV& q) {
                 float3 r = p - f3(q.x);
                 float r2 = dot3(r, r);
@@ -346,6 +442,12 @@ __global__ __launch_bounds__(kTpb) void k_predict(const float4* __restrict__ x, 
     block_max_atomic(err, &stats[ST_ERR_START]);
     block_sum_atomic(aerr, &stats[ST_MEAN_ABS_START]);
     block_sum_atomic(err, &stats[ST_MPOS_START]);
+}
+
+// Zeroes the statistics slots whose bit is set in `mask` (64 threads).
+__global__ void k_zero_slots(float* __restrict__ stats, unsigned long long mask) {
+    const uint t = threadIdx.x;
+    if (t < (uint)ST_N && ((mask >> t) & 1ull)) stats[t] = 0.0f;
 }
 
 // Folds the judge of the previous step (its final positions, measured by this step's prediction pass) into
@@ -398,7 +500,7 @@ __device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, const 
     uint b0 = 0, b1 = 0, b2 = 0, b3 = 0;  // the open chunk as a 128-bit shift register, newest entry at the top
     cands(plane, [&](uint j, float4 q4, uint pl_base) {  // q4 = (x*_j, density at the step start)
         float3 r = p - f3(q4);
-        float r2 = dot3(r, r);
+        float r2 = len2_fixed(r);
         if (r2 <= k.rc2) {
             if (c < (uint)k.maxn) {
                 const uint e = j - pl_base;  // pl_base = base - (plane << 14): the entry's plane bits come with it
@@ -456,7 +558,7 @@ __device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, const 
     float C = d * k.s.inv_rho0 - 1.0f;
     float lam = -C / (sd + k.eps);
     xl[i] = make_float4(p.x, p.y, p.z, lam * k.lam0_scale);
-    xbuild[i] = p4;
+    xbuild[i] = make_float4(p.x, p.y, p.z, 0.0f);  // w: mover flag, set by the correction pass
     dens[i] = d;
     float err = fmaxf(C, 0.0f);
     emax = fmaxf(emax, err);
@@ -663,13 +765,16 @@ __global__ __launch_bounds__(kTpb) void k_lambda(const float4* __restrict__ xa, 
     bool valid;
     uint i = slot_of(k, valid);
     float disp2 = stats[ST_DISP2];
+    const NearMovers nm = stage_movers(i, valid, k, disp2, stats, xb, bhash, false);
+    // the movers the next correction pass reads (it appends new ones while it runs)
+    if (blockIdx.x == 0 && threadIdx.x == 0) stats[ST_MOVERS_PUB] = stats[ST_MOVERS];
     float err = 0.0f;
     if (valid) {
         float3 p = f3(xa[i]);
         float d = 0.0f, sd = 0.0f;
         float3 gs = f3(0, 0, 0);
         for_neighbors<2>(
-            i, 0, k, list, cnt, bhash, off, disp2, p, xb, [&](uint j) { return xa[j]; },
+            i, 0, k, list, cnt, bhash, off, disp2, p, xb, nm, [&](uint j) { return xa[j]; },
             [&](uint, const float4& qj) {
                 float3 r = p - f3(qj);
                 float r2 = dot3(r, r);
@@ -700,7 +805,7 @@ __global__ __launch_bounds__(kTpb) void k_lambda(const float4* __restrict__ xa, 
 // Position correction (rx_sph_pbd.cpp:925-1026) + apply + wall (rx_sph_pbd.cpp:436-445): reads xl = (x, lambda),
 // writes xa = clamp(x + dp).  Tracks the displacement from the list build.  In learned mode the pass is
 // skipped once the previous check met the tolerance.
-__global__ __launch_bounds__(kTpb) void k_dp(const float4* __restrict__ xl, float4* __restrict__ xa, const float4* __restrict__ xbuild,
+__global__ __launch_bounds__(kTpb) void k_dp(const float4* __restrict__ xl, float4* __restrict__ xa, float4* __restrict__ xbuild,
                                              float* __restrict__ lamacc, float* __restrict__ stats, int it, PbfK k,
                                              const uint* __restrict__ list, const uint4* __restrict__ cnt,
                                              const uint* __restrict__ bhash, const uint* __restrict__ off, int adaptive) {
@@ -712,7 +817,8 @@ __global__ __launch_bounds__(kTpb) void k_dp(const float4* __restrict__ xl, floa
     bool valid;
     uint i = slot_of(k, valid);
     float disp2 = stats[ST_DISP2];
-    float mv = 0.0f;
+    const NearMovers nm = stage_movers(i, valid, k, disp2, stats, xbuild, bhash, true);
+    float mv = 0.0f, mv0 = 0.0f;
     if (valid) {
         float4 p4 = xl[i];
         float3 p = f3(p4);
@@ -720,7 +826,7 @@ __global__ __launch_bounds__(kTpb) void k_dp(const float4* __restrict__ xl, floa
         float dt2 = k.dt * k.dt;
         float3 acc = f3(0, 0, 0);
         for_neighbors<1>(
-            i, 0, k, list, cnt, bhash, off, disp2, p, xbuild, [&](uint j) { return xl[j]; },
+            i, 0, k, list, cnt, bhash, off, disp2, p, xbuild, nm, [&](uint j) { return xl[j]; },
             [&](uint, const float4& q4) {
                 float3 r = p - f3(q4);
                 float r2 = dot3(r, r);
@@ -739,8 +845,19 @@ __global__ __launch_bounds__(kTpb) void k_dp(const float4* __restrict__ xl, floa
         float3 xn = clamp_wall(p + acc, k.s);
         xa[i] = make_float4(xn.x, xn.y, xn.z, 0.0f);
         lamacc[i] += li;  // the multiplier this step has applied so far
-        float3 dd = xn - f3(xbuild[i]);
-        mv = dot3(dd, dd);
+        const float4 b4 = xbuild[i];
+        mv = len2_fixed(xn - f3(b4));
+        mv0 = len2_fixed(p - f3(b4));
+        if (mv > k.half_skin2 && b4.w == 0.0f) {  // a new mover: listed once until the next build
+            xbuild[i].w = 1.0f;
+            const uint t = atomicAdd((unsigned int*)&stats[ST_MOVERS], 1u);
+            if (t < kMoverCap) k.movers[t] = make_float4(b4.x, b4.y, b4.z, __uint_as_float(i));
+        }
+    }
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {  // one atomic per wavefront and level, when a lane crossed
+        const uint64_t c = __ballot(mv0 <= k.cross2[l] && mv > k.cross2[l]);
+        if (c && __lane_id() == (uint)__builtin_ctzll(c)) atomicAdd((unsigned int*)&stats[ST_CROSS + l], (uint)__builtin_popcountll(c));
     }
     block_max_atomic(mv, &stats[ST_DISP2]);
 }
@@ -762,13 +879,14 @@ __global__ __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(6))) void 
                                                const uint* __restrict__ off, const float4* __restrict__ xb) {
     bool valid;
     uint i = slot_of(k, valid);
-    if (!valid) return;
     float disp2 = stats[ST_DISP2];
+    const NearMovers nm = stage_movers(i, valid, k, disp2, stats, xb, bhash, false);
+    if (!valid) return;
     float3 p = f3(xa[i]);
     float3 vi = f3(v[i]);
     float3 acc = f3(0, 0, 0);
     for_neighbors<1>(
-        i, 0, k, list, cnt, bhash, off, disp2, p, xb, [&](uint j) { return XV{xa[j], v[j]}; },
+        i, 0, k, list, cnt, bhash, off, disp2, p, xb, nm, [&](uint j) { return XV{xa[j], v[j]}; },
         [&](uint, const XV& q4) {
             float3 r = p - f3(q4.x);
             float r2 = dot3(r, r);
@@ -793,14 +911,15 @@ __global__ __launch_bounds__(kTpb) void k_heat_diffuse(const float4* __restrict_
                                                        float hdt) {
     bool valid;
     uint i = slot_of(k, valid);
-    if (!valid) return;
     float disp2 = stats[ST_DISP2];
+    const NearMovers nm = stage_movers(i, valid, k, disp2, stats, xb, bhash, false);
+    if (!valid) return;
     float3 p = f3(x[i]);
     float ti = therm[i].x;
     const float inv_h = k.inv_h, ws = 10.0f / (7.0f * 3.14159265358979f * k.s.h2);
     float acc = 0.0f;
     for_neighbors(
-        i, 0, k, list, cnt, bhash, off, disp2, p, xb, [&](uint j) { return x[j]; },
+        i, 0, k, list, cnt, bhash, off, disp2, p, xb, nm, [&](uint j) { return x[j]; },
         [&](uint j, const float4& qj) {
             float3 r = p - f3(qj);
             float r2 = dot3(r, r);
@@ -887,7 +1006,8 @@ void rxPBDSPH_GPU::Finalize() {
     drop_graphs();
     for (void** p : {(void**)&m_x, (void**)&m_xs, (void**)&m_xl, (void**)&m_xb, (void**)&m_v, (void**)&m_t, (void**)&m_dens,
                      (void**)&m_lamacc, (void**)&m_id, (void**)&m_idt, (void**)&m_list, (void**)&m_cnt, (void**)&m_stats,
-                     (void**)&m_ftmp, (void**)&m_feat, (void**)&m_ftmp16, (void**)&m_therm, (void**)&m_therm_t, (void**)&m_mlp_mem})
+                     (void**)&m_ftmp, (void**)&m_feat, (void**)&m_ftmp16, (void**)&m_therm, (void**)&m_therm_t, (void**)&m_mlp_mem,
+                     (void**)&m_movers})
         if (*p) {
             RX_HIP(hipFree(*p));
             *p = nullptr;
@@ -936,12 +1056,16 @@ int rxPBDSPH_GPU::Initialize(const rxSphEnv& env) {
     return 0;
 }
 
-// Reads the completed records (oldest first) and updates the targets.  The wanted level is the lowest whose skin
-// is at least 2.2 times the step's largest displacement D: then d_i + D <= 2D stays within the skin and no
-// particle walks the grid.  A grid walk costs 10 to 20 list walks while each level up costs 1.4 to 1.9 times
-// the list work, so the level rises (at once, to the wanted one) only when more than 3% of the step's neighbour
-// passes walked the grid; it falls to the wanted level after 2 records in a row want a lower one.  The list
-// capacity rises by 64 when more than n/2000 lists overflowed.
+// Reads the completed records (oldest first) and updates the targets.  Level want_d is the lowest whose skin is at
+// least 2.2 times the step's largest displacement D: then d_i + D <= 2D stays within the skin and no particle
+// walks the grid.  Level want_m(a) is the lowest at which at most a particles crossed half its skin: those few
+// walk the grid as movers and the rest keep their lists, at a cost that stays small while they are sparse
+// (allowance A = min(kMoverCap / 2, n / 512)).  A grid walk costs 10 to 20 list walks while each level up costs
+// 1.4 to 1.9 times the list work, so the level rises only when more than 3% of the step's neighbour passes
+// walked the grid: at once to min(want_d, want_m(A)), and at least one level (up to want_d) since the walks show
+// the movers were not few enough.  It falls to min(want_d, want_m(A / 4)) (the lower allowance keeps it from
+// falling back into the walks) after 2 records in a row want a lower level.  The list capacity rises by 64 when
+// more than n/2000 lists overflowed.
 void rxPBDSPH_GPU::poll_records() {
     while (m_rec_tail < m_rec_head) {
         const int slot = (int)(m_rec_tail % kRecs);
@@ -951,11 +1075,12 @@ void rxPBDSPH_GPU::poll_records() {
             break;
         }
         if (!RX_HIP(e)) break;
-        const float* r = m_hrec + kRecLen * slot;  // ST_DISP2, ST_OVERFLOW, ST_ITERS, ST_FALLBACK, ST_ERR_FINAL, ST_BAD
+        const float* r = m_hrec + kRecLen * slot;  // stats slots ST_DISP2 .. ST_CROSS + 3
+        auto u = [&](int st) { return __builtin_bit_cast(uint, r[st - ST_DISP2]); };
         const float D = sqrtf(fmaxf(r[0], 0.0f));
-        const uint over = __builtin_bit_cast(uint, r[1]), fb = __builtin_bit_cast(uint, r[3]);
+        const uint over = u(ST_OVERFLOW), fb = u(ST_FALLBACK);
         ++m_rec_tail;
-        if (__builtin_bit_cast(uint, r[5])) {
+        if (u(ST_BAD)) {
             // a pass of that step walked less than the reach it needed: start the next step from a fresh grid and
             // let Update report it
             ++m_bad_steps;
@@ -963,11 +1088,19 @@ void rxPBDSPH_GPU::poll_records() {
         }
         if (!m_adapt) continue;
         const float need = 2.2f * D;
-        int want = 0;
-        while (want < 3 && m_skin_base * (float)(1 + want) < need) ++want;
+        int want_d = 0;
+        while (want_d < 3 && m_skin_base * (float)(1 + want_d) < need) ++want_d;
+        auto want_m = [&](uint allowance) {
+            int l = 0;
+            while (l < 3 && u(ST_CROSS + l) > allowance) ++l;
+            return l;
+        };
+        const uint allowance = std::min(kMoverCap / 2, m_n / 512);
         const float passes = (float)(2 * std::min(m_env.max_iter, 14) + 2);
-        if ((float)fb > 0.03f * (float)m_n * passes && want > m_level_target) {
-            m_level_target = want;
+        const int want_up = std::max(std::min(want_d, want_m(allowance)), std::min(want_d, m_level_target + 1));
+        const int want = std::min(want_d, want_m(allowance / 4));
+        if ((float)fb > 0.03f * (float)m_n * passes) {
+            if (want_up > m_level_target) m_level_target = want_up;
             m_calm = 0;
         } else if (want < m_level_target) {
             m_calm_want = m_calm ? std::max(m_calm_want, want) : want;
@@ -997,6 +1130,12 @@ void rxPBDSPH_GPU::push_record() {
 // the grid to h + skin; the cells only grow relative to level 0, so the reserved cell arrays suffice.
 bool rxPBDSPH_GPU::apply_targets(hipStream_t s) {
     bool changed = false;
+    if (m_level_target > m_level) {
+        // a wider skin holds more neighbours per list: the capacity grows with the volume of the cutoff at once
+        const float r = (m_env.h + m_skin_base * (float)(1 + m_level_target)) / (m_env.h + m_skin_base * (float)(1 + m_level));
+        const int grown = ((int)ceilf((float)m_maxn * r * r * r) + 31) & ~31;
+        m_maxn_target = std::min(m_maxn_cap, std::max(m_maxn_target, grown));
+    }
     if (m_maxn_target != m_maxn) {
         // the larger list is allocated before the old one is released; when the device has no room for it the
         // capacity stays (overflowed lists keep walking the grid) and no larger one is asked for again
@@ -1035,7 +1174,7 @@ bool rxPBDSPH_GPU::alloc(uint n) {
               RX_HIP(hipMalloc(&m_idt, u)) && RX_HIP(hipMalloc(&m_list, list_bytes(n, m_maxn))) && RX_HIP(hipMalloc(&m_cnt, v4)) &&
               RX_HIP(hipMalloc(&m_stats, sizeof(float) * ST_N)) && RX_HIP(hipMalloc(&m_ftmp, f)) &&
               RX_HIP(hipMalloc(&m_feat, v4 * 4)) &&
-              RX_HIP(hipMalloc(&m_ftmp16, v4 * 4));
+              RX_HIP(hipMalloc(&m_ftmp16, v4 * 4)) && RX_HIP(hipMalloc(&m_movers, sizeof(float4) * kMoverCap));
     m_grid.n = n;
     m_grid.setup(m_params, m_env.h);  // the cell-h grid of the first prediction has the most cells
     ok = ok && m_grid.reserve(n);
@@ -1060,7 +1199,7 @@ int rxPBDSPH_GPU::SetParticles(const float* pos4, const float* vel4, int n) {
     if ((uint)n != m_n) {
         for (void** p : {(void**)&m_x, (void**)&m_xs, (void**)&m_xl, (void**)&m_xb, (void**)&m_v, (void**)&m_t, (void**)&m_dens,
                          (void**)&m_lamacc, (void**)&m_id, (void**)&m_idt, (void**)&m_list, (void**)&m_cnt, (void**)&m_stats,
-                         (void**)&m_ftmp, (void**)&m_feat, (void**)&m_ftmp16, (void**)&m_therm, (void**)&m_therm_t})
+                         (void**)&m_ftmp, (void**)&m_feat, (void**)&m_ftmp16, (void**)&m_therm, (void**)&m_therm_t, (void**)&m_movers})
             if (*p) {
                 RX_HIP(hipFree(*p));
                 *p = nullptr;
@@ -1123,6 +1262,11 @@ PbfK rxPBDSPH_GPU::consts(float dt) const {
     k.wide2 = wide * wide;
     k.fallback = (uint*)(m_stats + ST_FALLBACK);
     k.bad = (uint*)(m_stats + ST_BAD);
+    k.movers = m_movers;
+    for (int l = 0; l < 4; ++l) {
+        const float t = 0.5f * m_skin_base * (float)(1 + l);
+        k.cross2[l] = t * t;
+    }
     k.maxn = m_maxn;
     k.eps = m_env.epsilon;
     k.gfac = k.s.gwspiky * k.s.inv_rho0 * (m_grad_mass ? k.s.mass : 1.0f);
@@ -1251,9 +1395,8 @@ int rxPBDSPH_GPU::enqueue_step(float dt) {
     unsigned nb = rx_blocks(n, kTpb);
     const int K = std::min(m_env.max_iter, 14);
     const int adaptive = m_adaptive ? 1 : 0;
-    // per-step statistics (ST_DISP2 carries over: it bounds the validity of the previous list)
-    k_fill<float><<<1, 64, 0, s>>>(m_stats, 0.0f, ST_DISP2);
-    k_fill<float><<<1, 64, 0, s>>>(m_stats + ST_OVERFLOW, 0.0f, ST_RUN - ST_OVERFLOW);
+    // per-step statistics (ST_DISP2 and the movers carry over: the prediction walks the previous list)
+    k_zero_slots<<<1, 64, 0, s>>>(m_stats, ((1ull << ST_RUN) - 1ull) & ~(1ull << ST_DISP2) & ~(1ull << ST_MOVERS));
 
     // ---- predict at x_n
     int mode = 0;
@@ -1286,7 +1429,7 @@ int rxPBDSPH_GPU::enqueue_step(float dt) {
     if (apply_targets(s)) k = consts(dt);  // list sizes for this step's build (the prediction used the old list)
 
     // ---- rebuild on x*; the step-start density follows the particles
-    k_fill<float><<<1, 64, 0, s>>>(m_stats + ST_DISP2, 0.0f, 1);
+    k_zero_slots<<<1, 64, 0, s>>>(m_stats, (1ull << ST_DISP2) | (1ull << ST_MOVERS) | (1ull << ST_MOVERS_PUB));
     mk = begin_mark(RX_PROF_GRID);
     {
         // the sort's last pass gathers the whole state into the new order (xl and xb are dead until the list
@@ -1485,7 +1628,9 @@ int rxPBDSPH_GPU::GetStats(float* out, int count) {
     h[ST_MEAN_ABS_START] = h[ST_MEAN_ABS_START] / (float)m_n;
     h[ST_MPOS_START] = h[ST_MPOS_START] / (float)m_n;
     for (int t = 0; t < 16; ++t) h[ST_MPOS_ITER + t] = h[ST_MPOS_ITER + t] / (float)m_n;
-    for (int t : {ST_RUN_ITERS, ST_RUN_STEPS, ST_RUN_FAIL, ST_RUN_BAD}) h[t] = (float)__builtin_bit_cast(uint, h[t]);
+    for (int t : {(int)ST_RUN_ITERS, (int)ST_RUN_STEPS, (int)ST_RUN_FAIL, (int)ST_RUN_BAD, (int)ST_MOVERS, (int)ST_MOVERS_PUB,
+                  ST_CROSS + 0, ST_CROSS + 1, ST_CROSS + 2, ST_CROSS + 3})
+        h[t] = (float)__builtin_bit_cast(uint, h[t]);
     h[ST_DISP2] = sqrtf(h[ST_DISP2]);
     h[60] = m_skin / m_env.h;  // host state: the list sizes in use
     h[61] = (float)m_maxn;

@@ -157,7 +157,11 @@ enum rxProfileClass {
  *           from a fresh grid (the list is rebuilt)
  *   23      mean |rho - rho0| / rho0 at the step's start positions
  *   24      mean (rho - rho0)+ / rho0 at the step's start positions
- *   32 + k  the same at the start of Jacobi iteration k (k < 16)
+ *   25      movers: particles that moved more than half the skin since the list build (at most 2048 are
+ *           listed; beyond that every particle bounds displacements by the largest one)
+ *   27 + l  corrections that carried a particle past half the skin of level l (l < 4: 1 + l times the set
+ *           skin), which the adaptive sizing reads
+ *   32 + k  slot 24 at the start of Jacobi iteration k (k < 16)
  * Run accumulators (since SetParticles or RX_PBF_OPT_RESET_RUN), over the steps judged so far (a step is
  * judged by the prediction pass of the step after it):
  *   48      max of slot 0        49  max of slot 24      50  corrections executed

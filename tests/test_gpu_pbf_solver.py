@@ -252,3 +252,52 @@ def test_one_step_parity_with_a_crowded_border_cell():
     err = np.abs(gpos[:, :3] - rpos[:, :3]).max() / scale
     print(f"crowded corner cell: {len(crowd)} particles, max position error {err:.2e} of the box scale")
     assert err <= 1e-5
+
+
+@pytest.mark.parametrize("skin", [0.02, 0.05, 0.15])
+def test_one_step_parity_with_movers(skin):
+    # a fixed skin small enough that the corrections move some particles past half of it: those are listed as
+    # movers and walk the grid, the rest walk their lists plus the nearby movers their lists lack (or, with
+    # too many movers, every particle bounds displacements by the largest); the step still matches the oracle
+    sc = scene.pbf_dam_break(20, 12, 10, seed=2)
+    pos, vel = sc.particles()
+    P = oracle.params_from(sc.params(), sc)
+    rpos, rvel, rdens, st = oracle.pbf_step(pos, vel, P)
+    s = PBFSolver(sc, pos, vel)
+    s.option(_abi.RX_PBF_OPT_ADAPT_SKIN, 0)
+    s.option(_abi.RX_PBF_OPT_SKIN, skin)
+    s.step(1)
+    stt = s.stats()
+    print(f"skin {skin} h: largest displacement {stt[17] / sc.h:.4f} h, movers {stt[25]:.0f}, grid walks {stt[20]:.0f}")
+    gpos, gvel = s.positions(), s.velocities()
+    scale = np.abs(rpos[:, :3]).max()
+    assert np.abs(gpos[:, :3] - rpos[:, :3]).max() <= 1e-5 * scale
+    assert np.abs(gvel[:, :3] - rvel[:, :3]).max() <= 1e-3 * np.abs(rvel[:, :3]).max()
+    for k in range(sc.iterations):
+        assert stt[1 + k] == pytest.approx(st.max_err[k], rel=1e-3, abs=1e-6)
+    assert stt[22] == 0
+    if skin == 0.15:
+        assert 0 < stt[25] <= 512  # the default skin takes the mover path here
+
+
+def test_multi_step_parity_with_movers():
+    """Ten steps at a fixed skin of 0.1 h: movers in the corrections, and in the next step's prediction pass
+    over the previous list."""
+    sc = scene.pbf_dam_break(16, 16, 16)
+    pos, vel = sc.particles()
+    P = oracle.params_from(sc.params(), sc)
+    rp, rv = pos, vel
+    for _ in range(10):
+        rp, rv, _, _ = oracle.pbf_step(rp, rv, P)
+    s = PBFSolver(sc, pos, vel)
+    s.option(_abi.RX_PBF_OPT_ADAPT_SKIN, 0)
+    s.option(_abi.RX_PBF_OPT_SKIN, 0.1)
+    movers = 0
+    for _ in range(10):
+        s.step(1)
+        movers = max(movers, s.stats()[25])
+    err = np.abs(s.positions()[:, :3] - rp[:, :3]).max() / sc.h
+    print(f"10 steps at skin 0.1 h: most movers in a step {movers:.0f}, GPU vs oracle {err:.2e} h")
+    assert movers > 0
+    assert err < 1e-3
+    assert s.stats()[22] == 0
