@@ -54,6 +54,14 @@ __device__ inline bf16x4 pack4_hw(float a, float b, float c, float d) {
 constexpr int kMlpW2Stride = kMlpH + 8;  // padded LDS row: the 16 rows a fragment read touches fall in different banks
 
 // The network staged in LDS by one block for inference inside another kernel.
+// Bound on the learned first multiplier: with kappa > 0 it stays between 0 and kappa times the plain first
+// multiplier lam0 (the same sign, at most kappa times its size); kappa = 0 leaves it as predicted.
+__device__ inline float warm_clamp(float lam, float lam0, float kappa) {
+    if (!(kappa > 0.0f)) return lam;
+    const float b = kappa * lam0;
+    return fminf(fmaxf(lam, fminf(b, 0.0f)), fmaxf(b, 0.0f));
+}
+
 struct MlpShared {
     __align__(16) unsigned short w2[kMlpH * kMlpW2Stride];  // bf16 [unit][input]
     __align__(16) float sb[3 * kMlpH];                       // b1, b2, w3
@@ -140,7 +148,7 @@ __device__ inline float mlp_predict_rows(const unsigned short* xrows, const MlpS
 }
 
 // Inference over n feature rows (16 floats each): xl[i].w = (y_i * out_scale + out_bias) * w_scale.
-void rx_mlp_infer_to_w(const float* feat, float4* xl, uint n, const MlpDev& m, float w_scale, hipStream_t s);
+void rx_mlp_infer_to_w(const float* feat, float4* xl, uint n, const MlpDev& m, float w_scale, float kappa, hipStream_t s);
 // Inference into a plain array: y[i] = y_i * out_scale + out_bias.
 void rx_mlp_infer(const float* feat, float* y, uint n, const MlpDev& m, hipStream_t s);
 // Copies the device view out of an rxMLP object (returns false for a null object).  The view borrows the

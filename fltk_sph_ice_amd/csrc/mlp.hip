@@ -124,7 +124,7 @@ __device__ inline f32x4 forward16(bf16x4 a1, const MlpDev& m, unsigned short* ld
 // the output weights sit in LDS (read as float4 per 4 units).  A wave walks 16-row tiles with a grid stride and
 // loads the next tile's features before computing the current one.
 __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ X, uint n, MlpDev m, float* __restrict__ y,
-                                                   float4* __restrict__ xl, float w_scale) {
+                                                   float4* __restrict__ xl, float w_scale, float kappa) {
     __shared__ __align__(16) float sb[3 * kMlpH];  // b1, b2, w3
     constexpr int kS = kMlpH + 8;  // padded row: the 16 rows a fragment read touches fall in different banks
     __shared__ __align__(16) unsigned short sw2[kMlpH * kS];  // W2, bf16 [unit][input]
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ X, 
         if (kb == 0 && r < n) {
             float out = (p + b3) * m.out_scale + m.out_bias;
             if (y) y[r] = out;
-            if (xl) xl[r].w = out * w_scale;
+            if (xl) xl[r].w = warm_clamp(out * w_scale, xl[r].w, kappa);  // xl.w held lambda_0
         }
         x = xn;
     }
@@ -567,11 +567,11 @@ static unsigned infer_blocks(uint n) {
     uint blocks = (tiles + 31) / 32;
     return blocks < 1 ? 1 : (blocks > 2048 ? 2048 : blocks);
 }
-void rx_mlp_infer_to_w(const float* feat, float4* xl, uint n, const MlpDev& m, float w_scale, hipStream_t s) {
-    if (n) k_mlp_infer<<<infer_blocks(n), 256, 0, s>>>(feat, n, m, nullptr, xl, w_scale);
+void rx_mlp_infer_to_w(const float* feat, float4* xl, uint n, const MlpDev& m, float w_scale, float kappa, hipStream_t s) {
+    if (n) k_mlp_infer<<<infer_blocks(n), 256, 0, s>>>(feat, n, m, nullptr, xl, w_scale, kappa);
 }
 void rx_mlp_infer(const float* feat, float* y, uint n, const MlpDev& m, hipStream_t s) {
-    if (n) k_mlp_infer<<<infer_blocks(n), 256, 0, s>>>(feat, n, m, y, nullptr, 1.0f);
+    if (n) k_mlp_infer<<<infer_blocks(n), 256, 0, s>>>(feat, n, m, y, nullptr, 1.0f, 0.0f);
 }
 bool rx_mlp_view(void* obj, MlpDev* out) {
     if (!obj) return false;

@@ -68,6 +68,8 @@ private:
     int m_mode = RX_PBF_MODE_PLAIN;
     bool m_adaptive = false;
     float m_warm_scale = 1.0f;
+    float m_warm_kappa = 0.0f;   // RX_PBF_OPT_WARM_CLAMP
+    float m_first_dmax = 0.0f;   // RX_PBF_OPT_FIRST_DMAX, as a fraction of h
     int m_judge = 0;
     bool m_grad_mass = false;  // RX_PBF_OPT_GRAD_MASS
     int m_list_tile = 0;  // LDS tiling variant of the list pass (RX_PBF_OPT_LIST_TILE)

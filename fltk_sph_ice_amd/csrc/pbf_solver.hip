@@ -45,6 +45,8 @@ struct PbfK {
     float tol;        // adaptive solve: stop once the judged density error is at most tol
     int judge;        // 0: max (rho - rho0)+ / rho0; 1: its mean over the particles
     float lam0_scale; // factor on the first multiplier (scaled mode; 1 otherwise)
+    float warm_kappa; // learned mode: the first multiplier stays within [0, warm_kappa * lambda_0] (0: unbounded)
+    float first_dmax; // largest displacement of the first correction of a step (0: unbounded)
     // feature normalisation of the learned warm start
     float sd_ref;     // lambda denominator of an interior particle of the rest lattice
     float inv_nref;   // 1 / neighbour count of the rest lattice
@@ -650,7 +652,7 @@ __global__ __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(6))) void 
     if (kNet) {  // every lane of the wavefront takes part (rows of lanes past the end are zero; their outputs unused)
         if (!valid) ((uint4*)xrow)[0] = ((uint4*)xrow)[1] = make_uint4(0u, 0u, 0u, 0u);
         const float y = mlp_predict_rows(s_rows + (threadIdx.x & ~63u) * kMlpIn, s_net[0], net);
-        if (valid) xl[i].w = y * w_scale;
+        if (valid) xl[i].w = warm_clamp(y * w_scale, xl[i].w, k.warm_kappa);  // xl.w held lambda_0
     }
     block_max_atomic(emax, &stats[ST_ERR_ITER + 0]);
     block_sum_atomic(esum, &stats[ST_MPOS_ITER + 0]);
@@ -842,6 +844,10 @@ __global__ __launch_bounds__(kTpb) void k_dp(const float4* __restrict__ xl, floa
                     acc += ((li + q4.w + sc) * k.gfac * q * q * ri) * r;
                 }
             });
+        if (it == 0 && k.first_dmax > 0.0f) {  // the warm start's correction, held to first_dmax
+            const float l2 = dot3(acc, acc);
+            if (l2 > k.first_dmax * k.first_dmax) acc = (k.first_dmax * rinv_of(l2)) * acc;
+        }
         float3 xn = clamp_wall(p + acc, k.s);
         xa[i] = make_float4(xn.x, xn.y, xn.z, 0.0f);
         lamacc[i] += li;  // the multiplier this step has applied so far
@@ -1276,6 +1282,8 @@ PbfK rxPBDSPH_GPU::consts(float dt) const {
     k.tol = m_tol;
     k.judge = m_judge;
     k.lam0_scale = m_mode == RX_PBF_MODE_SCALED ? m_warm_scale : 1.0f;
+    k.warm_kappa = m_mode == RX_PBF_MODE_LEARNED ? m_warm_kappa : 0.0f;
+    k.first_dmax = m_first_dmax * m_env.h;
     k.sd_ref = m_sd_ref;
     k.inv_nref = 1.0f / m_nref;
     k.inv_gref = 1.0f / sqrtf(m_sd_ref);
@@ -1490,7 +1498,7 @@ int rxPBDSPH_GPU::enqueue_step(float dt) {
     end_mark(mk);
     if (learned && !fused_net) {
         mk = begin_mark(RX_PROF_MLP);
-        rx_mlp_infer_to_w(m_feat, m_xl, n, m_mlp, m_warm_scale / k.sd_ref, s);
+        rx_mlp_infer_to_w(m_feat, m_xl, n, m_mlp, m_warm_scale / k.sd_ref, k.warm_kappa, s);
         end_mark(mk);
     }
     for (int it = 0; it < K; ++it) {
@@ -1814,6 +1822,15 @@ int rxPBDSPH_GPU::SetOption(int opt, float value) {
         if (!m_adapt) m_level_target = 0;  // back to the set skin at the next step
         return 0;
     case RX_PBF_OPT_HEAT_DT: m_heat_dt = value; return 0;
+    case RX_PBF_OPT_WARM_CLAMP:
+    case RX_PBF_OPT_FIRST_DMAX:
+        if (!(value >= 0.0f)) {
+            rx_set_error(1, "rxPBDSPH_GPU::SetOption: the bound must be >= 0 (0 = none)");
+            return 1;
+        }
+        (opt == RX_PBF_OPT_WARM_CLAMP ? m_warm_kappa : m_first_dmax) = value;
+        drop_graphs();
+        return 0;
     case RX_PBF_OPT_GRAD_MASS:
         m_grad_mass = value != 0.0f;
         lattice_refs();  // the feature scales follow the gradient

@@ -110,10 +110,11 @@ def feature_scales(solver: PBFSolver):
     return sd.value, nref.value
 
 
-def generate_dataset(scenes, steps: int, samples_per_step: int, teacher_iters: int = 12, warmup: int = 5, seed: int = 0):
+def generate_dataset(scenes, steps: int, samples_per_step: int, teacher_iters: int = 12, warmup: int = 5, seed: int = 0,
+                     stride: int = 1):
     """(features, target) pairs from the solver itself: a teacher run of every scene (plain Jacobi with
-    teacher_iters iterations) records, each step, the features at x* and the multiplier the step applied in
-    total (sum over the iterations of lambda, scaled by sd_ref)."""
+    teacher_iters iterations) records, every `stride`-th of its `steps` steps, the features at x* and the
+    multiplier the step applied in total (sum over the iterations of lambda, scaled by sd_ref)."""
     rng = np.random.default_rng(seed)
     Xs, Ts = [], []
     for sc in scenes:
@@ -123,8 +124,10 @@ def generate_dataset(scenes, steps: int, samples_per_step: int, teacher_iters: i
         s.option(_abi.RX_PBF_OPT_RECORD_FEATURES, 1)
         sd_ref, _ = feature_scales(s)
         s.step(warmup)
-        for _ in range(steps):
+        for t in range(steps):
             s.step(1)
+            if (t + 1) % stride:
+                continue
             F = np.zeros((s.n, NIN), np.float32)
             if s.lib.rxPBDSPH_GPU_GetArray(s.h, _abi.RX_ARRAY_FEATURES, F.ctypes.data, 16 * s.n):
                 check(s.lib)

@@ -124,11 +124,16 @@ enum rxPbfOption {
                                       lists overflow.  The host reads each step's figures once the device has
                                       finished it, without waiting, so the sizes follow with a lag.  Eager steps
                                       only (replayed graphs keep their sizes). */
-    RX_PBF_OPT_GRAD_MASS = 19      /* 1: the constraint gradients carry the particle mass, grad C_i =
+    RX_PBF_OPT_GRAD_MASS = 19,     /* 1: the constraint gradients carry the particle mass, grad C_i =
                                       (m / rho0) sum_j grad W_ij, in the multiplier's denominator and in the
                                       position correction (Macklin & Mueller 2013, section 3, with rho = sum m W).
                                       0 (default): the reference's grad W / rho0 (rx_sph_pbd.cpp:782-786,
                                       rx_sph_pbd.cpp:970-973), which scales each correction by m */
+    RX_PBF_OPT_WARM_CLAMP = 20,    /* learned mode: keep the network's first multiplier between 0 and this
+                                      many times the plain first multiplier lambda_0 (its sign, at most this
+                                      multiple of its size); 0 (default): as predicted                       */
+    RX_PBF_OPT_FIRST_DMAX = 21     /* largest displacement of a step's first correction, as a fraction of h
+                                      (the warm start's correction); 0 (default): unbounded                  */
 };
 
 /* Kernel classes of rxPBDSPH_GPU_GetProfile. */

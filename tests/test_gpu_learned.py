@@ -158,3 +158,24 @@ def test_coupled_scene_with_the_learned_fluid(net):
         print(f"coupled {mode:8s} worst max {st[48]:.4f} worst mean {st[49]:.4f} ice particles no longer frozen {res[mode][2]}")
     assert res["learned"][2] > 0  # heat crossed into the ice in learned mode too
     assert res["learned"][1] <= res["plain"][1]
+
+
+def test_capped_warm_start_stays_on_the_grid_over_a_long_run(net):
+    # 300 steps of a 40^3 dam break: the block collapses and the wave hits the far wall.  The warm start's
+    # correction is held to 0.1 h (RX_PBF_OPT_FIRST_DMAX): no step needs a neighbour walk beyond its reach, the
+    # state stays finite and the fastest particle stays near the free-fall speed of the block's height
+    sc = scene.pbf_dam_break(40, 40, 40, seed=8)
+    pos, vel = sc.particles()
+    s = learned.learned_solver(sc, net, 4, 0.01, pos=pos, vel=vel, warm_scale=0.5)
+    with pytest.raises(RuntimeError):
+        s.option(_abi.RX_PBF_OPT_FIRST_DMAX, -1.0)
+    s.option(_abi.RX_PBF_OPT_FIRST_DMAX, 0.1)
+    s.step(300)
+    st = s.stats()
+    p, v = s.positions(), s.velocities()
+    s.close()
+    vmax = float(np.abs(v[:, :3]).max())
+    print(f"300 steps, capped warm start: run capped-walk flag {st[53]:.0f}, max |v| {vmax:.2f}, worst max error {st[48]:.3f}")
+    assert st[53] == 0
+    assert np.isfinite(p).all()
+    assert vmax < 3.0 * np.sqrt(2.0 * 9.8 * 40 * sc.spacing)
