@@ -1062,7 +1062,7 @@ int rxPBDSPH_GPU::Initialize(const rxSphEnv& env) {
     return 0;
 }
 
-// Reads the completed records (oldest first) and updates the targets.  Level want_d is the lowest whose skin is at
+// Applies the step records (oldest first) and updates the targets.  Level want_d is the lowest whose skin is at
 // least 2.2 times the step's largest displacement D: then d_i + D <= 2D stays within the skin and no particle
 // walks the grid.  Level want_m(a) is the lowest at which at most a particles crossed half its skin: those few
 // walk the grid as movers and the rest keep their lists, at a cost that stays small while they are sparse
@@ -1073,14 +1073,12 @@ int rxPBDSPH_GPU::Initialize(const rxSphEnv& env) {
 // falling back into the walks) after 2 records in a row want a lower level.  The list capacity rises by 64 when
 // more than n/2000 lists overflowed.
 void rxPBDSPH_GPU::poll_records() {
-    while (m_rec_tail < m_rec_head) {
+    // Before step s the records of steps up to s - 2 are applied, waiting for them if need be, and never the record
+    // of step s - 1: the sizes of every step are then a function of the run alone, not of how far the host ran
+    // ahead (the device still has step s - 1 queued while the host issues step s).
+    while (m_rec_tail + 1 < m_rec_head) {
         const int slot = (int)(m_rec_tail % kRecs);
-        const hipError_t e = hipEventQuery(m_rec_ev[slot]);
-        if (e == hipErrorNotReady) {
-            (void)hipGetLastError();  // "not ready" is a state, not an error to report later
-            break;
-        }
-        if (!RX_HIP(e)) break;
+        if (!RX_HIP(hipEventSynchronize(m_rec_ev[slot]))) break;
         const float* r = m_hrec + kRecLen * slot;  // stats slots ST_DISP2 .. ST_CROSS + 3
         auto u = [&](int st) { return __builtin_bit_cast(uint, r[st - ST_DISP2]); };
         const float D = sqrtf(fmaxf(r[0], 0.0f));
@@ -1121,8 +1119,8 @@ void rxPBDSPH_GPU::poll_records() {
     }
 }
 
-// Copies the step's ST_DISP2..ST_FALLBACK behind its work (skipped while the ring is full: the host is more than
-// kRecs steps ahead of the device).
+// Copies the step's slots ST_DISP2 .. ST_CROSS + 3 behind its work, with an event (at most two records wait to be
+// applied; the copy is skipped only if the ring were full).
 void rxPBDSPH_GPU::push_record() {
     if (!m_hrec || m_rec_head - m_rec_tail >= kRecs) return;
     const int slot = (int)(m_rec_head % kRecs);

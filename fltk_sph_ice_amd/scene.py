@@ -250,10 +250,10 @@ class CoupledScene:
         return pos, vel, temps, phase
 
 
-def coupled_melt(fluid=(128, 128, 96), ice=(64, 128, 64), seed: int = 9) -> CoupledScene:
-    """Fluid + melting ice: 128 x 128 x 96 = 1,572,864 fluid particles (dam break at spacing h/2, h = 0.1,
-    plain-PBF parameters) beside a 64 x 128 x 64 = 524,288-particle ice block, 2,097,152 in all, in a
-    16 x 8 x 8 box.  Clusters from the ice neighbours within 2.5 spacings (uneven sizes up to 81 at the
+def coupled_melt(fluid=(128, 64, 192), ice=(64, 128, 64), seed: int = 9) -> CoupledScene:
+    """Fluid + melting ice: 128 x 64 x 192 = 1,572,864 fluid particles (a dam break 64 layers deep at spacing h/2,
+    h = 0.1, plain-PBF parameters) beside a 64 x 128 x 64 = 524,288-particle ice block, 2,097,152 in all, in a
+    16 x 8 x 14.4 box.  Clusters from the ice neighbours within 2.5 spacings (uneven sizes up to 81 at the
     block faces and corners), alpha 0.9, beta 0.3, 2 shape-matching iterations."""
     s = 0.05
     box = (2.5 * fluid[0] * s, 1.25 * max(fluid[1], ice[1]) * s, 1.5 * max(fluid[2], ice[2]) * s)

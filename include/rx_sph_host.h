@@ -117,13 +117,15 @@ enum rxPbfOption {
                                       hipGraph per buffer configuration and replayed (for launch-bound sizes);
                                       profiled steps (RX_PBF_OPT_PROFILE) run eagerly.  0 (default): eager */
     RX_PBF_OPT_ADAPT_SKIN = 18,    /* 1 (default): size the neighbour lists from the steps already completed:
-                                      the skin grows to 2, 3 or 4 times RX_PBF_OPT_SKIN (to 2.2 times the
-                                      largest in-step displacement) when more than 3% of a step's neighbour
-                                      passes walked the grid instead of their lists, and shrinks back after 2
-                                      steps that needed less; the list capacity grows by 64 (up to 512) when
-                                      lists overflow.  The host reads each step's figures once the device has
-                                      finished it, without waiting, so the sizes follow with a lag.  Eager steps
-                                      only (replayed graphs keep their sizes). */
+                                      the skin grows to 2, 3 or 4 times RX_PBF_OPT_SKIN when more than 3% of a
+                                      step's neighbour passes walked the grid instead of their lists (to the
+                                      level that holds the step's largest in-step displacement, or a lower one
+                                      at which the particles past half the skin are few enough to walk alone),
+                                      and shrinks back after 2 steps that needed less; the list capacity grows
+                                      with the cutoff volume and by 64 (up to 512) when lists overflow.  Step s
+                                      uses the figures of steps up to s - 2 (Update waits for step s - 2 if it
+                                      has not finished), so the sizes, and the run, do not depend on timing.
+                                      Eager steps only (replayed graphs keep their sizes). */
     RX_PBF_OPT_GRAD_MASS = 19,     /* 1: the constraint gradients carry the particle mass, grad C_i =
                                       (m / rho0) sum_j grad W_ij, in the multiplier's denominator and in the
                                       position correction (Macklin & Mueller 2013, section 3, with rho = sum m W).

@@ -301,3 +301,27 @@ def test_multi_step_parity_with_movers():
     assert movers > 0
     assert err < 1e-3
     assert s.stats()[22] == 0
+
+
+def test_adaptive_sizes_do_not_depend_on_host_timing():
+    """The list sizes of step s follow the records of steps up to s - 2 whatever the host's lead, so a run stepped
+    in one call and the same run synchronised after every step take the same sizes and end bitwise equal."""
+    sc, pos, vel = _colliding_halves()
+    runs = []
+    for stepwise in (False, True):
+        s = PBFSolver(sc, pos, vel)
+        levels = []
+        if stepwise:
+            for _ in range(40):
+                s.step(1)
+                s.sync()
+                levels.append(float(s.stats()[60]))
+        else:
+            s.step(40)
+        st = s.stats()
+        runs.append((s.positions(), s.velocities(), float(st[60]), float(st[61]), levels))
+        s.close()
+    print("skin levels stepwise:", sorted(set(runs[1][4])))
+    assert max(runs[1][4]) > 0.15  # the sizes did change during the run
+    assert runs[0][2:4] == runs[1][2:4]
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
