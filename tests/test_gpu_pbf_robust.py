@@ -66,8 +66,14 @@ def test_capped_walk_is_reported_and_recovers(lib):
     st = s.stats()
     assert st[22] == 1 and st[53] == 1, (st[22], st[53])
     lib.CuClearLastError()
-    rc = s.lib.rxPBDSPH_GPU_Update(s.h, C.c_float(sc.dt), 1)
-    assert rc == 3
+    # step s applies the records of steps up to s - 2: the call for step 2 reports the capped step 0 (and runs
+    # nothing); the call for step 1 runs normally
+    rcs = []
+    for _ in range(2):
+        rcs.append(s.lib.rxPBDSPH_GPU_Update(s.h, C.c_float(sc.dt), 1))
+        if rcs[-1] == 3:
+            break
+    assert rcs[-1] == 3, rcs
     assert b"reach" in lib.CuGetLastErrorString()
     lib.CuClearLastError()
     # the solver carries on from a fresh grid (a step that again outruns the reach is reported by the call

@@ -119,6 +119,7 @@ def test_list_tile_variants_agree_bitwise(squeeze):
     out = []
     for variant in range(5):
         s = PBFSolver(sc, pos, vel)
+        s.option(_abi.RX_PBF_OPT_ADAPT_SKIN, 0)  # fixed sizes: the squeezed lists keep overflowing
         s.option(_abi.RX_PBF_OPT_LIST_TILE, variant)
         s.option(_abi.RX_PBF_OPT_RECORD_FEATURES, 1)
         s.step(3)
