@@ -108,11 +108,11 @@ class Scene:
 
 
 # --------------------------------------------------------------------------------------------- workloads
-def wcsph_dam_break(nx: int = 32, ny: int = 32, nz: int = 32, seed: int = 1) -> Scene:
-    """WCSPH dam break: 32^3 block at spacing 0.5h, jitter 0.01h, unit box, h = 0.02, k = 3, mu = 3.5,
-    dt = 1e-3 (32,768 particles at the default size)."""
+def wcsph_dam_break(nx: int = 32, ny: int = 32, nz: int = 32, seed: int = 1, box: tuple = (1.0, 1.0, 1.0)) -> Scene:
+    """WCSPH dam break: block at spacing 0.5h, jitter 0.01h, h = 0.02, k = 3, mu = 3.5, dt = 1e-3; 32^3 = 32,768
+    particles in the unit box at the default size."""
     h = 0.02
-    return Scene("wcsph_dam_break", (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), h, 0.5 * h, (nx, ny, nz), (0.0, 0.0, 0.0),
+    return Scene("wcsph_dam_break", (0.0, 0.0, 0.0), tuple(box), h, 0.5 * h, (nx, ny, nz), (0.0, 0.0, 0.0),
                  0.01 * h, 0.0, seed, 1.0e-3, gas_k=3.0, viscosity=3.5)
 
 
