@@ -59,8 +59,8 @@ namespace {
 
 constexpr int kTpb = 256;
 constexpr int kMaxStencil = 4;  // largest cell radius of a fallback walk
-constexpr uint kMoverCap = 2048;  // movers listed per list build; beyond it every pass bounds displacements by the largest
-constexpr uint kMoverNear = 64;  // movers one workgroup stages; beyond it its particles bound displacements by the largest
+constexpr uint kMoverCap = 4096;  // movers listed per list build; beyond it every pass bounds displacements by the largest
+constexpr uint kMoverNear = 128;  // movers one workgroup stages; beyond it its particles bound displacements by the largest
 
 // Per-step statistics slots (float/uint in one array).
 enum Stat {
@@ -1099,7 +1099,7 @@ void rxPBDSPH_GPU::poll_records() {
             while (l < 3 && u(ST_CROSS + l) > allowance) ++l;
             return l;
         };
-        const uint allowance = std::min(kMoverCap / 2, m_n / 512);
+        const uint allowance = std::min(kMoverCap / 2, m_n / 256);
         const float passes = (float)(2 * std::min(m_env.max_iter, 14) + 2);
         const int want_up = std::max(std::min(want_d, want_m(allowance)), std::min(want_d, m_level_target + 1));
         const int want = std::min(want_d, want_m(allowance / 4));

@@ -164,7 +164,7 @@ enum rxProfileClass {
  *           from a fresh grid (the list is rebuilt)
  *   23      mean |rho - rho0| / rho0 at the step's start positions
  *   24      mean (rho - rho0)+ / rho0 at the step's start positions
- *   25      movers: particles that moved more than half the skin since the list build (at most 2048 are
+ *   25      movers: particles that moved more than half the skin since the list build (at most 4096 are
  *           listed; beyond that every particle bounds displacements by the largest one)
  *   27 + l  corrections that carried a particle past half the skin of level l (l < 4: 1 + l times the set
  *           skin), which the adaptive sizing reads
