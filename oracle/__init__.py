@@ -21,7 +21,7 @@ class OrcParams(C.Structure):
         ("bmin", C.c_double * 3), ("bmax", C.c_double * 3), ("origin", C.c_double * 3), ("cell", C.c_double * 3),
         ("gsize", C.c_int * 3), ("eps", C.c_double), ("use_ap", C.c_int), ("ap_k", C.c_double),
         ("ap_n", C.c_double), ("ap_q", C.c_double), ("xsph_c", C.c_double), ("iterations", C.c_int),
-        ("dt", C.c_double), ("grad_mass", C.c_int),
+        ("dt", C.c_double), ("grad_mass", C.c_int), ("first_scale", C.c_double), ("first_dmax", C.c_double),
     ]
 
 

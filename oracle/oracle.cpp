@@ -635,8 +635,18 @@ template <class T> void pbf_step(T* pos, T* vel, int n, const OrcParams& P, T* d
         neighbour_lists(ppos.data(), n, P, nb);  // the reference re-searches every iteration (rx_sph_pbd.cpp:427)
         pbf_lambda(ppos.data(), n, nb, P, dens.data(), lam.data());
         if (st && it < 16) density_stats(dens.data(), n, R<T>(P.rest_dens), &st->max_err[it], &st->mean_abs[it]);
+        if (it == 0 && P.first_scale != 0.0 && P.first_scale != 1.0)  // warm start by a constant factor
+            for (int i = 0; i < n; ++i) lam[i] = R<T>((double)lam[i] * R<T>(P.first_scale));
         if (lambda_sum) for (int i = 0; i < n; ++i) lambda_sum[i] = (T)((double)lambda_sum[i] + (double)lam[i]);
         pbf_dp(ppos.data(), lam.data(), n, nb, P, dt, dp.data());
+        if (it == 0 && P.first_dmax > 0.0)  // the first correction held to first_dmax
+            for (int i = 0; i < n; ++i) {
+                double l2 = 0.0;
+                for (int k = 0; k < 3; ++k) l2 += (double)dp[4 * i + k] * (double)dp[4 * i + k];
+                const double dmax = R<T>(P.first_dmax);
+                if (l2 > dmax * dmax)
+                    for (int k = 0; k < 3; ++k) dp[4 * i + k] = R<T>((double)dp[4 * i + k] * (dmax / std::sqrt(l2)));
+            }
         for (int i = 0; i < 4 * n; ++i) ppos[i] = (T)((double)ppos[i] + (double)dp[i]);
         integrate(ppos.data(), zero.data(), zero.data(), ppos.data(), pvel.data(), n, P, dt);  // collision only
     }

@@ -42,6 +42,10 @@ typedef struct OrcParams {
     double dt;
     int grad_mass;        /* PBF: 1 = constraint gradients carry the mass (m / rho0) sum grad W;
                              0 = the reference's grad W / rho0                    */
+    double first_scale;   /* PBF: factor on the multipliers of a step's first correction (0 or 1: none;
+                             the fused solver's RX_PBF_MODE_SCALED warm start)    */
+    double first_dmax;    /* PBF: largest displacement of a step's first correction (0: none;
+                             the fused solver's RX_PBF_OPT_FIRST_DMAX, in length units) */
 } OrcParams;
 
 /* Per-step PBF statistics. */

@@ -326,3 +326,33 @@ def test_adaptive_sizes_do_not_depend_on_host_timing():
     assert max(runs[1][4]) > 0.15  # the sizes did change during the run
     assert runs[0][2:4] == runs[1][2:4]
     assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+
+
+@pytest.mark.parametrize("mode,scale,dmax", [("scaled", 3.0, 0.0), ("scaled", 3.0, 0.02), ("plain", 1.0, 0.01)])
+def test_first_correction_factor_and_cap_match_oracle(mode, scale, dmax):
+    # RX_PBF_MODE_SCALED (a constant factor on the first multipliers) and RX_PBF_OPT_FIRST_DMAX (the first
+    # correction held to dmax h) against the oracle's first_scale / first_dmax, one step, all 4 corrections
+    sc = scene.pbf_dam_break(14, 14, 14)
+    pos, vel = sc.particles()
+    P = oracle.params_from(sc.params(), sc)
+    P.first_scale = scale
+    P.first_dmax = dmax * sc.h
+    rpos, rvel, _, _ = oracle.pbf_step(pos, vel, P)
+    P.first_dmax = 0.0
+    upos, _, _, _ = oracle.pbf_step(pos, vel, P)
+    s = PBFSolver(sc, pos, vel)
+    if mode == "scaled":
+        s.option(_abi.RX_PBF_OPT_MODE, _abi.RX_PBF_MODE_SCALED)
+        s.option(_abi.RX_PBF_OPT_WARM_SCALE, scale)
+    s.option(_abi.RX_PBF_OPT_FIRST_DMAX, dmax)
+    s.step(1)
+    gpos, gvel = s.positions(), s.velocities()
+    s.close()
+    box = np.abs(rpos[:, :3]).max()
+    err = np.abs(gpos[:, :3] - rpos[:, :3]).max() / box
+    print(f"{mode} x{scale} cap {dmax} h: GPU vs oracle {err:.2e} of the box; cap moved the result by "
+          f"{np.abs(upos[:, :3] - rpos[:, :3]).max() / sc.h:.3f} h")
+    assert err <= 1e-5
+    assert np.abs(gvel[:, :3] - rvel[:, :3]).max() <= 1e-3 * np.abs(rvel[:, :3]).max()
+    if dmax > 0:
+        assert np.abs(upos[:, :3] - rpos[:, :3]).max() > 1e-3 * sc.h  # the cap bit

@@ -443,3 +443,39 @@ def test_pbf_step_fp64_matches_fp32_to_rounding(orc):
     for k in range(sc.iterations):
         assert st32.max_err[k] == pytest.approx(st64.max_err[k], rel=1e-3, abs=1e-6)
     assert st64.iterations == sc.iterations
+
+
+# ------------------------------------------------------------------------------------------ first-correction factor and cap
+def test_first_correction_factor_scales_and_cap_bounds_the_correction(orc):
+    # One correction, no s_corr, no XSPH: the correction is sum_j (lambda_i + lambda_j) grad W, linear in the
+    # multipliers, so a factor s on them gives s times the correction (to FP64 rounding of the FP32 sums);
+    # a cap holds every particle's correction to that length and leaves the shorter ones as they were.
+    from fltk_sph_ice_amd import scene
+
+    sc = scene.pbf_dam_break(10, 10, 10)
+    pos, vel = sc.particles()
+    P = oracle.params_from(sc.params(), sc)
+    P.use_ap = 0
+    P.xsph_c = 0.0
+    P.iterations = 0
+    pred, _, _, _ = oracle.pbf_step(pos, vel, P, double=True)  # x*: the prediction alone
+    P.iterations = 1
+    one, _, _, _ = oracle.pbf_step(pos, vel, P, double=True)
+    P.first_scale = 2.0
+    two, _, _, _ = oracle.pbf_step(pos, vel, P, double=True)
+    d1 = one[:, :3] - pred[:, :3]
+    d2 = two[:, :3] - pred[:, :3]
+    inside = np.all((two[:, :3] > sc.box_min[0] + 0.06) & (two[:, :3] < np.asarray(sc.box_max) - 0.06), axis=1)  # no wall clamp
+    assert inside.sum() > 100
+    np.testing.assert_allclose(d2[inside], 2.0 * d1[inside], rtol=1e-6, atol=1e-12)
+    P.first_scale = 1.0
+    cap = 0.5 * float(np.linalg.norm(d1, axis=1).max())
+    P.first_dmax = cap
+    capped, _, _, _ = oracle.pbf_step(pos, vel, P, double=True)
+    dc = capped[:, :3] - pred[:, :3]
+    lc, l1 = np.linalg.norm(dc, axis=1), np.linalg.norm(d1, axis=1)
+    assert lc.max() <= cap * (1 + 1e-6)
+    short = l1 < cap * 0.999
+    assert np.array_equal(capped[short], one[short]) and (~short).sum() > 0
+    longer = ~short & inside
+    np.testing.assert_allclose(dc[longer], d1[longer] * (cap / l1[longer])[:, None], rtol=1e-6)
