@@ -77,6 +77,7 @@ enum Stat {
     ST_MOVERS = 25,          // movers listed since the list build (uint; kept until the next build)
     ST_MOVERS_PUB = 26,      // ST_MOVERS as of the last lambda pass: the movers a correction pass may read
     ST_CROSS = 27,           // .. 27 + l (l < 4): corrections that carried a particle past half the skin of level l
+    ST_LIST_CHUNKS = 31,     // 16-byte chunks stored by the step's list build (uint; the lists the passes walk)
     ST_MPOS_ITER = 32,       // .. 32 + k: the same at the start of Jacobi iteration k (k < 16)
     ST_RUN = 48,             // run accumulators from here on, kept across steps:
     ST_RUN_MAX = 48,         //   max over the judged steps of the max error
@@ -85,6 +86,7 @@ enum Stat {
     ST_RUN_STEPS = 51,       //   judged steps (uint)
     ST_RUN_FAIL = 52,        //   judged steps whose judged error exceeded tol (uint)
     ST_RUN_BAD = 53,         //   1 once any pass since the reset had its fallback reach capped (uint)
+    ST_RUN_CHUNKS = 54,      //   list chunks stored by the list builds (float sum)
     ST_N = 64
 };
 
@@ -137,6 +139,25 @@ __device__ inline void block_sum_atomic(float v, float* dst) {
         atomicAdd(dst, m);
     }
     __syncthreads();  // ws is read by thread 0 above: the next reduction of the block must not overwrite it first
+}
+
+// Block-level sum of a uint, then one atomic per block (and its value added to run_dst as a float).
+__device__ inline void block_sum_atomic_u(uint v, uint* dst, float* run_dst) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    __shared__ uint wsu[kTpb / 64];
+    if (lane == 0) wsu[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint m = 0;
+        for (uint w = 0; w < (blockDim.x >> 6); ++w) m += wsu[w];
+        if (m) {
+            atomicAdd(dst, m);
+            atomicAdd(run_dst, (float)m);
+        }
+    }
+    __syncthreads();
 }
 
 __device__ inline uint slot_of(const PbfK& k, bool& valid) {
@@ -479,8 +500,8 @@ __device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, const 
                                         float4* __restrict__ xbuild,
                                         float* __restrict__ dens, uint* __restrict__ list, uint4* __restrict__ cnt,
                                         float* __restrict__ stats, const PbfK& k, float* __restrict__ feat,
-                                        const float4* __restrict__ v, float& emax, float& esum, unsigned short* xrow = nullptr,
-                                        const MlpShared* snet = nullptr) {
+                                        const float4* __restrict__ v, float& emax, float& esum, uint& chunks,
+                                        unsigned short* xrow = nullptr, const MlpShared* snet = nullptr) {
     const bool want_feat = feat || xrow;
     float3 p = f3(p4);
     float d = 0.0f, sd = 0.0f;
@@ -553,6 +574,7 @@ __device__ inline void list_lambda_slot(uint i, float4 p4, Cands&& cands, const 
     }
     const bool over = c > (uint)k.maxn;  // (kWide makes c larger than any capacity)
     cnt[i] = make_uint4(base0, base1, base2, over ? (c | kListOverflow) : c);
+    chunks += over ? 0u : (c + 7u) >> 3;
     if (over) atomicAdd((unsigned int*)&stats[ST_OVERFLOW], 1u);
     d *= k.s.mass * k.s.wpoly6;
     float sgs = dot3(gs, gs);
@@ -643,11 +665,12 @@ __global__ __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(6))) void 
     bool valid;
     uint i = slot_of(k, valid);
     float emax = 0.0f, esum = 0.0f;
+    uint chunks = 0;
     unsigned short* xrow = kNet ? s_rows + threadIdx.x * kMlpIn : nullptr;
     if (valid) {
         const float4 p4 = xs[i];
         auto cands = [&](auto&& plane, auto&& f) { build_candidates(i, k, xs, bhash, off, plane, f); };
-        list_lambda_slot(i, p4, cands, xs, xl, xbuild, dens, list, cnt, stats, k, feat, v, emax, esum, xrow, s_net);
+        list_lambda_slot(i, p4, cands, xs, xl, xbuild, dens, list, cnt, stats, k, feat, v, emax, esum, chunks, xrow, s_net);
     }
     if (kNet) {  // every lane of the wavefront takes part (rows of lanes past the end are zero; their outputs unused)
         if (!valid) ((uint4*)xrow)[0] = ((uint4*)xrow)[1] = make_uint4(0u, 0u, 0u, 0u);
@@ -656,6 +679,7 @@ __global__ __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(6))) void 
     }
     block_max_atomic(emax, &stats[ST_ERR_ITER + 0]);
     block_sum_atomic(esum, &stats[ST_MPOS_ITER + 0]);
+    block_sum_atomic_u(chunks, (uint*)&stats[ST_LIST_CHUNKS], &stats[ST_RUN_CHUNKS]);
 }
 
 // Alternative with LDS staging (RX_PBF_OPT_LIST_TILE 1..4).  One workgroup per segment of kSegW cells along x
@@ -724,6 +748,7 @@ __global__ __launch_bounds__(kThreads) void k_list_lambda(const float4* __restri
         __syncthreads();
     }
     float emax = 0.0f, esum = 0.0f;
+    uint chunks = 0;
     for (uint i = hs + threadIdx.x; i < he; i += blockDim.x) {
         const int cx = (int)(bhash[i] % (uint)g.gx);
         const int a = max(cx - 1, 0) - xl0, b = min(cx + 1, g.gx - 1) - xl0 + 1;
@@ -751,10 +776,11 @@ __global__ __launch_bounds__(kThreads) void k_list_lambda(const float4* __restri
                 }
             }
         };
-        list_lambda_slot(i, p4, cands, xs, xl, xbuild, dens, list, cnt, stats, k, feat, v, emax, esum);
+        list_lambda_slot(i, p4, cands, xs, xl, xbuild, dens, list, cnt, stats, k, feat, v, emax, esum, chunks);
     }
     block_max_atomic(emax, &stats[ST_ERR_ITER + 0]);
     block_sum_atomic(esum, &stats[ST_MPOS_ITER + 0]);
+    block_sum_atomic_u(chunks, (uint*)&stats[ST_LIST_CHUNKS], &stats[ST_RUN_CHUNKS]);
 }
 
 // lambda pass (rx_sph_pbd.cpp:699-822) over the list: reads positions xa, writes xl = (x, lambda).
@@ -1635,7 +1661,7 @@ int rxPBDSPH_GPU::GetStats(float* out, int count) {
     h[ST_MPOS_START] = h[ST_MPOS_START] / (float)m_n;
     for (int t = 0; t < 16; ++t) h[ST_MPOS_ITER + t] = h[ST_MPOS_ITER + t] / (float)m_n;
     for (int t : {(int)ST_RUN_ITERS, (int)ST_RUN_STEPS, (int)ST_RUN_FAIL, (int)ST_RUN_BAD, (int)ST_MOVERS, (int)ST_MOVERS_PUB,
-                  ST_CROSS + 0, ST_CROSS + 1, ST_CROSS + 2, ST_CROSS + 3})
+                  ST_CROSS + 0, ST_CROSS + 1, ST_CROSS + 2, ST_CROSS + 3, (int)ST_LIST_CHUNKS})
         h[t] = (float)__builtin_bit_cast(uint, h[t]);
     h[ST_DISP2] = sqrtf(h[ST_DISP2]);
     h[60] = m_skin / m_env.h;  // host state: the list sizes in use

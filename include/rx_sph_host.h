@@ -168,12 +168,15 @@ enum rxProfileClass {
  *           listed; beyond that every particle bounds displacements by the largest one)
  *   27 + l  corrections that carried a particle past half the skin of level l (l < 4: 1 + l times the set
  *           skin), which the adaptive sizing reads
+ *   31      16-byte list chunks the step's list build stored (the passes read these plus a 16-byte head per
+ *           particle)
  *   32 + k  slot 24 at the start of Jacobi iteration k (k < 16)
  * Run accumulators (since SetParticles or RX_PBF_OPT_RESET_RUN), over the steps judged so far (a step is
  * judged by the prediction pass of the step after it):
  *   48      max of slot 0        49  max of slot 24      50  corrections executed
  *   51      steps judged         52  steps whose judged error exceeded TOL
  *   53      1 once any pass since the reset needed a walk beyond the cap (slot 22 of any step)
+ *   54      list chunks stored by the list builds since the reset (sum of slot 31)
  * Host state at the call:
  *   60      list skin in use, as a fraction of h      61  list capacity in use                     */
 
