@@ -292,6 +292,13 @@ CXX_SYMBOLS = {
     "LaunchInterPolationGPU": ("_Z22LaunchInterPolationGPUiPfS_S_S_", (None, [i32, vp, vp, vp, vp])),
     "LaunchHeatTransferGPU": ("_Z21LaunchHeatTransferGPUPfS_S_PKiS1_PKfPiS4_S1_fffffi",
                               (None, [vp, vp, vp, vp, vp, vp, vp, vp, vp, flt, flt, flt, flt, flt, i32])),
+    # the prefix-sum ("path") shape matching: linked with the reference's signatures, records error 5
+    "LaunchShapeMatchingUsePathGPU": ("_Z29LaunchShapeMatchingUsePathGPUiPfS_S_S_S_S_S_S_PiS0_f",
+                                      (None, [i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, flt])),
+    "LaunchUpdatePrefixSumGPU": ("_Z24LaunchUpdatePrefixSumGPUiiiiiPiPsPfS1_S0_S1_S1_PjS2_PKfS4_",
+                                 (None, [i32] * 5 + [vp] * 11)),
+    "LauchUpdateSMFromPath": ("_Z21LauchUpdateSMFromPathiPfS_S_S_S_S_S_S_PiS0_PsS1_S_S_S0_S0_iif",
+                              (None, [i32] + [vp] * 17 + [i32, i32, flt])),
 }
 
 (RX_ARRAY_POSITION, RX_ARRAY_VELOCITY, RX_ARRAY_DENSITY, RX_ARRAY_LAMBDA_SUM, RX_ARRAY_PRESSURE, RX_ARRAY_FEATURES,

@@ -13,8 +13,9 @@
  *    Clusters: one per particle, {p} plus every particle within `cluster_radius` of p in the rest state
  *    (IceObject::SetClusterMoveInfoFromNeight, IceObject.cpp:608-633).
  *
- * 2. The C++-linkage launch functions of Ice_SM.h:23-36, IceObject.h:388-395 and HeatTransfer.cu:19-36, same
- *    names and arguments.
+ * 2. The C++-linkage launch functions of Ice_SM.h:23-69, IceObject.h:370-425, HeatTransfer.cu:19-36 and
+ *    Surf_SM.h:34-52, same names and arguments (the three prefix-sum "path" launches record an error and do
+ *    nothing).
  *
  * Entry points returning int give 0 on success; CuGetLastErrorString (rx_cu_funcs.h) holds the message.
  */
@@ -108,6 +109,19 @@ void LaunchInterPolationGPU(int prtNum, float* sldPrtPos, float* sldPrtVel, floa
 void LaunchHeatTransferGPU(float* heats, float* temps, float* dtemps, const int* surfParticles, const int* neightsParticles,
                            const float* densty, int* phase, int* phaseChangef, const int* meltPrtIndx, float radius, float airTemp,
                            float cffCnHt, float cffCnTd, float dt, int prtNum);
+/* Surf_SM.h:34-52 (LaunchUpdatePrefixSumGPU), Ice_SM.h:55-69 (LaunchShapeMatchingUsePathGPU) and
+ * IceObject.h:399-425 (LauchUpdateSMFromPath, the reference's spelling): the prefix-sum ("path") variant of shape
+ * matching.  Not implemented: all three link with the reference's C++ signatures, do no work and record error 5
+ * (CuGetLastErrorCode). */
+void LaunchUpdatePrefixSumGPU(int prtNum, int PosSizeX, int PosSizeY, int ApqSizeX, int ApqSizeY, int* md_2DiPTHtoPRT,
+                              short* md_2DiPRTtoPTH, float* md_2Df3PrfxPos, float* md_2Df9PrfxApq, short* md_3DiPTHandPrfxSet,
+                              float* md_f3OrgPos, float* md_f3OrgCm, unsigned* dgroupPos, unsigned* dgroupApq,
+                              const float* md_fPos, const float* md_fVel);
+void LaunchShapeMatchingUsePathGPU(int prtNum, float* prtPos, float* prtVel, float* orgPos, float* curPos, float* orgCm,
+                                   float* curCm, float* curApq, float* vel, int* pIndxes, int* d_IndxSet, float dt);
+void LauchUpdateSMFromPath(int prtNum, float* prtPos, float* prtVel, float* orgPos, float* curPos, float* orgCm, float* curCm,
+                           float* clstrApq, float* vel, int* pIndxes, int* startEndSet, short* PRTtoPTH, short* PTHandPrfxSet,
+                           float* prfxPos, float* prfxApq, int* CtoP, int* CtoPNum, int CtoPSizeY, int CtoPSizeZ, float dt);
 #endif
 
 #endif /* RX_ICE_FUNCS_H_ */

@@ -75,3 +75,35 @@ def test_out_of_scope_entry_points_report_an_error(lib):
     lib.CuClearLastError()
     assert lib.CuCheckNumValidHashData() == 0
     lib.CuClearLastError()
+
+
+def test_path_shape_matching_links_with_the_reference_signatures_and_reports_an_error(lib):
+    # Surf_SM.h:34-52, Ice_SM.h:55-69 and IceObject.h:399-425 (C++ linkage): the stubs take the reference's argument lists and touch
+    # no memory, so null pointers are safe here
+    import subprocess
+
+    from fltk_sph_ice_amd._abi import CXX_SYMBOLS
+
+    demangled = subprocess.run(["c++filt"], input="\n".join(CXX_SYMBOLS[n][0] for n in ("LaunchShapeMatchingUsePathGPU",
+                                                                                     "LauchUpdateSMFromPath")),
+                               capture_output=True, text=True).stdout.splitlines()
+    assert demangled[0] == ("LaunchShapeMatchingUsePathGPU(int, float*, float*, float*, float*, float*, float*, float*, float*, "
+                            "int*, int*, float)")
+    assert demangled[1] == ("LauchUpdateSMFromPath(int, float*, float*, float*, float*, float*, float*, float*, float*, int*, "
+                            "int*, short*, short*, float*, float*, int*, int*, int, int, float)")
+    lib.CuClearLastError()
+    lib.LaunchShapeMatchingUsePathGPU(4, *([None] * 10), C.c_float(0.01))
+    assert lib.CuGetLastErrorCode() == 5
+    assert b"LaunchShapeMatchingUsePathGPU" in lib.CuGetLastErrorString()
+    lib.CuClearLastError()
+    lib.LauchUpdateSMFromPath(4, *([None] * 17), 8, 2, C.c_float(0.01))
+    assert lib.CuGetLastErrorCode() == 5
+    assert b"LauchUpdateSMFromPath" in lib.CuGetLastErrorString()
+    lib.CuClearLastError()
+    lib.LaunchUpdatePrefixSumGPU(4, 1, 1, 1, 1, *([None] * 11))
+    assert lib.CuGetLastErrorCode() == 5
+    assert b"LaunchUpdatePrefixSumGPU" in lib.CuGetLastErrorString()
+    lib.CuClearLastError()
+    assert subprocess.run(["c++filt", CXX_SYMBOLS["LaunchUpdatePrefixSumGPU"][0]], capture_output=True, text=True).stdout.strip() == (
+        "LaunchUpdatePrefixSumGPU(int, int, int, int, int, int*, short*, float*, float*, short*, float*, float*, unsigned int*, "
+        "unsigned int*, float const*, float const*)")
