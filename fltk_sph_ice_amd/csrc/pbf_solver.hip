@@ -289,25 +289,35 @@ __device__ inline float len2_fixed(float3 r) { return __builtin_fmaf(r.z, r.z, _
 
 // Movers: particles whose displacement since the list build exceeded skin/2 at some correction this step.  The
 // correction pass lists each once (build position, slot) the first time it crosses; they stay listed until the
-// next build.  The movers a workgroup's particles may meet are staged in LDS: n = -1 when the workgroup bounds
-// every displacement by the largest one instead (too many movers in the step, or near the workgroup).
+// next build.  Every particle that is not listed is within skin/2 of its build position.  The movers a
+// workgroup's particles may meet are staged in LDS with their displacement at the pass's positions: n = -1 when
+// the workgroup bounds every displacement by the largest one instead (too many movers in the step, or near the
+// workgroup).
 struct NearMovers {
     const float4* x;   // build position, slot bits
     const uint* cell;  // build cell hash
+    const float* d;    // displacement from the build position at the pass's positions
     int n;
 };
 
-// Call from every thread of the workgroup (the result is the same for all).  A particle i that has moved at most
-// skin/2 meets a mover j only if |xb_i - xb_j| <= h + skin/2 + D, so the movers within that reach of the box of
-// the workgroup's build positions are staged.  `published`: read the movers as of the last lambda pass (a
+// Rounding margin of the reach tests below (their distances are sums of rounded lengths).
+constexpr float kReachMargin = 1.0001f;
+
+// Call from every thread of the workgroup (the result is the same for all).  xp: the positions the pass reads.
+// A particle i that has moved d_i meets a mover j (moved d_j) only if |xb_i - xb_j| <= h + d_i + d_j, so a mover
+// is staged when it lies within h + d_w + d_j of the box of the workgroup's build positions, d_w = the largest
+// of skin/2 and the displacements of the workgroup's particles: one fast particle widens only its own
+// workgroup's reach and the movers near it.  `published`: read the movers as of the last lambda pass (a
 // correction pass appends to the list while it runs).
 __device__ inline NearMovers stage_movers(uint i, bool valid, const PbfK& k, float disp2, const float* __restrict__ stats,
-                                          const float4* __restrict__ xb, const uint* __restrict__ bhash, bool published) {
+                                          const float4* __restrict__ xb, const float4* __restrict__ xp,
+                                          const uint* __restrict__ bhash, bool published) {
     __shared__ float4 s_x[kMoverNear];
     __shared__ uint s_cell[kMoverNear];
-    __shared__ float s_box[6][kTpb / 64];
+    __shared__ float s_d[kMoverNear];
+    __shared__ float s_box[7][kTpb / 64];
     __shared__ uint s_n;
-    NearMovers nm{s_x, s_cell, 0};
+    NearMovers nm{s_x, s_cell, s_d, 0};
     if (!(disp2 > k.half_skin2)) return nm;  // no particle has moved more than skin/2: the lists hold every pair
     const uint nmov = __float_as_uint(stats[published ? ST_MOVERS_PUB : ST_MOVERS]);
     if (nmov > kMoverCap) {
@@ -315,10 +325,11 @@ __device__ inline NearMovers stage_movers(uint i, bool valid, const PbfK& k, flo
         return nm;
     }
     if (nmov == 0) return nm;
-    float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    float b[7] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY, 0.0f};
     if (valid) {
         const float4 q = xb[i];
         b[0] = b[3] = q.x, b[1] = b[4] = q.y, b[2] = b[5] = q.z;
+        b[6] = len2_fixed(f3(xp[i]) - f3(q));
     }
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) {
@@ -327,24 +338,34 @@ __device__ inline NearMovers stage_movers(uint i, bool valid, const PbfK& k, flo
             b[c] = fminf(b[c], __shfl_xor(b[c], d, 64));
             b[c + 3] = fmaxf(b[c + 3], __shfl_xor(b[c + 3], d, 64));
         }
+        b[6] = fmaxf(b[6], __shfl_xor(b[6], d, 64));
     }
     if ((threadIdx.x & 63) == 0)
-        for (int c = 0; c < 6; ++c) s_box[c][threadIdx.x >> 6] = b[c];
+        for (int c = 0; c < 7; ++c) s_box[c][threadIdx.x >> 6] = b[c];
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
         for (int c = 0; c < 3; ++c) {
             b[c] = fminf(b[c], s_box[c][w]);
             b[c + 3] = fmaxf(b[c + 3], s_box[c + 3][w]);
         }
-    const float R = k.s.h + 0.5f * k.skin + sqrtf(disp2);
+        b[6] = fmaxf(b[6], s_box[6][w]);
+    }
+    const float dw = fmaxf(0.5f * k.skin, sqrtf(b[6]));
+    const float R0 = (k.s.h + dw + sqrtf(disp2)) * kReachMargin;  // every mover's displacement is at most sqrt(disp2)
     for (uint m = threadIdx.x; m < nmov; m += blockDim.x) {
         const float4 mv = k.movers[m];
-        if (mv.x >= b[0] - R && mv.x <= b[3] + R && mv.y >= b[1] - R && mv.y <= b[4] + R && mv.z >= b[2] - R && mv.z <= b[5] + R) {
-            const uint t = atomicAdd(&s_n, 1u);
-            if (t < kMoverNear) {
-                s_x[t] = mv;
-                s_cell[t] = bhash[min(__float_as_uint(mv.w), k.n - 1u)];
+        if (mv.x >= b[0] - R0 && mv.x <= b[3] + R0 && mv.y >= b[1] - R0 && mv.y <= b[4] + R0 && mv.z >= b[2] - R0 && mv.z <= b[5] + R0) {
+            const uint slot = min(__float_as_uint(mv.w), k.n - 1u);
+            const float dj = sqrtf(len2_fixed(f3(xp[slot]) - f3(mv)));
+            const float R = (k.s.h + dw + dj) * kReachMargin;
+            if (mv.x >= b[0] - R && mv.x <= b[3] + R && mv.y >= b[1] - R && mv.y <= b[4] + R && mv.z >= b[2] - R && mv.z <= b[5] + R) {
+                const uint t = atomicAdd(&s_n, 1u);
+                if (t < kMoverNear) {
+                    s_x[t] = mv;
+                    s_cell[t] = bhash[slot];
+                    s_d[t] = dj;
+                }
             }
         }
     }
@@ -358,12 +379,14 @@ __device__ inline NearMovers stage_movers(uint i, bool valid, const PbfK& k, flo
 // `xb` the build positions, d = displacement since the build, D = its largest value (disp2 = D^2).
 // mode 1: the 3x3x3 stencil of a grid built on the current positions (cell >= h).
 // mode 0: the skin list, which holds every j with |xb_i - xb_j| <= h + skin.  A pair within h now was within
-// h + d_i + d_j at the build, so the list of i alone holds every such j while d_i + D <= skin (always so while
-// D <= skin/2).  With movers staged (nm.n >= 0) a particle with d_i <= skin/2 walks its list, which holds every
-// j that has stayed within skin/2, and then the staged movers its list does not hold (the build's own test on
-// the build positions and cells decides that, so no pair is visited twice).  Every other particle (and every
-// overflowed list) walks the build-time grid over the cells within L = h + d_i + D of its build position, which
-// hold every such j; the radius is capped at kMaxStencil cells (beyond it the step and the run are flagged).
+// h + d_i + d_j at the build.  With movers staged (nm.n >= 0): every particle that is not a mover has d <= skin/2,
+// so a particle with d_i <= skin/2 walks its list, which holds every such j, and then the staged movers its list
+// does not hold (the build's own test on the build positions and cells decides that, so no pair is visited
+// twice); a mover or an overflowed list walks the build-time grid over the cells within L = h + d_i + min(skin/2,
+// D) of its build position, which hold every j that is not a mover, and then the staged movers outside those
+// cells.
+// Without staged movers (nm.n < 0) a particle walks its list while d_i + D <= skin, else the grid within
+// L = h + d_i + D.  The grid radius is capped at kMaxStencil cells (beyond it the step and the run are flagged).
 template <int G = 1, class Fetch, class Body>
 __device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint* __restrict__ list, const uint4* __restrict__ cnt,
                                      const uint* __restrict__ bhash, const uint* __restrict__ off, float disp2, float3 p,
@@ -381,7 +404,7 @@ __device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint
         di = sqrtf(di2);
         if (nm.n >= 0) {
             ok = ok && di2 <= k.half_skin2;
-            movers = ok && nm.n > 0;
+            movers = nm.n > 0;
         } else {
             ok = ok && di + sqrtf(disp2) <= k.skin;
         }
@@ -391,10 +414,10 @@ __device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint
         if (movers) {
             const float3 bi = f3(xb[i]);
             const int3 ci = fg_unhash(bhash[i], k.g);
-            const float R = k.s.h + di + sqrtf(disp2);
             for (int m = 0; m < nm.n; ++m) {
                 const float4 mv = nm.x[m];
                 const float r2 = len2_fixed(bi - f3(mv));
+                const float R = (k.s.h + di + nm.d[m]) * kReachMargin;
                 if (r2 > R * R) continue;  // out of reach now
                 const int3 cj = fg_unhash(nm.cell[m], k.g);
                 const bool listed = abs(cj.x - ci.x) <= 1 && abs(cj.y - ci.y) <= 1 && abs(cj.z - ci.z) <= 1 && r2 <= k.rc2;
@@ -409,14 +432,26 @@ __device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint
         // axis once L exceeds a cell, not the 5 of a cube of radius 2 around the cell)
         const uint64_t wave = __ballot(1);
         if (__lane_id() == (uint)__builtin_ctzll(wave)) atomicAdd(k.fallback, (uint)__builtin_popcountll(wave));
-        const float Lwant = (k.s.h + di + sqrtf(disp2)) * 1.0001f;
+        const bool staged = nm.n >= 0;
+        const float D = sqrtf(disp2);
+        const float Lwant = (k.s.h + di + (staged ? fminf(0.5f * k.skin, D) : D)) * kReachMargin;
         const float L = fminf(Lwant, kMaxStencil * k.g.cell);
         if (Lwant > L) {  // the capped walk may miss pairs: flag the step and the run (never silently)
             atomicOr(k.bad, 1u);
             atomicOr(k.bad + (ST_RUN_BAD - ST_BAD), 1u);
         }
         const float3 c = f3(xb[i]);
-        stencil_box(fg_pos(c - f3(L, L, L), k.g), fg_pos(c + f3(L, L, L), k.g), k.g, off, fetch, body);
+        const int3 lo = fg_pos(c - f3(L, L, L), k.g), hi = fg_pos(c + f3(L, L, L), k.g);
+        stencil_box(lo, hi, k.g, off, fetch, body);
+        for (int m = 0; m < (staged ? nm.n : 0); ++m) {  // the movers beyond those cells
+            const float4 mv = nm.x[m];
+            const int3 cj = fg_unhash(nm.cell[m], k.g);
+            if (cj.x >= lo.x && cj.x <= hi.x && cj.y >= lo.y && cj.y <= hi.y && cj.z >= lo.z && cj.z <= hi.z) continue;
+            const float R = (k.s.h + di + nm.d[m]) * kReachMargin;
+            if (len2_fixed(c - f3(mv)) > R * R) continue;
+            const uint j = __float_as_uint(mv.w);
+            body(j, fetch(j));
+        }
     }
 }
 
@@ -435,7 +470,7 @@ __global__ __launch_bounds__(kTpb) void k_predict(const float4* __restrict__ x, 
     bool valid;
     uint i = slot_of(k, valid);
     float disp2 = stats[ST_DISP2];
-    const NearMovers nm = mode == 0 ? stage_movers(i, valid, k, disp2, stats, xb, bhash, false) : NearMovers{nullptr, nullptr, 0};
+    const NearMovers nm = mode == 0 ? stage_movers(i, valid, k, disp2, stats, xb, x, bhash, false) : NearMovers{nullptr, nullptr, nullptr, 0};
     float err = 0.0f, aerr = 0.0f;
     if (valid) {
         float3 p = f3(x[i]), vi = f3(v[i]);
@@ -793,7 +828,7 @@ __global__ __launch_bounds__(kTpb) void k_lambda(const float4* __restrict__ xa, 
     bool valid;
     uint i = slot_of(k, valid);
     float disp2 = stats[ST_DISP2];
-    const NearMovers nm = stage_movers(i, valid, k, disp2, stats, xb, bhash, false);
+    const NearMovers nm = stage_movers(i, valid, k, disp2, stats, xb, xa, bhash, false);
     // the movers the next correction pass reads (it appends new ones while it runs)
     if (blockIdx.x == 0 && threadIdx.x == 0) stats[ST_MOVERS_PUB] = stats[ST_MOVERS];
     float err = 0.0f;
@@ -845,7 +880,7 @@ __global__ __launch_bounds__(kTpb) void k_dp(const float4* __restrict__ xl, floa
     bool valid;
     uint i = slot_of(k, valid);
     float disp2 = stats[ST_DISP2];
-    const NearMovers nm = stage_movers(i, valid, k, disp2, stats, xbuild, bhash, true);
+    const NearMovers nm = stage_movers(i, valid, k, disp2, stats, xbuild, xl, bhash, true);
     float mv = 0.0f, mv0 = 0.0f;
     if (valid) {
         float4 p4 = xl[i];
@@ -912,7 +947,7 @@ __global__ __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(6))) void 
     bool valid;
     uint i = slot_of(k, valid);
     float disp2 = stats[ST_DISP2];
-    const NearMovers nm = stage_movers(i, valid, k, disp2, stats, xb, bhash, false);
+    const NearMovers nm = stage_movers(i, valid, k, disp2, stats, xb, xa, bhash, false);
     if (!valid) return;
     float3 p = f3(xa[i]);
     float3 vi = f3(v[i]);
@@ -944,7 +979,7 @@ __global__ __launch_bounds__(kTpb) void k_heat_diffuse(const float4* __restrict_
     bool valid;
     uint i = slot_of(k, valid);
     float disp2 = stats[ST_DISP2];
-    const NearMovers nm = stage_movers(i, valid, k, disp2, stats, xb, bhash, false);
+    const NearMovers nm = stage_movers(i, valid, k, disp2, stats, xb, x, bhash, false);
     if (!valid) return;
     float3 p = f3(x[i]);
     float ti = therm[i].x;
