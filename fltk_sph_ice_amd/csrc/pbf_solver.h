@@ -70,6 +70,7 @@ private:
     float m_warm_scale = 1.0f;
     float m_warm_kappa = 0.0f;   // RX_PBF_OPT_WARM_CLAMP
     float m_first_dmax = 0.0f;   // RX_PBF_OPT_FIRST_DMAX, as a fraction of h
+    int m_defer_on = 1;          // RX_PBF_OPT_DEFER
     int m_judge = 0;
     bool m_grad_mass = false;  // RX_PBF_OPT_GRAD_MASS
     int m_list_tile = 0;  // LDS tiling variant of the list pass (RX_PBF_OPT_LIST_TILE)
@@ -99,6 +100,7 @@ private:
     uint4* m_cnt = nullptr;     // list head per slot: 3 plane bases + count (bit 31: use the grid instead)
     float* m_stats = nullptr;
     float4* m_movers = nullptr;  // particles that moved more than skin/2 since the list build (bounded list)
+    uint* m_defer = nullptr;     // slots a neighbour pass left to its deferred kernel
     // profiling
     bool m_profile = false;
     unsigned m_profile_mask = ~0u;  // rxProfileClass bits timed while profiling

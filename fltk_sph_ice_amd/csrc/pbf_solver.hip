@@ -36,6 +36,9 @@ struct PbfK {
     uint* fallback;   // counts the passes of particles that walked the grid instead of their list
     uint* bad;        // stats slots ST_BAD and ST_RUN_BAD: set when a walk's reach was capped (pairs missed)
     float4* movers;   // (build position, slot bits) of the particles that moved more than skin/2 since the build
+    uint* defer;      // slots whose grid walk a pass left to its deferred kernel (kDeferCap entries)
+    uint* defer_n;    // their count (stats slot ST_DEFER)
+    int defer_on;     // 0: every particle walks in its own pass
     float cross2[4];  // (skin of level l / 2)^2: the crossings the skin sizing counts
     int maxn;
     float eps;
@@ -61,6 +64,8 @@ constexpr int kTpb = 256;
 constexpr int kMaxStencil = 4;  // largest cell radius of a fallback walk
 constexpr uint kMoverCap = 4096;  // movers listed per list build; beyond it every pass bounds displacements by the largest
 constexpr uint kMoverNear = 128;  // movers one workgroup stages; beyond it its particles bound displacements by the largest
+constexpr uint kDeferCap = 8192;   // grid walks one pass leaves to its deferred kernel; beyond it they walk in the pass
+constexpr unsigned kDeferBlocks = 128;  // workgroups of a deferred kernel (one wavefront per queued slot at a time)
 
 // Per-step statistics slots (float/uint in one array).
 enum Stat {
@@ -87,6 +92,8 @@ enum Stat {
     ST_RUN_FAIL = 52,        //   judged steps whose judged error exceeded tol (uint)
     ST_RUN_BAD = 53,         //   1 once any pass since the reset had its fallback reach capped (uint)
     ST_RUN_CHUNKS = 54,      //   list chunks stored by the list builds (float sum)
+    ST_DEFER = 55,           // deferred walks queued by the running pass (uint; zero between passes)
+    ST_DEFER_DONE = 56,      // workgroups of the running deferred kernel that finished (uint; zero between passes)
     ST_N = 64
 };
 
@@ -387,13 +394,15 @@ __device__ inline NearMovers stage_movers(uint i, bool valid, const PbfK& k, flo
 // cells.
 // Without staged movers (nm.n < 0) a particle walks its list while d_i + D <= skin, else the grid within
 // L = h + d_i + D.  The grid radius is capped at kMaxStencil cells (beyond it the step and the run are flagged).
+// With k.defer_on and movers staged, a grid walk is queued instead (deferred walks, below): the function then
+// returns false and the pass leaves slot i to its deferred kernel.
 template <int G = 1, class Fetch, class Body>
-__device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint* __restrict__ list, const uint4* __restrict__ cnt,
+__device__ inline bool for_neighbors(uint i, int mode, const PbfK& k, const uint* __restrict__ list, const uint4* __restrict__ cnt,
                                      const uint* __restrict__ bhash, const uint* __restrict__ off, float disp2, float3 p,
                                      const float4* __restrict__ xb, const NearMovers& nm, Fetch&& fetch, Body&& body) {
     if (mode == 1) {
         stencil_batched(fg_unhash(bhash[i], k.g), k.g, off, fetch, body, 1);
-        return;
+        return true;
     }
     const uint4 hd = cnt[i];
     bool ok = hd.w <= (uint)k.maxn;
@@ -433,6 +442,13 @@ __device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint
         const uint64_t wave = __ballot(1);
         if (__lane_id() == (uint)__builtin_ctzll(wave)) atomicAdd(k.fallback, (uint)__builtin_popcountll(wave));
         const bool staged = nm.n >= 0;
+        if (staged && k.defer_on) {
+            const uint t = atomicAdd(k.defer_n, 1u);
+            if (t < kDeferCap) {
+                k.defer[t] = i;
+                return false;
+            }
+        }
         const float D = sqrtf(disp2);
         const float Lwant = (k.s.h + di + (staged ? fminf(0.5f * k.skin, D) : D)) * kReachMargin;
         const float L = fminf(Lwant, kMaxStencil * k.g.cell);
@@ -453,6 +469,80 @@ __device__ inline void for_neighbors(uint i, int mode, const PbfK& k, const uint
             body(j, fetch(j));
         }
     }
+    return true;
+}
+
+// ------------------------------------------------------------------------------------------ deferred walks
+// A particle that a pass would send to the grid while its movers are staged (a mover, or an overflowed list) is
+// queued instead, and after the pass one wavefront walks it (the k_*_deferred kernels): lane l takes every 64th
+// candidate of the build-time grid cells within L = h + d_i + min(skin/2, D) of its build position, then every
+// 64th listed mover outside those cells (within h + d_i + D), and the lanes' partial sums are added across the
+// wavefront.  A few such walks of several hundred candidates then no longer hold whole wavefronts of 40-60-entry
+// list walks.  The queue count is stats slot ST_DEFER; the last workgroup of the deferred kernel clears it.
+__device__ inline float wave_sum(float v) {  // every lane ends with the same sum (each step adds a symmetric pair)
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+template <bool kCoop>
+__device__ inline float wsum(float v) {
+    if constexpr (kCoop) return wave_sum(v);
+    return v;
+}
+template <bool kCoop>
+__device__ inline float3 wsum3(float3 v) {
+    if constexpr (kCoop) return make_float3(wave_sum(v.x), wave_sum(v.y), wave_sum(v.z));
+    return v;
+}
+template <bool kCoop>
+__device__ inline bool writer() {  // the lane that stores a slot's results
+    if constexpr (kCoop) return __lane_id() == 0;
+    return true;
+}
+__device__ inline uint wave_index() { return (blockIdx.x * blockDim.x + threadIdx.x) >> 6; }
+__device__ inline uint wave_count() { return (gridDim.x * blockDim.x) >> 6; }
+__device__ inline uint defer_count(const float* __restrict__ stats) { return min(__float_as_uint(stats[ST_DEFER]), kDeferCap); }
+// Every thread, at the end of a deferred kernel: the last workgroup to finish clears the queue for the next pass
+// (every workgroup read the count before it counted itself done).
+__device__ inline void defer_done(float* __restrict__ stats) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd((uint*)&stats[ST_DEFER_DONE], 1u) == gridDim.x - 1u) {
+            atomicExch((uint*)&stats[ST_DEFER], 0u);
+            atomicExch((uint*)&stats[ST_DEFER_DONE], 0u);
+        }
+    }
+}
+
+template <class Fetch, class Body>
+__device__ inline void coop_neighbors(uint i, float di, const PbfK& k, const uint* __restrict__ bhash, const uint* __restrict__ off,
+                                      float disp2, const float4* __restrict__ xb, uint nmov, Fetch&& fetch, Body&& body) {
+    const uint lane = __lane_id();
+    const float D = sqrtf(disp2);
+    const float Lwant = (k.s.h + di + fminf(0.5f * k.skin, D)) * kReachMargin;
+    const float L = fminf(Lwant, kMaxStencil * k.g.cell);
+    if (Lwant > L && lane == 0) {
+        atomicOr(k.bad, 1u);
+        atomicOr(k.bad + (ST_RUN_BAD - ST_BAD), 1u);
+    }
+    const float3 c = f3(xb[i]);
+    const int3 lo = fg_pos(c - f3(L, L, L), k.g), hi = fg_pos(c + f3(L, L, L), k.g);
+    for (int z = lo.z; z <= hi.z; ++z)
+        for (int y = lo.y; y <= hi.y; ++y) {
+            const uint hb = (uint)((z * k.g.gy + y) * k.g.gx);
+            const uint e = off[hb + hi.x + 1];
+            for (uint j = off[hb + lo.x] + lane; j < e; j += 64u) body(j, fetch(j));
+        }
+    const float R = (k.s.h + di + D) * kReachMargin;
+    for (uint m = lane; m < min(nmov, kMoverCap); m += 64u) {
+        const float4 mv = k.movers[m];
+        if (len2_fixed(c - f3(mv)) > R * R) continue;
+        const uint j = min(__float_as_uint(mv.w), k.n - 1u);
+        const int3 cj = fg_unhash(bhash[j], k.g);
+        if (cj.x >= lo.x && cj.x <= hi.x && cj.y >= lo.y && cj.y <= hi.y && cj.z >= lo.z && cj.z <= hi.z) continue;
+        body(j, fetch(j));
+    }
 }
 
 struct XV {  // position and velocity of a neighbour
@@ -462,6 +552,38 @@ struct XV {  // position and velocity of a neighbour
 // ------------------------------------------------------------------------------------------ predict
 // rxPBDSPH::calDensity + calForceExtAndVisc + integrate (rx_sph_pbd.cpp:599-689, 1409-1446), fused.
 // Reads x, v (w unused); writes (x*, rho_start) to xs.  Also judges the previous step: max (rho - rho0)+ / rho0.
+// One slot: walk(p, fetch, body) visits the neighbours (false: deferred); e = (rho - rho0) / rho0.
+template <bool kCoop, class Walk>
+__device__ inline bool predict_slot(uint i, const float4* __restrict__ x, const float4* __restrict__ v, float4* __restrict__ xs,
+                                    float* __restrict__ dens, const PbfK& k, Walk&& walk, float& e) {
+    float3 p = f3(x[i]), vi = f3(v[i]);
+    float d = 0.0f;
+    float3 fv = f3(0, 0, 0);
+    if (!walk(p, [&](uint j) { return XV{x[j], v[j]}; },
+              [&](uint, const XV& q) {
+                  float3 r = p - f3(q.x);
+                  float r2 = dot3(r, r);
+                  if (r2 <= k.s.h2) {
+                      float qq = k.s.h2 - r2;
+                      d += qq * qq * qq;
+                      if (r2 > 0.0f) fv += (k.s.h - r2 * rinv_of(r2)) * (f3(q.v) - vi);
+                  }
+              }))
+        return false;
+    d = wsum<kCoop>(d);
+    fv = wsum3<kCoop>(fv);
+    d *= k.s.mass * k.s.wpoly6;
+    float3 a = (k.s.visc * k.s.mass * k.s.lwvisc / d) * fv + k.s.gravity;  // viscosity / rho_i (rx_sph_pbd.cpp:676)
+    float3 vn = vi + k.dt * a;
+    float3 xn = clamp_wall(p + k.dt * vn, k.s);
+    if (writer<kCoop>()) {
+        xs[i] = make_float4(xn.x, xn.y, xn.z, d);  // w: density at the step start (features)
+        dens[i] = d;
+    }
+    e = (d - k.s.rho0) * k.s.inv_rho0;
+    return true;
+}
+
 __global__ __launch_bounds__(kTpb) void k_predict(const float4* __restrict__ x, const float4* __restrict__ v, float4* __restrict__ xs,
                                                   float* __restrict__ dens, float* __restrict__ stats, int mode, PbfK k,
                                                   const uint* __restrict__ list, const uint4* __restrict__ cnt,
@@ -472,34 +594,47 @@ __global__ __launch_bounds__(kTpb) void k_predict(const float4* __restrict__ x, 
     float disp2 = stats[ST_DISP2];
     const NearMovers nm = mode == 0 ? stage_movers(i, valid, k, disp2, stats, xb, x, bhash, false) : NearMovers{nullptr, nullptr, nullptr, 0};
     float err = 0.0f, aerr = 0.0f;
-    if (valid) {
-        float3 p = f3(x[i]), vi = f3(v[i]);
-        float d = 0.0f;
-        float3 fv = f3(0, 0, 0);
-        for_neighbors<1>(
-            i, mode, k, list, cnt, bhash, off, disp2, p, xb, nm, [&](uint j) { return XV{x[j], v[j]}; },
-            [&](uint, const XV& q) {
-                float3 r = p - f3(q.x);
-                float r2 = dot3(r, r);
-                if (r2 <= k.s.h2) {
-                    float qq = k.s.h2 - r2;
-                    d += qq * qq * qq;
-                    if (r2 > 0.0f) fv += (k.s.h - r2 * rinv_of(r2)) * (f3(q.v) - vi);
-                }
-            });
-        d *= k.s.mass * k.s.wpoly6;
-        float3 a = (k.s.visc * k.s.mass * k.s.lwvisc / d) * fv + k.s.gravity;  // viscosity / rho_i (rx_sph_pbd.cpp:676)
-        float3 vn = vi + k.dt * a;
-        float3 xn = clamp_wall(p + k.dt * vn, k.s);
-        xs[i] = make_float4(xn.x, xn.y, xn.z, d);  // w: density at the step start (features)
-        dens[i] = d;
-        float e = (d - k.s.rho0) * k.s.inv_rho0;
+    float e;
+    if (valid && predict_slot<false>(i, x, v, xs, dens, k,
+                                     [&](float3 p, auto&& fetch, auto&& body) {
+                                         return for_neighbors<1>(i, mode, k, list, cnt, bhash, off, disp2, p, xb, nm, fetch, body);
+                                     },
+                                     e)) {
         err = fmaxf(e, 0.0f);
         aerr = fabsf(e);
     }
     block_max_atomic(err, &stats[ST_ERR_START]);
     block_sum_atomic(aerr, &stats[ST_MEAN_ABS_START]);
     block_sum_atomic(err, &stats[ST_MPOS_START]);
+}
+
+__global__ __launch_bounds__(kTpb) void k_predict_deferred(const float4* __restrict__ x, const float4* __restrict__ v,
+                                                           float4* __restrict__ xs, float* __restrict__ dens,
+                                                           float* __restrict__ stats, PbfK k, const uint* __restrict__ bhash,
+                                                           const uint* __restrict__ off, const float4* __restrict__ xb) {
+    const uint nq = defer_count(stats), nmov = __float_as_uint(stats[ST_MOVERS]);
+    const float disp2 = stats[ST_DISP2];
+    float err = 0.0f, aerr = 0.0f, perr = 0.0f;
+    for (uint q = wave_index(); q < nq; q += wave_count()) {
+        const uint i = k.defer[q];
+        const float di = sqrtf(len2_fixed(f3(x[i]) - f3(xb[i])));
+        float e;
+        predict_slot<true>(i, x, v, xs, dens, k,
+                           [&](float3, auto&& fetch, auto&& body) {
+                               coop_neighbors(i, di, k, bhash, off, disp2, xb, nmov, fetch, body);
+                               return true;
+                           },
+                           e);
+        if (writer<true>()) {
+            err = fmaxf(err, fmaxf(e, 0.0f));
+            perr += fmaxf(e, 0.0f);
+            aerr += fabsf(e);
+        }
+    }
+    block_max_atomic(err, &stats[ST_ERR_START]);
+    block_sum_atomic(aerr, &stats[ST_MEAN_ABS_START]);
+    block_sum_atomic(perr, &stats[ST_MPOS_START]);
+    defer_done(stats);
 }
 
 // Zeroes the statistics slots whose bit is set in `mask` (64 threads).
@@ -820,6 +955,45 @@ __global__ __launch_bounds__(kThreads) void k_list_lambda(const float4* __restri
 
 // lambda pass (rx_sph_pbd.cpp:699-822) over the list: reads positions xa, writes xl = (x, lambda).
 // Adaptive solve: once a check met the tolerance the remaining passes exit at once.
+// One slot: C = rho / rho0 - 1 in C_out.
+template <bool kCoop, class Walk>
+__device__ inline bool lambda_slot(uint i, const float4* __restrict__ xa, float4* __restrict__ xl, float* __restrict__ dens,
+                                   const PbfK& k, Walk&& walk, float& C_out) {
+    float3 p = f3(xa[i]);
+    float d = 0.0f, sd = 0.0f;
+    float3 gs = f3(0, 0, 0);
+    if (!walk(p, [&](uint j) { return xa[j]; },
+              [&](uint, const float4& qj) {
+                  float3 r = p - f3(qj);
+                  float r2 = dot3(r, r);
+                  if (r2 <= k.s.h2) {
+                      float q2 = k.s.h2 - r2;
+                      d += q2 * q2 * q2;
+                      if (r2 > 0.0f) {
+                          float ri = rinv_of(r2);
+                          float q = k.s.h - r2 * ri;
+                          float3 g = (k.gfac * q * q * ri) * r;
+                          gs += g;
+                          sd += dot3(g, g);
+                      }
+                  }
+              }))
+        return false;
+    d = wsum<kCoop>(d);
+    sd = wsum<kCoop>(sd);
+    gs = wsum3<kCoop>(gs);
+    d *= k.s.mass * k.s.wpoly6;
+    sd += dot3(gs, gs);
+    float C = d * k.s.inv_rho0 - 1.0f;
+    float lam = -C / (sd + k.eps);
+    if (writer<kCoop>()) {
+        xl[i] = make_float4(p.x, p.y, p.z, lam);
+        dens[i] = d;
+    }
+    C_out = C;
+    return true;
+}
+
 __global__ __launch_bounds__(kTpb) void k_lambda(const float4* __restrict__ xa, float4* __restrict__ xl, float* __restrict__ dens,
                                                  float* __restrict__ stats, int it, PbfK k, const uint* __restrict__ list,
                                                  const uint4* __restrict__ cnt, const uint* __restrict__ bhash,
@@ -832,42 +1006,95 @@ __global__ __launch_bounds__(kTpb) void k_lambda(const float4* __restrict__ xa, 
     // the movers the next correction pass reads (it appends new ones while it runs)
     if (blockIdx.x == 0 && threadIdx.x == 0) stats[ST_MOVERS_PUB] = stats[ST_MOVERS];
     float err = 0.0f;
-    if (valid) {
-        float3 p = f3(xa[i]);
-        float d = 0.0f, sd = 0.0f;
-        float3 gs = f3(0, 0, 0);
-        for_neighbors<2>(
-            i, 0, k, list, cnt, bhash, off, disp2, p, xb, nm, [&](uint j) { return xa[j]; },
-            [&](uint, const float4& qj) {
-                float3 r = p - f3(qj);
-                float r2 = dot3(r, r);
-                if (r2 <= k.s.h2) {
-                    float q2 = k.s.h2 - r2;
-                    d += q2 * q2 * q2;
-                    if (r2 > 0.0f) {
-                        float ri = rinv_of(r2);
-                        float q = k.s.h - r2 * ri;
-                        float3 g = (k.gfac * q * q * ri) * r;
-                        gs += g;
-                        sd += dot3(g, g);
-                    }
-                }
-            });
-        d *= k.s.mass * k.s.wpoly6;
-        sd += dot3(gs, gs);
-        float C = d * k.s.inv_rho0 - 1.0f;
-        float lam = -C / (sd + k.eps);
-        xl[i] = make_float4(p.x, p.y, p.z, lam);
-        dens[i] = d;
+    float C;
+    if (valid && lambda_slot<false>(i, xa, xl, dens, k,
+                                    [&](float3 p, auto&& fetch, auto&& body) {
+                                        return for_neighbors<2>(i, 0, k, list, cnt, bhash, off, disp2, p, xb, nm, fetch, body);
+                                    },
+                                    C))
         err = fmaxf(C, 0.0f);
-    }
     block_max_atomic(err, &stats[ST_ERR_ITER + it]);
     block_sum_atomic(err, &stats[ST_MPOS_ITER + it]);
+}
+
+__global__ __launch_bounds__(kTpb) void k_lambda_deferred(const float4* __restrict__ xa, float4* __restrict__ xl,
+                                                          float* __restrict__ dens, float* __restrict__ stats, int it, PbfK k,
+                                                          const uint* __restrict__ bhash, const uint* __restrict__ off,
+                                                          const float4* __restrict__ xb) {
+    const uint nq = defer_count(stats), nmov = __float_as_uint(stats[ST_MOVERS]);
+    const float disp2 = stats[ST_DISP2];
+    float err = 0.0f, perr = 0.0f;
+    for (uint q = wave_index(); q < nq; q += wave_count()) {
+        const uint i = k.defer[q];
+        const float di = sqrtf(len2_fixed(f3(xa[i]) - f3(xb[i])));
+        float C;
+        lambda_slot<true>(i, xa, xl, dens, k,
+                          [&](float3, auto&& fetch, auto&& body) {
+                              coop_neighbors(i, di, k, bhash, off, disp2, xb, nmov, fetch, body);
+                              return true;
+                          },
+                          C);
+        if (writer<true>()) {
+            err = fmaxf(err, fmaxf(C, 0.0f));
+            perr += fmaxf(C, 0.0f);
+        }
+    }
+    block_max_atomic(err, &stats[ST_ERR_ITER + it]);
+    block_sum_atomic(perr, &stats[ST_MPOS_ITER + it]);
+    defer_done(stats);
 }
 
 // Position correction (rx_sph_pbd.cpp:925-1026) + apply + wall (rx_sph_pbd.cpp:436-445): reads xl = (x, lambda),
 // writes xa = clamp(x + dp).  Tracks the displacement from the list build.  In learned mode the pass is
 // skipped once the previous check met the tolerance.
+// One slot: mv / mv0 = squared displacement from the build position after / before the correction.
+template <bool kCoop, class Walk>
+__device__ inline bool dp_slot(uint i, const float4* __restrict__ xl, float4* __restrict__ xa, float4* __restrict__ xbuild,
+                               float* __restrict__ lamacc, float* __restrict__ stats, int it, const PbfK& k, Walk&& walk,
+                               float& mv, float& mv0) {
+    float4 p4 = xl[i];
+    float3 p = f3(p4);
+    float li = p4.w;
+    float dt2 = k.dt * k.dt;
+    float3 acc = f3(0, 0, 0);
+    if (!walk(p, [&](uint j) { return xl[j]; },
+              [&](uint, const float4& q4) {
+                  float3 r = p - f3(q4);
+                  float r2 = dot3(r, r);
+                  if (r2 <= k.s.h2 && r2 > 0.0f) {
+                      float ri = rinv_of(r2);
+                      float sc = 0.0f;
+                      if (k.s.ap) {
+                          float q2 = k.s.h2 - r2;
+                          float ww = k.s.wpoly6 * q2 * q2 * q2 * k.s.inv_ap_wq;
+                          sc = -k.s.ap_k * (k.s.ap_n == 4.0f ? pow4(ww) : powf(ww, k.s.ap_n)) * dt2;
+                      }
+                      float q = k.s.h - r2 * ri;
+                      acc += ((li + q4.w + sc) * k.gfac * q * q * ri) * r;
+                  }
+              }))
+        return false;
+    acc = wsum3<kCoop>(acc);
+    if (it == 0 && k.first_dmax > 0.0f) {  // the warm start's correction, held to first_dmax
+        const float l2 = dot3(acc, acc);
+        if (l2 > k.first_dmax * k.first_dmax) acc = (k.first_dmax * rinv_of(l2)) * acc;
+    }
+    float3 xn = clamp_wall(p + acc, k.s);
+    const float4 b4 = xbuild[i];
+    mv = len2_fixed(xn - f3(b4));
+    mv0 = len2_fixed(p - f3(b4));
+    if (writer<kCoop>()) {
+        xa[i] = make_float4(xn.x, xn.y, xn.z, 0.0f);
+        lamacc[i] += li;  // the multiplier this step has applied so far
+        if (mv > k.half_skin2 && b4.w == 0.0f) {  // a new mover: listed once until the next build
+            xbuild[i].w = 1.0f;
+            const uint t = atomicAdd((unsigned int*)&stats[ST_MOVERS], 1u);
+            if (t < kMoverCap) k.movers[t] = make_float4(b4.x, b4.y, b4.z, __uint_as_float(i));
+        }
+    }
+    return true;
+}
+
 __global__ __launch_bounds__(kTpb) void k_dp(const float4* __restrict__ xl, float4* __restrict__ xa, float4* __restrict__ xbuild,
                                              float* __restrict__ lamacc, float* __restrict__ stats, int it, PbfK k,
                                              const uint* __restrict__ list, const uint4* __restrict__ cnt,
@@ -882,44 +1109,14 @@ __global__ __launch_bounds__(kTpb) void k_dp(const float4* __restrict__ xl, floa
     float disp2 = stats[ST_DISP2];
     const NearMovers nm = stage_movers(i, valid, k, disp2, stats, xbuild, xl, bhash, true);
     float mv = 0.0f, mv0 = 0.0f;
-    if (valid) {
-        float4 p4 = xl[i];
-        float3 p = f3(p4);
-        float li = p4.w;
-        float dt2 = k.dt * k.dt;
-        float3 acc = f3(0, 0, 0);
-        for_neighbors<1>(
-            i, 0, k, list, cnt, bhash, off, disp2, p, xbuild, nm, [&](uint j) { return xl[j]; },
-            [&](uint, const float4& q4) {
-                float3 r = p - f3(q4);
-                float r2 = dot3(r, r);
-                if (r2 <= k.s.h2 && r2 > 0.0f) {
-                    float ri = rinv_of(r2);
-                    float sc = 0.0f;
-                    if (k.s.ap) {
-                        float q2 = k.s.h2 - r2;
-                        float ww = k.s.wpoly6 * q2 * q2 * q2 * k.s.inv_ap_wq;
-                        sc = -k.s.ap_k * (k.s.ap_n == 4.0f ? pow4(ww) : powf(ww, k.s.ap_n)) * dt2;
-                    }
-                    float q = k.s.h - r2 * ri;
-                    acc += ((li + q4.w + sc) * k.gfac * q * q * ri) * r;
-                }
-            });
-        if (it == 0 && k.first_dmax > 0.0f) {  // the warm start's correction, held to first_dmax
-            const float l2 = dot3(acc, acc);
-            if (l2 > k.first_dmax * k.first_dmax) acc = (k.first_dmax * rinv_of(l2)) * acc;
-        }
-        float3 xn = clamp_wall(p + acc, k.s);
-        xa[i] = make_float4(xn.x, xn.y, xn.z, 0.0f);
-        lamacc[i] += li;  // the multiplier this step has applied so far
-        const float4 b4 = xbuild[i];
-        mv = len2_fixed(xn - f3(b4));
-        mv0 = len2_fixed(p - f3(b4));
-        if (mv > k.half_skin2 && b4.w == 0.0f) {  // a new mover: listed once until the next build
-            xbuild[i].w = 1.0f;
-            const uint t = atomicAdd((unsigned int*)&stats[ST_MOVERS], 1u);
-            if (t < kMoverCap) k.movers[t] = make_float4(b4.x, b4.y, b4.z, __uint_as_float(i));
-        }
+    float a, b;
+    if (valid && dp_slot<false>(i, xl, xa, xbuild, lamacc, stats, it, k,
+                                [&](float3 p, auto&& fetch, auto&& body) {
+                                    return for_neighbors<1>(i, 0, k, list, cnt, bhash, off, disp2, p, xbuild, nm, fetch, body);
+                                },
+                                a, b)) {
+        mv = a;
+        mv0 = b;
     }
 #pragma unroll
     for (int l = 0; l < 4; ++l) {  // one atomic per wavefront and level, when a lane crossed
@@ -927,6 +1124,34 @@ __global__ __launch_bounds__(kTpb) void k_dp(const float4* __restrict__ xl, floa
         if (c && __lane_id() == (uint)__builtin_ctzll(c)) atomicAdd((unsigned int*)&stats[ST_CROSS + l], (uint)__builtin_popcountll(c));
     }
     block_max_atomic(mv, &stats[ST_DISP2]);
+}
+
+__global__ __launch_bounds__(kTpb) void k_dp_deferred(const float4* __restrict__ xl, float4* __restrict__ xa,
+                                                      float4* __restrict__ xbuild, float* __restrict__ lamacc,
+                                                      float* __restrict__ stats, int it, PbfK k, const uint* __restrict__ bhash,
+                                                      const uint* __restrict__ off) {
+    // the movers as of the lambda pass, as the correction pass read them
+    const uint nq = defer_count(stats), nmov = __float_as_uint(stats[ST_MOVERS_PUB]);
+    const float disp2 = stats[ST_DISP2];
+    float mvmax = 0.0f;
+    for (uint q = wave_index(); q < nq; q += wave_count()) {
+        const uint i = k.defer[q];
+        const float di = sqrtf(len2_fixed(f3(xl[i]) - f3(xbuild[i])));
+        float mv, mv0;
+        dp_slot<true>(i, xl, xa, xbuild, lamacc, stats, it, k,
+                      [&](float3, auto&& fetch, auto&& body) {
+                          coop_neighbors(i, di, k, bhash, off, disp2, xbuild, nmov, fetch, body);
+                          return true;
+                      },
+                      mv, mv0);
+        if (writer<true>()) {
+            mvmax = fmaxf(mvmax, mv);
+            for (int l = 0; l < 4; ++l)
+                if (mv0 <= k.cross2[l] && mv > k.cross2[l]) atomicAdd((unsigned int*)&stats[ST_CROSS + l], 1u);
+        }
+    }
+    block_max_atomic(mvmax, &stats[ST_DISP2]);
+    defer_done(stats);
 }
 
 // v = (x* - x_n)/dt (rx_sph_pbd.cpp:465-471), with rho in w for the XSPH pass.
@@ -940,6 +1165,28 @@ __global__ void k_velocity(const float4* __restrict__ xa, const float4* __restri
 
 // XSPH (rx_pbdsph_kernel.cu:1090-1162 form): reads v = (v, rho), writes vout.  Held to 80 registers (six
 // wavefronts per SIMD, no spills): the pass waits on neighbour gathers, and the extra wavefront hides more of it.
+template <bool kCoop, class Walk>
+__device__ inline bool xsph_slot(uint i, const float4* __restrict__ xa, const float4* __restrict__ v, float4* __restrict__ vout,
+                                 const PbfK& k, Walk&& walk) {
+    float3 p = f3(xa[i]);
+    float3 vi = f3(v[i]);
+    float3 acc = f3(0, 0, 0);
+    if (!walk(p, [&](uint j) { return XV{xa[j], v[j]}; },
+              [&](uint, const XV& q4) {
+                  float3 r = p - f3(q4.x);
+                  float r2 = dot3(r, r);
+                  if (r2 <= k.s.h2) {
+                      float q = k.s.h2 - r2;
+                      acc += (q * q * q / q4.v.w) * (f3(q4.v) - vi);
+                  }
+              }))
+        return false;
+    acc = wsum3<kCoop>(acc);
+    float3 vn = vi + (k.xsph_c * k.s.mass * k.s.wpoly6) * acc;
+    if (writer<kCoop>()) vout[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
+    return true;
+}
+
 __global__ __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(6))) void k_xsph(const float4* __restrict__ xa, const float4* __restrict__ v, float4* __restrict__ vout,
                                                float* __restrict__ stats, PbfK k, const uint* __restrict__ list,
                                                const uint4* __restrict__ cnt, const uint* __restrict__ bhash,
@@ -949,27 +1196,57 @@ __global__ __launch_bounds__(kTpb) __attribute__((amdgpu_waves_per_eu(6))) void 
     float disp2 = stats[ST_DISP2];
     const NearMovers nm = stage_movers(i, valid, k, disp2, stats, xb, xa, bhash, false);
     if (!valid) return;
-    float3 p = f3(xa[i]);
-    float3 vi = f3(v[i]);
-    float3 acc = f3(0, 0, 0);
-    for_neighbors<1>(
-        i, 0, k, list, cnt, bhash, off, disp2, p, xb, nm, [&](uint j) { return XV{xa[j], v[j]}; },
-        [&](uint, const XV& q4) {
-            float3 r = p - f3(q4.x);
-            float r2 = dot3(r, r);
-            if (r2 <= k.s.h2) {
-                float q = k.s.h2 - r2;
-                acc += (q * q * q / q4.v.w) * (f3(q4.v) - vi);
-            }
+    xsph_slot<false>(i, xa, v, vout, k, [&](float3 p, auto&& fetch, auto&& body) {
+        return for_neighbors<1>(i, 0, k, list, cnt, bhash, off, disp2, p, xb, nm, fetch, body);
+    });
+}
+
+__global__ __launch_bounds__(kTpb) void k_xsph_deferred(const float4* __restrict__ xa, const float4* __restrict__ v,
+                                                        float4* __restrict__ vout, float* __restrict__ stats, PbfK k,
+                                                        const uint* __restrict__ bhash, const uint* __restrict__ off,
+                                                        const float4* __restrict__ xb) {
+    const uint nq = defer_count(stats), nmov = __float_as_uint(stats[ST_MOVERS]);
+    const float disp2 = stats[ST_DISP2];
+    for (uint q = wave_index(); q < nq; q += wave_count()) {
+        const uint i = k.defer[q];
+        const float di = sqrtf(len2_fixed(f3(xa[i]) - f3(xb[i])));
+        xsph_slot<true>(i, xa, v, vout, k, [&](float3, auto&& fetch, auto&& body) {
+            coop_neighbors(i, di, k, bhash, off, disp2, xb, nmov, fetch, body);
+            return true;
         });
-    float3 vn = vi + (k.xsph_c * k.s.mass * k.s.wpoly6) * acc;
-    vout[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
+    }
+    defer_done(stats);
 }
 
 // ------------------------------------------------------------------------------------------ heat
 // HeatTransfar::heatParticleAndParticle (HeatTransfar.cpp:429-473): over the neighbours within h at the
 // final positions, dT_i = td sum_{j != i} dt (T_j - T_i) / max(rho_j, 0.1) W_spline(r_ij, h), with the
 // cubic spline of HeatTransfar.cpp:200-236 (sigma = 10 / (7 pi h^2)).  therm = (T, heat, phase, changed).
+template <bool kCoop, class Walk>
+__device__ inline bool heat_slot(uint i, const float4* __restrict__ x, const float* __restrict__ dens,
+                                 const float4* __restrict__ therm, float* __restrict__ dtemp, const PbfK& k, float td, float hdt,
+                                 Walk&& walk) {
+    float3 p = f3(x[i]);
+    float ti = therm[i].x;
+    const float inv_h = k.inv_h, ws = 10.0f / (7.0f * 3.14159265358979f * k.s.h2);
+    float acc = 0.0f;
+    if (!walk(p, [&](uint j) { return x[j]; },
+              [&](uint j, const float4& qj) {
+                  float3 r = p - f3(qj);
+                  float r2 = dot3(r, r);
+                  if (r2 <= k.s.h2 && j != i) {
+                      float q = sqrtf(r2) * inv_h;
+                      float w = q < 1.0f ? ws * (1.0f - 1.5f * q * q + 0.75f * q * q * q) : ws * 0.25f * (2.0f - q) * (2.0f - q) * (2.0f - q);
+                      float d = fmaxf(dens[j], 0.1f);
+                      acc += hdt * (therm[j].x - ti) / d * w;
+                  }
+              }))
+        return false;
+    acc = wsum<kCoop>(acc);
+    if (writer<kCoop>()) dtemp[i] = acc * td;
+    return true;
+}
+
 __global__ __launch_bounds__(kTpb) void k_heat_diffuse(const float4* __restrict__ x, const float* __restrict__ dens,
                                                        const float4* __restrict__ therm, float* __restrict__ dtemp,
                                                        const float* __restrict__ stats, PbfK k, const uint* __restrict__ list,
@@ -981,23 +1258,27 @@ __global__ __launch_bounds__(kTpb) void k_heat_diffuse(const float4* __restrict_
     float disp2 = stats[ST_DISP2];
     const NearMovers nm = stage_movers(i, valid, k, disp2, stats, xb, x, bhash, false);
     if (!valid) return;
-    float3 p = f3(x[i]);
-    float ti = therm[i].x;
-    const float inv_h = k.inv_h, ws = 10.0f / (7.0f * 3.14159265358979f * k.s.h2);
-    float acc = 0.0f;
-    for_neighbors(
-        i, 0, k, list, cnt, bhash, off, disp2, p, xb, nm, [&](uint j) { return x[j]; },
-        [&](uint j, const float4& qj) {
-            float3 r = p - f3(qj);
-            float r2 = dot3(r, r);
-            if (r2 <= k.s.h2 && j != i) {
-                float q = sqrtf(r2) * inv_h;
-                float w = q < 1.0f ? ws * (1.0f - 1.5f * q * q + 0.75f * q * q * q) : ws * 0.25f * (2.0f - q) * (2.0f - q) * (2.0f - q);
-                float d = fmaxf(dens[j], 0.1f);
-                acc += hdt * (therm[j].x - ti) / d * w;
-            }
+    heat_slot<false>(i, x, dens, therm, dtemp, k, td, hdt, [&](float3 p, auto&& fetch, auto&& body) {
+        return for_neighbors(i, 0, k, list, cnt, bhash, off, disp2, p, xb, nm, fetch, body);
+    });
+}
+
+__global__ __launch_bounds__(kTpb) void k_heat_deferred(const float4* __restrict__ x, const float* __restrict__ dens,
+                                                        const float4* __restrict__ therm, float* __restrict__ dtemp,
+                                                        float* __restrict__ stats, PbfK k, const uint* __restrict__ bhash,
+                                                        const uint* __restrict__ off, const float4* __restrict__ xb, float td,
+                                                        float hdt) {
+    const uint nq = defer_count(stats), nmov = __float_as_uint(stats[ST_MOVERS]);
+    const float disp2 = stats[ST_DISP2];
+    for (uint q = wave_index(); q < nq; q += wave_count()) {
+        const uint i = k.defer[q];
+        const float di = sqrtf(len2_fixed(f3(x[i]) - f3(xb[i])));
+        heat_slot<true>(i, x, dens, therm, dtemp, k, td, hdt, [&](float3, auto&& fetch, auto&& body) {
+            coop_neighbors(i, di, k, bhash, off, disp2, xb, nmov, fetch, body);
+            return true;
         });
-    dtemp[i] = acc * td;
+    }
+    defer_done(stats);
 }
 
 // HeatTransfar::calcTempAndHeat (HeatTransfar.cpp:280-350): latent / sensible heat and the phase state
@@ -1074,7 +1355,7 @@ void rxPBDSPH_GPU::Finalize() {
     for (void** p : {(void**)&m_x, (void**)&m_xs, (void**)&m_xl, (void**)&m_xb, (void**)&m_v, (void**)&m_t, (void**)&m_dens,
                      (void**)&m_lamacc, (void**)&m_id, (void**)&m_idt, (void**)&m_list, (void**)&m_cnt, (void**)&m_stats,
                      (void**)&m_ftmp, (void**)&m_feat, (void**)&m_ftmp16, (void**)&m_therm, (void**)&m_therm_t, (void**)&m_mlp_mem,
-                     (void**)&m_movers})
+                     (void**)&m_movers, (void**)&m_defer})
         if (*p) {
             RX_HIP(hipFree(*p));
             *p = nullptr;
@@ -1239,7 +1520,8 @@ bool rxPBDSPH_GPU::alloc(uint n) {
               RX_HIP(hipMalloc(&m_idt, u)) && RX_HIP(hipMalloc(&m_list, list_bytes(n, m_maxn))) && RX_HIP(hipMalloc(&m_cnt, v4)) &&
               RX_HIP(hipMalloc(&m_stats, sizeof(float) * ST_N)) && RX_HIP(hipMalloc(&m_ftmp, f)) &&
               RX_HIP(hipMalloc(&m_feat, v4 * 4)) &&
-              RX_HIP(hipMalloc(&m_ftmp16, v4 * 4)) && RX_HIP(hipMalloc(&m_movers, sizeof(float4) * kMoverCap));
+              RX_HIP(hipMalloc(&m_ftmp16, v4 * 4)) && RX_HIP(hipMalloc(&m_movers, sizeof(float4) * kMoverCap)) &&
+              RX_HIP(hipMalloc(&m_defer, sizeof(uint) * kDeferCap));
     m_grid.n = n;
     m_grid.setup(m_params, m_env.h);  // the cell-h grid of the first prediction has the most cells
     ok = ok && m_grid.reserve(n);
@@ -1264,7 +1546,8 @@ int rxPBDSPH_GPU::SetParticles(const float* pos4, const float* vel4, int n) {
     if ((uint)n != m_n) {
         for (void** p : {(void**)&m_x, (void**)&m_xs, (void**)&m_xl, (void**)&m_xb, (void**)&m_v, (void**)&m_t, (void**)&m_dens,
                          (void**)&m_lamacc, (void**)&m_id, (void**)&m_idt, (void**)&m_list, (void**)&m_cnt, (void**)&m_stats,
-                         (void**)&m_ftmp, (void**)&m_feat, (void**)&m_ftmp16, (void**)&m_therm, (void**)&m_therm_t, (void**)&m_movers})
+                         (void**)&m_ftmp, (void**)&m_feat, (void**)&m_ftmp16, (void**)&m_therm, (void**)&m_therm_t, (void**)&m_movers,
+                         (void**)&m_defer})
             if (*p) {
                 RX_HIP(hipFree(*p));
                 *p = nullptr;
@@ -1328,6 +1611,9 @@ PbfK rxPBDSPH_GPU::consts(float dt) const {
     k.fallback = (uint*)(m_stats + ST_FALLBACK);
     k.bad = (uint*)(m_stats + ST_BAD);
     k.movers = m_movers;
+    k.defer = m_defer;
+    k.defer_n = (uint*)(m_stats + ST_DEFER);
+    k.defer_on = m_defer_on;
     for (int l = 0; l < 4; ++l) {
         const float t = 0.5f * m_skin_base * (float)(1 + l);
         k.cross2[l] = t * t;
@@ -1487,6 +1773,7 @@ int rxPBDSPH_GPU::enqueue_step(float dt) {
     }
     int mk = begin_mark(RX_PROF_PREDICT);
     k_predict<<<nb, kTpb, 0, s>>>(m_x, m_v, m_xs, m_dens, m_stats, mode, k, m_list, m_cnt, m_grid.hash, m_grid.off, m_xb);
+    if (mode == 0 && m_defer_on) k_predict_deferred<<<kDeferBlocks, kTpb, 0, s>>>(m_x, m_v, m_xs, m_dens, m_stats, k, m_grid.hash, m_grid.off, m_xb);
     end_mark(mk);
     if (m_step > 0) k_fold_run<<<1, 64, 0, s>>>(m_stats, k);
     if (mode == 1) {  // back to the list's grid
@@ -1564,10 +1851,12 @@ int rxPBDSPH_GPU::enqueue_step(float dt) {
         if (it > 0) {
             mk = begin_mark(RX_PROF_LAMBDA);
             k_lambda<<<nb, kTpb, 0, s>>>(m_xs, m_xl, m_dens, m_stats, it, k, m_list, m_cnt, m_grid.hash, m_grid.off, adaptive, m_xb);
+            if (m_defer_on) k_lambda_deferred<<<kDeferBlocks, kTpb, 0, s>>>(m_xs, m_xl, m_dens, m_stats, it, k, m_grid.hash, m_grid.off, m_xb);
             end_mark(mk);
         }
         mk = begin_mark(RX_PROF_DP);
         k_dp<<<nb, kTpb, 0, s>>>(m_xl, m_xs, m_xb, m_lamacc, m_stats, it, k, m_list, m_cnt, m_grid.hash, m_grid.off, adaptive);
+        if (m_defer_on) k_dp_deferred<<<kDeferBlocks, kTpb, 0, s>>>(m_xl, m_xs, m_xb, m_lamacc, m_stats, it, k, m_grid.hash, m_grid.off);
         end_mark(mk);
     }
     // (the density of the final positions is judged by the next step's prediction pass, slot ST_ERR_START)
@@ -1578,6 +1867,7 @@ int rxPBDSPH_GPU::enqueue_step(float dt) {
     if (m_env.xsph_c != 0.0f) {
         mk = begin_mark(RX_PROF_XSPH);
         k_xsph<<<nb, kTpb, 0, s>>>(m_xs, m_t, m_v, m_stats, k, m_list, m_cnt, m_grid.hash, m_grid.off, m_xb);
+        if (m_defer_on) k_xsph_deferred<<<kDeferBlocks, kTpb, 0, s>>>(m_xs, m_t, m_v, m_stats, k, m_grid.hash, m_grid.off, m_xb);
         end_mark(mk);
     } else {
         std::swap(m_v, m_t);
@@ -1587,6 +1877,9 @@ int rxPBDSPH_GPU::enqueue_step(float dt) {
         mk = begin_mark(RX_PROF_HEAT);
         k_heat_diffuse<<<nb, kTpb, 0, s>>>(m_x, m_dens, m_therm, m_ftmp, m_stats, k, m_list, m_cnt, m_grid.hash, m_grid.off, m_xb,
                                            m_heat_td, m_heat_dt > 0.0f ? m_heat_dt : dt);
+        if (m_defer_on)
+            k_heat_deferred<<<kDeferBlocks, kTpb, 0, s>>>(m_x, m_dens, m_therm, m_ftmp, m_stats, k, m_grid.hash, m_grid.off, m_xb,
+                                                          m_heat_td, m_heat_dt > 0.0f ? m_heat_dt : dt);
         k_heat_phase<<<rx_blocks(n, 256), 256, 0, s>>>(m_therm, m_ftmp, n);
         end_mark(mk);
     }
@@ -1882,6 +2175,10 @@ int rxPBDSPH_GPU::SetOption(int opt, float value) {
         return 0;
     case RX_PBF_OPT_HEAT_DT: m_heat_dt = value; return 0;
     case RX_PBF_OPT_WARM_CLAMP:
+    case RX_PBF_OPT_DEFER:
+        m_defer_on = value != 0.0f ? 1 : 0;
+        drop_graphs();
+        return 0;
     case RX_PBF_OPT_FIRST_DMAX:
         if (!(value >= 0.0f)) {
             rx_set_error(1, "rxPBDSPH_GPU::SetOption: the bound must be >= 0 (0 = none)");

@@ -134,8 +134,12 @@ enum rxPbfOption {
     RX_PBF_OPT_WARM_CLAMP = 20,    /* learned mode: keep the network's first multiplier between 0 and this
                                       many times the plain first multiplier lambda_0 (its sign, at most this
                                       multiple of its size); 0 (default): as predicted                       */
-    RX_PBF_OPT_FIRST_DMAX = 21     /* largest displacement of a step's first correction, as a fraction of h
+    RX_PBF_OPT_FIRST_DMAX = 21,    /* largest displacement of a step's first correction, as a fraction of h
                                       (the warm start's correction); 0 (default): unbounded                  */
+    RX_PBF_OPT_DEFER = 22          /* 1 (default): a particle that walks the grid in a neighbour pass (a
+                                      particle past half the skin, an overflowed list) is walked after the
+                                      pass by a whole wavefront; 0: in its own lane, holding its wavefront.
+                                      The neighbours are the same; the sums' order differs               */
 };
 
 /* Kernel classes of rxPBDSPH_GPU_GetProfile. */
@@ -177,6 +181,7 @@ enum rxProfileClass {
  *   51      steps judged         52  steps whose judged error exceeded TOL
  *   53      1 once any pass since the reset needed a walk beyond the cap (slot 22 of any step)
  *   54      list chunks stored by the list builds since the reset (sum of slot 31)
+ *   55, 56  the running pass's deferred walks (internal; zero between calls)
  * Host state at the call:
  *   60      list skin in use, as a fraction of h      61  list capacity in use                     */
 

@@ -304,6 +304,38 @@ def test_multi_step_parity_with_movers():
     assert s.stats()[22] == 0
 
 
+@pytest.mark.parametrize("skin,maxn", [(0.05, 96), (0.15, 96), (0.15, 16)])
+def test_deferred_walks_match_walks_in_the_pass(skin, maxn):
+    """A particle that walks the grid in a neighbour pass (a mover, or an overflowed list: capacity 16 here) is
+    walked after the pass by a whole wavefront (RX_PBF_OPT_DEFER 1, the default) or in its own lane (0).  Both
+    visit the same neighbours and sum them in another order: three steps agree to the parity bar."""
+    sc = scene.pbf_dam_break(20, 12, 10, seed=2)
+    pos, vel = sc.particles()
+    out = {}
+    for defer in (0, 1):
+        s = PBFSolver(sc, pos, vel)
+        s.option(_abi.RX_PBF_OPT_ADAPT_SKIN, 0)
+        s.option(_abi.RX_PBF_OPT_SKIN, skin)
+        s.option(_abi.RX_PBF_OPT_MAX_NEIGHBORS, maxn)
+        s.option(_abi.RX_PBF_OPT_DEFER, defer)
+        walks = 0.0
+        for _ in range(3):
+            s.step(1)
+            st = s.stats()
+            walks += st[20]
+            assert st[55] == 0 and st[56] == 0  # the queue of deferred walks is empty between passes
+            assert st[22] == 0
+        out[defer] = (s.positions(), s.velocities(), s.array(_abi.RX_ARRAY_DENSITY), walks)
+        s.close()
+    (p0, v0, d0, w0), (p1, v1, d1, w1) = out[0], out[1]
+    print(f"skin {skin} h, capacity {maxn}: grid walks {w0:.0f} in the passes, {w1:.0f} deferred")
+    assert w0 > 0 and w1 > 0
+    ptol = 1e-5 * np.abs(p0[:, :3]).max()
+    assert np.abs(p1[:, :3] - p0[:, :3]).max() <= ptol
+    assert np.abs(v1[:, :3] - v0[:, :3]).max() <= ptol / sc.dt  # v = (x - x_n) / dt carries the position bar
+    assert np.abs(d1 - d0).max() <= 1e-5 * np.abs(d0).max()
+
+
 def test_adaptive_sizes_do_not_depend_on_host_timing():
     """The list sizes of step s follow the records of steps up to s - 2 whatever the host's lead, so a run stepped
     in one call and the same run synchronised after every step take the same sizes and end bitwise equal."""
