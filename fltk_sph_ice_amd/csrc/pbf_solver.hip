@@ -94,6 +94,7 @@ enum Stat {
     ST_RUN_CHUNKS = 54,      //   list chunks stored by the list builds (float sum)
     ST_DEFER = 55,           // deferred walks queued by the running pass (uint; zero between passes)
     ST_DEFER_DONE = 56,      // workgroups of the running deferred kernel that finished (uint; zero between passes)
+    ST_MOVERS_SORTED = 57,   // leading entries of the mover list already in slot order (uint)
     ST_N = 64
 };
 
@@ -323,7 +324,6 @@ __device__ inline NearMovers stage_movers(uint i, bool valid, const PbfK& k, flo
     __shared__ uint s_cell[kMoverNear];
     __shared__ float s_d[kMoverNear];
     __shared__ float s_box[7][kTpb / 64];
-    __shared__ uint s_n;
     NearMovers nm{s_x, s_cell, s_d, 0};
     if (!(disp2 > k.half_skin2)) return nm;  // no particle has moved more than skin/2: the lists hold every pair
     const uint nmov = __float_as_uint(stats[published ? ST_MOVERS_PUB : ST_MOVERS]);
@@ -349,7 +349,6 @@ __device__ inline NearMovers stage_movers(uint i, bool valid, const PbfK& k, flo
     }
     if ((threadIdx.x & 63) == 0)
         for (int c = 0; c < 7; ++c) s_box[c][threadIdx.x >> 6] = b[c];
-    if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
         for (int c = 0; c < 3; ++c) {
@@ -360,25 +359,104 @@ __device__ inline NearMovers stage_movers(uint i, bool valid, const PbfK& k, flo
     }
     const float dw = fmaxf(0.5f * k.skin, sqrtf(b[6]));
     const float R0 = (k.s.h + dw + sqrtf(disp2)) * kReachMargin;  // every mover's displacement is at most sqrt(disp2)
-    for (uint m = threadIdx.x; m < nmov; m += blockDim.x) {
-        const float4 mv = k.movers[m];
-        if (mv.x >= b[0] - R0 && mv.x <= b[3] + R0 && mv.y >= b[1] - R0 && mv.y <= b[4] + R0 && mv.z >= b[2] - R0 && mv.z <= b[5] + R0) {
-            const uint slot = min(__float_as_uint(mv.w), k.n - 1u);
-            const float dj = sqrtf(len2_fixed(f3(xp[slot]) - f3(mv)));
-            const float R = (k.s.h + dw + dj) * kReachMargin;
-            if (mv.x >= b[0] - R && mv.x <= b[3] + R && mv.y >= b[1] - R && mv.y <= b[4] + R && mv.z >= b[2] - R && mv.z <= b[5] + R) {
-                const uint t = atomicAdd(&s_n, 1u);
-                if (t < kMoverNear) {
-                    s_x[t] = mv;
-                    s_cell[t] = bhash[slot];
-                    s_d[t] = dj;
-                }
+    // staged in list order (the list is sorted by slot after every correction pass): a prefix over the
+    // workgroup per round of blockDim.x movers, so the order in which a particle sums them is the same every run
+    __shared__ uint s_cnt[kTpb / 64];
+    uint staged = 0;
+    for (uint m0 = 0; m0 < nmov; m0 += blockDim.x) {
+        const uint m = m0 + threadIdx.x;
+        bool take = false;
+        float4 mv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float dj = 0.0f;
+        uint slot = 0;
+        if (m < nmov) {
+            mv = k.movers[m];
+            if (mv.x >= b[0] - R0 && mv.x <= b[3] + R0 && mv.y >= b[1] - R0 && mv.y <= b[4] + R0 && mv.z >= b[2] - R0 && mv.z <= b[5] + R0) {
+                slot = min(__float_as_uint(mv.w), k.n - 1u);
+                dj = sqrtf(len2_fixed(f3(xp[slot]) - f3(mv)));
+                const float R = (k.s.h + dw + dj) * kReachMargin;
+                take = mv.x >= b[0] - R && mv.x <= b[3] + R && mv.y >= b[1] - R && mv.y <= b[4] + R && mv.z >= b[2] - R && mv.z <= b[5] + R;
             }
         }
+        const uint64_t bal = __ballot(take);
+        const uint wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+        if (lane == 0) s_cnt[wv] = (uint)__builtin_popcountll(bal);
+        __syncthreads();
+        uint pos = staged + (uint)__builtin_popcountll(bal & ((1ull << lane) - 1ull)), round = 0;
+        for (uint w = 0; w < (blockDim.x >> 6); ++w) {
+            pos += w < wv ? s_cnt[w] : 0u;
+            round += s_cnt[w];
+        }
+        if (take && pos < kMoverNear) {
+            s_x[pos] = mv;
+            s_cell[pos] = bhash[slot];
+            s_d[pos] = dj;
+        }
+        staged += round;
+        __syncthreads();  // s_cnt is rewritten by the next round
+    }
+    nm.n = staged > kMoverNear ? -1 : (int)staged;
+    return nm;
+}
+
+// Keeps the listed movers in slot order (one workgroup), so that every pass reads them in one order whatever
+// order the correction pass's atomics listed them in: the sums over movers, and with them the positions, are the
+// same from run to run.  Runs after every correction pass.  The entries up to stats[ST_MOVERS_SORTED] are in
+// order from the previous run; the new tail's keys (slot, index) are sorted in LDS (bitonic), then both runs
+// merge by rank: an entry's place is its index in its own run plus the number of keys below it in the other.
+constexpr uint kSortThreads = 1024;
+__global__ __launch_bounds__(kSortThreads) void k_sort_movers(float4* __restrict__ movers, float* __restrict__ stats) {
+    __shared__ float4 s_out[kMoverCap];
+    __shared__ uint2 s_tail[kMoverCap];  // (slot, index) of the new entries
+    __shared__ uint s_head[kMoverCap];   // slots of the sorted entries
+    const uint n = __float_as_uint(stats[ST_MOVERS]);
+    const uint h = min(__float_as_uint(stats[ST_MOVERS_SORTED]), n);
+    if (n > kMoverCap || h == n) return;  // (beyond the cap the passes do not read the list)
+    const uint tn = n - h;
+    uint np = 1u;
+    while (np < tn) np <<= 1;
+    for (uint t = threadIdx.x; t < np; t += blockDim.x)
+        s_tail[t] = t < tn ? make_uint2(__float_as_uint(movers[h + t].w), h + t) : make_uint2(0xFFFFFFFFu, 0u);
+    for (uint t = threadIdx.x; t < h; t += blockDim.x) s_head[t] = __float_as_uint(movers[t].w);
+    __syncthreads();
+    for (uint kk = 2u; kk <= np; kk <<= 1)
+        for (uint jj = kk >> 1; jj > 0u; jj >>= 1) {
+            for (uint t = threadIdx.x; t < np; t += blockDim.x) {
+                const uint u = t ^ jj;
+                if (u > t) {
+                    const uint2 a = s_tail[t], c = s_tail[u];
+                    if ((a.x > c.x) == ((t & kk) == 0u)) {
+                        s_tail[t] = c;
+                        s_tail[u] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    // slots are distinct (a particle is listed once), so the ranks place every entry exactly once
+    for (uint t = threadIdx.x; t < h; t += blockDim.x) {
+        const uint key = s_head[t];
+        uint lo = 0u, hi = tn;  // tail keys below key
+        while (lo < hi) {
+            const uint mid = (lo + hi) >> 1;
+            if (s_tail[mid].x < key) lo = mid + 1u;
+            else hi = mid;
+        }
+        s_out[t + lo] = movers[t];
+    }
+    for (uint t = threadIdx.x; t < tn; t += blockDim.x) {
+        const uint key = s_tail[t].x;
+        uint lo = 0u, hi = h;  // head keys below key
+        while (lo < hi) {
+            const uint mid = (lo + hi) >> 1;
+            if (s_head[mid] < key) lo = mid + 1u;
+            else hi = mid;
+        }
+        s_out[t + lo] = movers[s_tail[t].y];
     }
     __syncthreads();
-    nm.n = s_n > kMoverNear ? -1 : (int)s_n;
-    return nm;
+    for (uint t = threadIdx.x; t < n; t += blockDim.x) movers[t] = s_out[t];
+    if (threadIdx.x == 0) stats[ST_MOVERS_SORTED] = __uint_as_float(n);
 }
 
 // Neighbour candidates of slot i for a pass that needs every j with |x_i - x_j| <= h at the current positions;
@@ -394,8 +472,8 @@ __device__ inline NearMovers stage_movers(uint i, bool valid, const PbfK& k, flo
 // cells.
 // Without staged movers (nm.n < 0) a particle walks its list while d_i + D <= skin, else the grid within
 // L = h + d_i + D.  The grid radius is capped at kMaxStencil cells (beyond it the step and the run are flagged).
-// With k.defer_on and movers staged, a grid walk is queued instead (deferred walks, below): the function then
-// returns false and the pass leaves slot i to its deferred kernel.
+// With k.defer_on and movers staged, the grid walk of a mover is queued instead (deferred walks, below): the
+// function then returns false and the pass leaves slot i to its deferred kernel.
 template <int G = 1, class Fetch, class Body>
 __device__ inline bool for_neighbors(uint i, int mode, const PbfK& k, const uint* __restrict__ list, const uint4* __restrict__ cnt,
                                      const uint* __restrict__ bhash, const uint* __restrict__ off, float disp2, float3 p,
@@ -406,13 +484,14 @@ __device__ inline bool for_neighbors(uint i, int mode, const PbfK& k, const uint
     }
     const uint4 hd = cnt[i];
     bool ok = hd.w <= (uint)k.maxn;
-    bool movers = false;
+    bool movers = false, mover = false;
     float di = 0.0f;
     if (disp2 > k.half_skin2) {
         const float di2 = len2_fixed(p - f3(xb[i]));
         di = sqrtf(di2);
         if (nm.n >= 0) {
-            ok = ok && di2 <= k.half_skin2;
+            mover = di2 > k.half_skin2;
+            ok = ok && !mover;
             movers = nm.n > 0;
         } else {
             ok = ok && di + sqrtf(disp2) <= k.skin;
@@ -442,7 +521,7 @@ __device__ inline bool for_neighbors(uint i, int mode, const PbfK& k, const uint
         const uint64_t wave = __ballot(1);
         if (__lane_id() == (uint)__builtin_ctzll(wave)) atomicAdd(k.fallback, (uint)__builtin_popcountll(wave));
         const bool staged = nm.n >= 0;
-        if (staged && k.defer_on) {
+        if (mover && k.defer_on) {  // (movers are at most kMoverCap <= kDeferCap: the queue never fills)
             const uint t = atomicAdd(k.defer_n, 1u);
             if (t < kDeferCap) {
                 k.defer[t] = i;
@@ -473,12 +552,14 @@ __device__ inline bool for_neighbors(uint i, int mode, const PbfK& k, const uint
 }
 
 // ------------------------------------------------------------------------------------------ deferred walks
-// A particle that a pass would send to the grid while its movers are staged (a mover, or an overflowed list) is
-// queued instead, and after the pass one wavefront walks it (the k_*_deferred kernels): lane l takes every 64th
+// A mover that a pass would send to the grid while its movers are staged is queued instead, and after the pass
+// one wavefront walks it (the k_*_deferred kernels): lane l takes every 64th
 // candidate of the build-time grid cells within L = h + d_i + min(skin/2, D) of its build position, then every
 // 64th listed mover outside those cells (within h + d_i + D), and the lanes' partial sums are added across the
 // wavefront.  A few such walks of several hundred candidates then no longer hold whole wavefronts of 40-60-entry
 // list walks.  The queue count is stats slot ST_DEFER; the last workgroup of the deferred kernel clears it.
+// Which particles are queued depends on the positions alone (movers, at most kMoverCap), and each is summed by
+// a fixed split across the lanes, so the results do not depend on the order the queue was filled in.
 __device__ inline float wave_sum(float v) {  // every lane ends with the same sum (each step adds a symmetric pair)
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
@@ -528,20 +609,48 @@ __device__ inline void coop_neighbors(uint i, float di, const PbfK& k, const uin
     }
     const float3 c = f3(xb[i]);
     const int3 lo = fg_pos(c - f3(L, L, L), k.g), hi = fg_pos(c + f3(L, L, L), k.g);
-    for (int z = lo.z; z <= hi.z; ++z)
-        for (int y = lo.y; y <= hi.y; ++y) {
-            const uint hb = (uint)((z * k.g.gy + y) * k.g.gx);
-            const uint e = off[hb + hi.x + 1];
-            for (uint j = off[hb + lo.x] + lane; j < e; j += 64u) body(j, fetch(j));
+    // the box's rows of contiguous slots, four at a time: their bounds, then one candidate per lane of each, are
+    // loaded together (rows are mostly shorter than a wavefront, so a row at a time waits on every load)
+    const int ny = hi.y - lo.y + 1, nrow = ny * (hi.z - lo.z + 1);
+    using Dt = decltype(fetch(0u));
+    for (int r0 = 0; r0 < nrow; r0 += 4) {
+        uint b[4], e[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            b[u] = e[u] = 0u;
+            if (r0 + u < nrow) {
+                const int r = r0 + u, z = lo.z + r / ny, y = lo.y + r % ny;
+                const uint hb = (uint)((z * k.g.gy + y) * k.g.gx);
+                b[u] = off[hb + lo.x];
+                e[u] = off[hb + hi.x + 1];
+            }
         }
+        Dt dd[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (b[u] + lane < e[u]) dd[u] = fetch(b[u] + lane);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (b[u] + lane < e[u]) body(b[u] + lane, dd[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)  // rows longer than a wavefront
+            for (uint j = b[u] + lane + 64u; j < e[u]; j += 64u) body(j, fetch(j));
+    }
+    // the listed movers outside the box, four loads per lane at a time
     const float R = (k.s.h + di + D) * kReachMargin;
-    for (uint m = lane; m < min(nmov, kMoverCap); m += 64u) {
-        const float4 mv = k.movers[m];
-        if (len2_fixed(c - f3(mv)) > R * R) continue;
-        const uint j = min(__float_as_uint(mv.w), k.n - 1u);
-        const int3 cj = fg_unhash(bhash[j], k.g);
-        if (cj.x >= lo.x && cj.x <= hi.x && cj.y >= lo.y && cj.y <= hi.y && cj.z >= lo.z && cj.z <= hi.z) continue;
-        body(j, fetch(j));
+    const uint nm = min(nmov, kMoverCap);
+    for (uint m0 = lane; m0 < nm; m0 += 256u) {
+        float4 mv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) mv[u] = m0 + 64u * u < nm ? k.movers[m0 + 64u * u] : make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (!(len2_fixed(c - f3(mv[u])) <= R * R)) continue;
+            const uint j = min(__float_as_uint(mv[u].w), k.n - 1u);
+            const int3 cj = fg_unhash(bhash[j], k.g);
+            if (cj.x >= lo.x && cj.x <= hi.x && cj.y >= lo.y && cj.y <= hi.y && cj.z >= lo.z && cj.z <= hi.z) continue;
+            body(j, fetch(j));
+        }
     }
 }
 
@@ -1783,7 +1892,7 @@ int rxPBDSPH_GPU::enqueue_step(float dt) {
     if (apply_targets(s)) k = consts(dt);  // list sizes for this step's build (the prediction used the old list)
 
     // ---- rebuild on x*; the step-start density follows the particles
-    k_zero_slots<<<1, 64, 0, s>>>(m_stats, (1ull << ST_DISP2) | (1ull << ST_MOVERS) | (1ull << ST_MOVERS_PUB));
+    k_zero_slots<<<1, 64, 0, s>>>(m_stats, (1ull << ST_DISP2) | (1ull << ST_MOVERS) | (1ull << ST_MOVERS_PUB) | (1ull << ST_MOVERS_SORTED));
     mk = begin_mark(RX_PROF_GRID);
     {
         // the sort's last pass gathers the whole state into the new order (xl and xb are dead until the list
@@ -1857,6 +1966,7 @@ int rxPBDSPH_GPU::enqueue_step(float dt) {
         mk = begin_mark(RX_PROF_DP);
         k_dp<<<nb, kTpb, 0, s>>>(m_xl, m_xs, m_xb, m_lamacc, m_stats, it, k, m_list, m_cnt, m_grid.hash, m_grid.off, adaptive);
         if (m_defer_on) k_dp_deferred<<<kDeferBlocks, kTpb, 0, s>>>(m_xl, m_xs, m_xb, m_lamacc, m_stats, it, k, m_grid.hash, m_grid.off);
+        k_sort_movers<<<1, kSortThreads, 0, s>>>(m_movers, m_stats);
         end_mark(mk);
     }
     // (the density of the final positions is judged by the next step's prediction pass, slot ST_ERR_START)

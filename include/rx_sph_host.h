@@ -136,10 +136,10 @@ enum rxPbfOption {
                                       multiple of its size); 0 (default): as predicted                       */
     RX_PBF_OPT_FIRST_DMAX = 21,    /* largest displacement of a step's first correction, as a fraction of h
                                       (the warm start's correction); 0 (default): unbounded                  */
-    RX_PBF_OPT_DEFER = 22          /* 1 (default): a particle that walks the grid in a neighbour pass (a
-                                      particle past half the skin, an overflowed list) is walked after the
-                                      pass by a whole wavefront; 0: in its own lane, holding its wavefront.
-                                      The neighbours are the same; the sums' order differs               */
+    RX_PBF_OPT_DEFER = 22          /* 1 (default): a particle past half the skin that walks the grid in a
+                                      neighbour pass is walked after the pass by a whole wavefront; 0: in its
+                                      own lane, holding its wavefront.  The neighbours are the same; the
+                                      sums' order differs                                                */
 };
 
 /* Kernel classes of rxPBDSPH_GPU_GetProfile. */
@@ -181,7 +181,8 @@ enum rxProfileClass {
  *   51      steps judged         52  steps whose judged error exceeded TOL
  *   53      1 once any pass since the reset needed a walk beyond the cap (slot 22 of any step)
  *   54      list chunks stored by the list builds since the reset (sum of slot 31)
- *   55, 56  the running pass's deferred walks (internal; zero between calls)
+ *   55-57   internal: the running pass's deferred walks (zero between calls), the mover list's sorted
+ *           length
  * Host state at the call:
  *   60      list skin in use, as a fraction of h      61  list capacity in use                     */
 

@@ -304,11 +304,35 @@ def test_multi_step_parity_with_movers():
     assert s.stats()[22] == 0
 
 
+def test_runs_with_movers_and_deferred_walks_are_bitwise_reproducible():
+    """The correction pass lists movers through atomics, in whatever order its wavefronts run.  The list is
+    sorted by slot after every correction pass, workgroups stage it in that order, and deferred walks (the
+    movers') split it across lanes by that order: two runs of the same steps, with movers, deferred walks and
+    overflowed lists (capacity 24), give the same bits."""
+    sc = scene.pbf_dam_break(48, 40, 32, seed=11)
+    outs = []
+    for _ in range(2):
+        s = PBFSolver(sc)
+        s.option(_abi.RX_PBF_OPT_MAX_NEIGHBORS, 24)
+        movers = walks = 0.0
+        for _ in range(10):
+            s.step(1)
+            st = s.stats()
+            movers, walks = max(movers, st[25]), walks + st[20]
+        outs.append((s.positions(), s.velocities(), s.array(_abi.RX_ARRAY_DENSITY), movers, walks))
+        s.close()
+    print(f"most movers in a step {outs[0][3]:.0f}, grid walks {outs[0][4]:.0f}")
+    assert outs[0][3] > 1 and outs[0][4] > 0
+    for a, b in zip(outs[0][:3], outs[1][:3]):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 @pytest.mark.parametrize("skin,maxn", [(0.05, 96), (0.15, 96), (0.15, 16)])
 def test_deferred_walks_match_walks_in_the_pass(skin, maxn):
-    """A particle that walks the grid in a neighbour pass (a mover, or an overflowed list: capacity 16 here) is
-    walked after the pass by a whole wavefront (RX_PBF_OPT_DEFER 1, the default) or in its own lane (0).  Both
-    visit the same neighbours and sum them in another order: three steps agree to the parity bar."""
+    """A mover that walks the grid in a neighbour pass is walked after the pass by a whole wavefront
+    (RX_PBF_OPT_DEFER 1, the default) or in its own lane (0); overflowed lists (capacity 16 here) walk in their
+    lane either way.  Both visit the same neighbours and sum them in another order: three steps agree to the
+    parity bar."""
     sc = scene.pbf_dam_break(20, 12, 10, seed=2)
     pos, vel = sc.particles()
     out = {}
