@@ -309,7 +309,7 @@ RX_PBF_MODE_PLAIN, RX_PBF_MODE_SCALED, RX_PBF_MODE_LEARNED = 0, 1, 2
  RX_PBF_OPT_JUDGE, RX_PBF_OPT_RESET_RUN, RX_PBF_OPT_PROFILE_MASK, RX_PBF_OPT_LIST_TILE, RX_PBF_OPT_HEAT_TD,
  RX_PBF_OPT_HEAT_DT, RX_PBF_OPT_MAX_NEIGHBORS, RX_PBF_OPT_GRAPH, RX_PBF_OPT_ADAPT_SKIN, RX_PBF_OPT_GRAD_MASS,
  RX_PBF_OPT_WARM_CLAMP, RX_PBF_OPT_FIRST_DMAX, RX_PBF_OPT_DEFER) = range(23)
-PROF_CLASSES = ["predict", "grid", "list_lambda", "lambda", "dp", "velocity", "xsph", "features", "mlp", "heat"]
+PROF_CLASSES = ["predict", "grid", "list_lambda", "lambda", "dp", "velocity", "xsph", "features", "mlp", "heat", "deferred"]
 
 _lib = None
 

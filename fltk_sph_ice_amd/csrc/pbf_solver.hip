@@ -1517,12 +1517,13 @@ int rxPBDSPH_GPU::Initialize(const rxSphEnv& env) {
 // least 2.2 times the step's largest displacement D: then d_i + D <= 2D stays within the skin and no particle
 // walks the grid.  Level want_m(a) is the lowest at which at most a particles crossed half its skin: those few
 // walk the grid as movers and the rest keep their lists, at a cost that stays small while they are sparse
-// (allowance A = min(kMoverCap / 2, n / 512)).  A grid walk costs 10 to 20 list walks while each level up costs
+// (allowance A = min(kMoverCap / 2, n / 256)).  A grid walk costs 10 to 20 list walks while each level up costs
 // 1.4 to 1.9 times the list work, so the level rises only when more than 3% of the step's neighbour passes
-// walked the grid: at once to min(want_d, want_m(A)), and at least one level (up to want_d) since the walks show
-// the movers were not few enough.  It falls to min(want_d, want_m(A / 4)) (the lower allowance keeps it from
-// falling back into the walks) after 2 records in a row want a lower level.  The list capacity rises by 64 when
-// more than n/2000 lists overflowed.
+// walked the grid for want of a wide enough skin (the walks of overflowed lists, one per pass, do not count: a
+// wider skin only lengthens the lists): at once to min(want_d, want_m(A)), and at least one level (up to
+// want_d) since the walks show the movers were not few enough.  It falls to min(want_d, want_m(A / 4)) (the
+// lower allowance keeps it from falling back into the walks) after 2 records in a row want a lower level.  The
+// list capacity rises by 64 when more than n/2000 lists overflowed.
 void rxPBDSPH_GPU::poll_records() {
     // Before step s the records of steps up to s - 2 are applied, waiting for them if need be, and never the record
     // of step s - 1: the sizes of every step are then a function of the run alone, not of how far the host ran
@@ -1554,7 +1555,8 @@ void rxPBDSPH_GPU::poll_records() {
         const float passes = (float)(2 * std::min(m_env.max_iter, 14) + 2);
         const int want_up = std::max(std::min(want_d, want_m(allowance)), std::min(want_d, m_level_target + 1));
         const int want = std::min(want_d, want_m(allowance / 4));
-        if ((float)fb > 0.03f * (float)m_n * passes) {
+        const float fb_skin = fmaxf((float)fb - (float)over * passes, 0.0f);  // walks a wider skin would spare
+        if (fb_skin > 0.03f * (float)m_n * passes) {
             if (want_up > m_level_target) m_level_target = want_up;
             m_calm = 0;
         } else if (want < m_level_target) {
@@ -1882,8 +1884,12 @@ int rxPBDSPH_GPU::enqueue_step(float dt) {
     }
     int mk = begin_mark(RX_PROF_PREDICT);
     k_predict<<<nb, kTpb, 0, s>>>(m_x, m_v, m_xs, m_dens, m_stats, mode, k, m_list, m_cnt, m_grid.hash, m_grid.off, m_xb);
-    if (mode == 0 && m_defer_on) k_predict_deferred<<<kDeferBlocks, kTpb, 0, s>>>(m_x, m_v, m_xs, m_dens, m_stats, k, m_grid.hash, m_grid.off, m_xb);
     end_mark(mk);
+    if (mode == 0 && m_defer_on) {
+        mk = begin_mark(RX_PROF_DEFERRED);
+        k_predict_deferred<<<kDeferBlocks, kTpb, 0, s>>>(m_x, m_v, m_xs, m_dens, m_stats, k, m_grid.hash, m_grid.off, m_xb);
+        end_mark(mk);
+    }
     if (m_step > 0) k_fold_run<<<1, 64, 0, s>>>(m_stats, k);
     if (mode == 1) {  // back to the list's grid
         m_grid.setup(m_params, m_env.h + m_skin);
@@ -1960,11 +1966,17 @@ int rxPBDSPH_GPU::enqueue_step(float dt) {
         if (it > 0) {
             mk = begin_mark(RX_PROF_LAMBDA);
             k_lambda<<<nb, kTpb, 0, s>>>(m_xs, m_xl, m_dens, m_stats, it, k, m_list, m_cnt, m_grid.hash, m_grid.off, adaptive, m_xb);
-            if (m_defer_on) k_lambda_deferred<<<kDeferBlocks, kTpb, 0, s>>>(m_xs, m_xl, m_dens, m_stats, it, k, m_grid.hash, m_grid.off, m_xb);
             end_mark(mk);
+            if (m_defer_on) {
+                mk = begin_mark(RX_PROF_DEFERRED);
+                k_lambda_deferred<<<kDeferBlocks, kTpb, 0, s>>>(m_xs, m_xl, m_dens, m_stats, it, k, m_grid.hash, m_grid.off, m_xb);
+                end_mark(mk);
+            }
         }
         mk = begin_mark(RX_PROF_DP);
         k_dp<<<nb, kTpb, 0, s>>>(m_xl, m_xs, m_xb, m_lamacc, m_stats, it, k, m_list, m_cnt, m_grid.hash, m_grid.off, adaptive);
+        end_mark(mk);
+        mk = begin_mark(RX_PROF_DEFERRED);
         if (m_defer_on) k_dp_deferred<<<kDeferBlocks, kTpb, 0, s>>>(m_xl, m_xs, m_xb, m_lamacc, m_stats, it, k, m_grid.hash, m_grid.off);
         k_sort_movers<<<1, kSortThreads, 0, s>>>(m_movers, m_stats);
         end_mark(mk);
@@ -1977,8 +1989,12 @@ int rxPBDSPH_GPU::enqueue_step(float dt) {
     if (m_env.xsph_c != 0.0f) {
         mk = begin_mark(RX_PROF_XSPH);
         k_xsph<<<nb, kTpb, 0, s>>>(m_xs, m_t, m_v, m_stats, k, m_list, m_cnt, m_grid.hash, m_grid.off, m_xb);
-        if (m_defer_on) k_xsph_deferred<<<kDeferBlocks, kTpb, 0, s>>>(m_xs, m_t, m_v, m_stats, k, m_grid.hash, m_grid.off, m_xb);
         end_mark(mk);
+        if (m_defer_on) {
+            mk = begin_mark(RX_PROF_DEFERRED);
+            k_xsph_deferred<<<kDeferBlocks, kTpb, 0, s>>>(m_xs, m_t, m_v, m_stats, k, m_grid.hash, m_grid.off, m_xb);
+            end_mark(mk);
+        }
     } else {
         std::swap(m_v, m_t);
     }

@@ -153,7 +153,8 @@ enum rxProfileClass {
     RX_PROF_XSPH = 6,
     RX_PROF_FEATURES = 7,     /* learned solve: feature pass               */
     RX_PROF_MLP = 8,          /* learned solve: network inference          */
-    RX_PROF_HEAT = 9          /* thermal state: diffusion + phase          */
+    RX_PROF_HEAT = 9,         /* thermal state: diffusion + phase          */
+    RX_PROF_DEFERRED = 10     /* walks deferred by the neighbour passes and the mover list's ordering */
 };
 
 /* Per-step statistics of rxPBDSPH_GPU_GetStats (float slots):
