@@ -384,6 +384,27 @@ def test_adaptive_sizes_do_not_depend_on_host_timing():
     assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
 
 
+def test_overflowed_lists_grow_the_capacity_not_the_skin():
+    """Lists that overflow walk the grid in every pass, but a wider skin would only lengthen them: the adaptive
+    sizing grows the capacity (by 64 per record) and leaves the skin at its base while the particles stay put."""
+    sc = scene.pbf_dam_break(20, 20, 20, seed=4)
+    s = PBFSolver(sc)
+    s.option(_abi.RX_PBF_OPT_MAX_NEIGHBORS, 16)
+    skins, caps, over = [], [], 0.0
+    for _ in range(8):
+        s.step(1)
+        s.sync()
+        st = s.stats()
+        skins.append(float(st[60]))
+        caps.append(float(st[61]))
+        over = max(over, float(st[18]))
+    s.close()
+    print(f"overflowed lists up to {over:.0f}; capacities {sorted(set(caps))}; skins {sorted(set(skins))}")
+    assert over > sc.n / 2
+    assert max(caps) > 16
+    assert max(skins) == pytest.approx(0.15)
+
+
 @pytest.mark.parametrize("mode,scale,dmax", [("scaled", 3.0, 0.0), ("scaled", 3.0, 0.02), ("plain", 1.0, 0.01)])
 def test_first_correction_factor_and_cap_match_oracle(mode, scale, dmax):
     # RX_PBF_MODE_SCALED (a constant factor on the first multipliers) and RX_PBF_OPT_FIRST_DMAX (the first
