@@ -306,6 +306,7 @@ struct NearMovers {
     const uint* cell;  // build cell hash
     const float* d;    // displacement from the build position at the pass's positions
     int n;
+    float dmax;        // n < 0: the largest displacement a neighbour of the workgroup's particles can have
 };
 
 // Rounding margin of the reach tests below (their distances are sums of rounded lengths).
@@ -324,10 +325,10 @@ __device__ inline NearMovers stage_movers(uint i, bool valid, const PbfK& k, flo
     __shared__ uint s_cell[kMoverNear];
     __shared__ float s_d[kMoverNear];
     __shared__ float s_box[7][kTpb / 64];
-    NearMovers nm{s_x, s_cell, s_d, 0};
+    NearMovers nm{s_x, s_cell, s_d, 0, sqrtf(disp2)};
     if (!(disp2 > k.half_skin2)) return nm;  // no particle has moved more than skin/2: the lists hold every pair
     const uint nmov = __float_as_uint(stats[published ? ST_MOVERS_PUB : ST_MOVERS]);
-    if (nmov > kMoverCap) {
+    if (nmov > kMoverCap) {  // (the list is incomplete: every displacement is bounded by the largest)
         nm.n = -1;
         return nm;
     }
@@ -363,6 +364,7 @@ __device__ inline NearMovers stage_movers(uint i, bool valid, const PbfK& k, flo
     // workgroup per round of blockDim.x movers, so the order in which a particle sums them is the same every run
     __shared__ uint s_cnt[kTpb / 64];
     uint staged = 0;
+    float dtake = 0.0f;  // the largest displacement of a mover within reach (this thread's share)
     for (uint m0 = 0; m0 < nmov; m0 += blockDim.x) {
         const uint m = m0 + threadIdx.x;
         bool take = false;
@@ -392,10 +394,23 @@ __device__ inline NearMovers stage_movers(uint i, bool valid, const PbfK& k, flo
             s_cell[pos] = bhash[slot];
             s_d[pos] = dj;
         }
+        dtake = take ? fmaxf(dtake, dj) : dtake;
         staged += round;
         __syncthreads();  // s_cnt is rewritten by the next round
     }
     nm.n = staged > kMoverNear ? -1 : (int)staged;
+    if (nm.n < 0) {
+        // too many to stage: a neighbour j of one of the workgroup's particles is either within skin/2 of its
+        // build position or a mover within reach, so the largest of those bounds every such d_j
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) dtake = fmaxf(dtake, __shfl_xor(dtake, d, 64));
+        if ((threadIdx.x & 63) == 0) s_box[0][threadIdx.x >> 6] = dtake;
+        __syncthreads();
+        float dm = 0.5f * k.skin;
+        for (uint w = 0; w < (blockDim.x >> 6); ++w) dm = fmaxf(dm, s_box[0][w]);
+        nm.dmax = dm;
+        __syncthreads();  // s_box is reused by the next staging
+    }
     return nm;
 }
 
@@ -470,8 +485,10 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_movers(float4* __restrict
 // twice); a mover or an overflowed list walks the build-time grid over the cells within L = h + d_i + min(skin/2,
 // D) of its build position, which hold every j that is not a mover, and then the staged movers outside those
 // cells.
-// Without staged movers (nm.n < 0) a particle walks its list while d_i + D <= skin, else the grid within
-// L = h + d_i + D.  The grid radius is capped at kMaxStencil cells (beyond it the step and the run are flagged).
+// Without staged movers (nm.n < 0) a particle walks its list while d_i + D_w <= skin, else the grid within
+// L = h + d_i + D_w, where D_w = nm.dmax bounds the displacement of any neighbour: the largest of skin/2 and the
+// displacements of the movers within the workgroup's reach (too many to stage), or D when the mover list is
+// incomplete.  The grid radius is capped at kMaxStencil cells (beyond it the step and the run are flagged).
 // With k.defer_on and movers staged, the grid walk of a mover is queued instead (deferred walks, below): the
 // function then returns false and the pass leaves slot i to its deferred kernel.
 template <int G = 1, class Fetch, class Body>
@@ -494,7 +511,7 @@ __device__ inline bool for_neighbors(uint i, int mode, const PbfK& k, const uint
             ok = ok && !mover;
             movers = nm.n > 0;
         } else {
-            ok = ok && di + sqrtf(disp2) <= k.skin;
+            ok = ok && di + nm.dmax <= k.skin;
         }
     }
     if (ok) {
@@ -529,7 +546,7 @@ __device__ inline bool for_neighbors(uint i, int mode, const PbfK& k, const uint
             }
         }
         const float D = sqrtf(disp2);
-        const float Lwant = (k.s.h + di + (staged ? fminf(0.5f * k.skin, D) : D)) * kReachMargin;
+        const float Lwant = (k.s.h + di + (staged ? fminf(0.5f * k.skin, D) : nm.dmax)) * kReachMargin;
         const float L = fminf(Lwant, kMaxStencil * k.g.cell);
         if (Lwant > L) {  // the capped walk may miss pairs: flag the step and the run (never silently)
             atomicOr(k.bad, 1u);
@@ -701,7 +718,7 @@ __global__ __launch_bounds__(kTpb) void k_predict(const float4* __restrict__ x, 
     bool valid;
     uint i = slot_of(k, valid);
     float disp2 = stats[ST_DISP2];
-    const NearMovers nm = mode == 0 ? stage_movers(i, valid, k, disp2, stats, xb, x, bhash, false) : NearMovers{nullptr, nullptr, nullptr, 0};
+    const NearMovers nm = mode == 0 ? stage_movers(i, valid, k, disp2, stats, xb, x, bhash, false) : NearMovers{nullptr, nullptr, nullptr, 0, 0.0f};
     float err = 0.0f, aerr = 0.0f;
     float e;
     if (valid && predict_slot<false>(i, x, v, xs, dens, k,
