@@ -168,10 +168,11 @@ def test_thermal_step_matches_oracle():
     assert np.isfinite(T3).all() and (ph3 == -1).sum() > 0
 
 
-@pytest.mark.parametrize("variant", ["plain", "adaptive_scaled", "thermal"])
+@pytest.mark.parametrize("variant", ["plain", "adaptive_scaled", "thermal", "movers"])
 def test_graph_replay_matches_eager_launches(variant):
     """Steps replayed from the captured hipGraphs give bitwise the results of eager launches, with host reads
-    and eager work between the replays (the buffers rotate through several configurations)."""
+    and eager work between the replays (the buffers rotate through several configurations).  "movers": a skin
+    of 0.05 h, so the replays run deferred walks, clear their queue and keep the mover list in order."""
     sc = scene.pbf_dam_break(18, 14, 12)
     pos, vel = sc.particles()
     temps = np.where(pos[:, 0] < 0.3, 240.0, 270.0).astype(np.float32)
@@ -190,6 +191,8 @@ def test_graph_replay_matches_eager_launches(variant):
         if variant == "thermal":
             assert s.lib.rxPBDSPH_GPU_SetThermal(s.h, temps.ctypes.data, None, phase.ctypes.data) == 0
             s.option(_abi.RX_PBF_OPT_HEAT_TD, 20.0)
+        if variant == "movers":
+            s.option(_abi.RX_PBF_OPT_SKIN, 0.05)
         return s
 
     eager, graphed = make(0), make(1)
@@ -204,6 +207,8 @@ def test_graph_replay_matches_eager_launches(variant):
         se, sg = eager.stats(), graphed.stats()
         bad = np.nonzero(~np.isclose(se, sg, rtol=1e-4, atol=1e-7))[0]
         assert bad.size == 0, [(int(b), float(se[b]), float(sg[b])) for b in bad]
+        if variant == "movers":
+            assert se[25] > 0 and se[20] > 0, (se[25], se[20])  # movers, and their grid walks (deferred)
     eager.close()
     graphed.close()
 
