@@ -25,6 +25,12 @@ constexpr int kP_B2 = kP_W2 + kMlpH * kMlpH;
 constexpr int kP_W3 = kP_B2 + kMlpH;
 constexpr int kP_B3 = kP_W3 + kMlpH;
 constexpr int kNumParams = kP_B3 + 1;
+// Gradient partials, summed in a fixed order by k_mlp_reduce (no float atomics: a run's training is the same
+// every time).  Per wavefront of k_mlp_bwd_rows: b1, b2, w3, b3 and the loss; per workgroup of k_mlp_wgrad:
+// W2 then W1.
+constexpr int kRowB1 = 0, kRowB2 = kMlpH, kRowW3 = 2 * kMlpH, kRowB3 = 3 * kMlpH, kRowLoss = 3 * kMlpH + 1, kRowP = 3 * kMlpH + 2;
+constexpr int kWgW2 = 0, kWgW1 = kMlpH * kMlpH, kWgP = kMlpH * kMlpH + kMlpH * kMlpIn;
+constexpr int kWgRowsPerBlock = 256;
 
 __device__ inline unsigned short f2bf(float f) {  // round to nearest even
     uint u = __float_as_uint(f);
@@ -209,13 +215,15 @@ __global__ __launch_bounds__(256) void k_mlp_fwd_train(const float* __restrict__
 }
 
 // Backward over rows: dy = (y - t) / n, dz2 = dy w3 silu'(z2), dh1 = dz2 W2 (MFMA), dz1 = dh1 silu'(z1).
-// Accumulates db1, db2, dw3, db3 and the loss with atomics; stores dz1, dz2 for the weight gradients.
+// Writes this wavefront's db1, db2, dw3, db3 and loss to its partial (rowp + wavefront * kRowP); stores dz1, dz2
+// for the weight gradients.
 __global__ __launch_bounds__(256) void k_mlp_bwd_rows(const float* __restrict__ t, const float* __restrict__ y, const float* __restrict__ z1,
                                                       const float* __restrict__ z2, uint n, MlpDev m, float* __restrict__ dz1,
-                                                      float* __restrict__ dz2, float* __restrict__ grad, float* __restrict__ loss) {
+                                                      float* __restrict__ dz2, float* __restrict__ rowp) {
     __shared__ __align__(16) unsigned short lds[4][16 * kMlpH];
     const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, j = l & 15, kb = l >> 4, rb = l >> 4;
     uint r0 = (blockIdx.x * 4 + wave) * 16;
+    float* part = rowp + (size_t)(blockIdx.x * 4 + wave) * kRowP;
     const float inv_n = 1.0f / (float)n;
     float dy[4];
     float lsum = 0.0f;
@@ -252,8 +260,8 @@ __global__ __launch_bounds__(256) void k_mlp_bwd_rows(const float* __restrict__ 
         gw3 += __shfl_xor(gw3, 16, 64);
         gw3 += __shfl_xor(gw3, 32, 64);
         if (rb == 0) {
-            atomicAdd(&grad[kP_B2 + o], gb2);
-            atomicAdd(&grad[kP_W3 + o], gw3);
+            part[kRowB2 + o] = gb2;
+            part[kRowW3 + o] = gw3;
         }
     }
     if (j == 0)
@@ -280,7 +288,7 @@ __global__ __launch_bounds__(256) void k_mlp_bwd_rows(const float* __restrict__ 
         }
         gb1 += __shfl_xor(gb1, 16, 64);
         gb1 += __shfl_xor(gb1, 32, 64);
-        if (rb == 0) atomicAdd(&grad[kP_B1 + k], gb1);
+        if (rb == 0) part[kRowB1 + k] = gb1;
     }
     // loss and db3: wave reduction
 #pragma unroll
@@ -289,8 +297,8 @@ __global__ __launch_bounds__(256) void k_mlp_bwd_rows(const float* __restrict__ 
         db3 += __shfl_xor(db3, off, 64);
     }
     if (l == 0) {
-        atomicAdd(&grad[kP_B3], db3);
-        atomicAdd(loss, lsum * inv_n);
+        part[kRowB3] = db3;
+        part[kRowLoss] = lsum * inv_n;
     }
 }
 
@@ -299,7 +307,7 @@ __global__ __launch_bounds__(256) void k_mlp_bwd_rows(const float* __restrict__ 
 constexpr int kWgRows = 64;
 __global__ __launch_bounds__(256) void k_mlp_wgrad(const float* __restrict__ X, const float* __restrict__ z1,
                                                    const float* __restrict__ dz1, const float* __restrict__ dz2, uint n,
-                                                   int rows_per_block, MlpDev m, float* __restrict__ grad) {
+                                                   int rows_per_block, MlpDev m, float* __restrict__ wgp) {
     __shared__ __align__(16) unsigned short s_dz2[kWgRows * kMlpH], s_h1[kWgRows * kMlpH], s_dz1[kWgRows * kMlpH], s_x[kWgRows * kMlpIn];
     const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, j = l & 15, kb = l >> 4;
     const int mt = wave;  // output rows 16 mt .. 16 mt + 15 (units o)
@@ -352,9 +360,31 @@ __global__ __launch_bounds__(256) void k_mlp_wgrad(const float* __restrict__ X, 
     for (int e = 0; e < 4; ++e) {
         int o = 16 * mt + 4 * (l >> 4) + e;
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) atomicAdd(&grad[kP_W2 + o * kMlpH + 16 * nt + j], acc2[nt][e]);
-        atomicAdd(&grad[kP_W1 + o * kMlpIn + j], acc1[e]);
+        for (int nt = 0; nt < 4; ++nt) wgp[(size_t)blockIdx.x * kWgP + kWgW2 + o * kMlpH + 16 * nt + j] = acc2[nt][e];
+        wgp[(size_t)blockIdx.x * kWgP + kWgW1 + o * kMlpIn + j] = acc1[e];
     }
+}
+
+// The gradient (and the loss at kNumParams) from the partials, each summed in partial order.
+__global__ void k_mlp_reduce(const float* __restrict__ rowp, uint nrow, const float* __restrict__ wgp, uint nwg,
+                             float* __restrict__ grad) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > kNumParams) return;
+    int q;
+    bool rows = true;
+    if (p < kP_B1) q = kWgW1 + (p - kP_W1), rows = false;
+    else if (p < kP_W2) q = kRowB1 + (p - kP_B1);
+    else if (p < kP_B2) q = kWgW2 + (p - kP_W2), rows = false;
+    else if (p < kP_W3) q = kRowB2 + (p - kP_B2);
+    else if (p < kP_B3) q = kRowW3 + (p - kP_W3);
+    else if (p == kP_B3) q = kRowB3;
+    else q = kRowLoss;
+    float s = 0.0f;
+    if (rows)
+        for (uint b = 0; b < nrow; ++b) s += rowp[(size_t)b * kRowP + q];
+    else
+        for (uint b = 0; b < nwg; ++b) s += wgp[(size_t)b * kWgP + q];
+    grad[p] = s;
 }
 
 // Adam on the FP32 master parameters, then the bf16 copies used by the MFMA kernels.
@@ -470,7 +500,9 @@ public:
         const bool ok = RX_HIP(hipMalloc(&d_z1, sizeof(float) * kMlpH * b)) && RX_HIP(hipMalloc(&d_z2, sizeof(float) * kMlpH * b)) &&
                         RX_HIP(hipMalloc(&d_dz1, sizeof(float) * kMlpH * b)) && RX_HIP(hipMalloc(&d_dz2, sizeof(float) * kMlpH * b)) &&
                         RX_HIP(hipMalloc(&d_y, sizeof(float) * b)) && RX_HIP(hipMalloc(&d_xb, sizeof(float) * kMlpIn * b)) &&
-                        RX_HIP(hipMalloc(&d_tb, sizeof(float) * b));
+                        RX_HIP(hipMalloc(&d_tb, sizeof(float) * b)) &&
+                        RX_HIP(hipMalloc(&d_rowp, sizeof(float) * kRowP * 4 * ((b + 63) / 64))) &&
+                        RX_HIP(hipMalloc(&d_wgp, sizeof(float) * kWgP * ((b + kWgRowsPerBlock - 1) / kWgRowsPerBlock)));
         if (!ok) {
             free_batch();
             return false;
@@ -479,7 +511,7 @@ public:
         return true;
     }
     void free_batch() {
-        for (float** q : {&d_z1, &d_z2, &d_dz1, &d_dz2, &d_y, &d_xb, &d_tb})
+        for (float** q : {&d_z1, &d_z2, &d_dz1, &d_dz2, &d_y, &d_xb, &d_tb, &d_rowp, &d_wgp})
             if (*q) {
                 RX_HIP(hipFree(*q));
                 *q = nullptr;
@@ -492,12 +524,12 @@ public:
         if (!ensure_batch(n)) return false;
         hipStream_t s = m_stream;
         MlpDev v = view();
-        rx_fill<float>(d_g, 0.0f, kNumParams + 1, s);
         unsigned blocks = (n + 63) / 64;
         k_mlp_fwd_train<<<blocks, 256, 0, s>>>(X, n, v, d_z1, d_z2, d_y);
-        k_mlp_bwd_rows<<<blocks, 256, 0, s>>>(T, d_y, d_z1, d_z2, n, v, d_dz1, d_dz2, d_g, d_g + kNumParams);
-        int rpb = 256;
-        k_mlp_wgrad<<<(n + rpb - 1) / rpb, 256, 0, s>>>(X, d_z1, d_dz1, d_dz2, n, rpb, v, d_g);
+        k_mlp_bwd_rows<<<blocks, 256, 0, s>>>(T, d_y, d_z1, d_z2, n, v, d_dz1, d_dz2, d_rowp);
+        const unsigned wblocks = (n + kWgRowsPerBlock - 1) / kWgRowsPerBlock;
+        k_mlp_wgrad<<<wblocks, 256, 0, s>>>(X, d_z1, d_dz1, d_dz2, n, kWgRowsPerBlock, v, d_wgp);
+        k_mlp_reduce<<<rx_blocks(kNumParams + 1, 256), 256, 0, s>>>(d_rowp, 4 * blocks, d_wgp, wblocks, d_g);
         return true;
     }
 
@@ -555,6 +587,7 @@ public:
     float *d_p = nullptr, *d_g = nullptr, *d_m1 = nullptr, *d_m2 = nullptr, *d_norm = nullptr;
     unsigned short *d_w1 = nullptr, *d_w2 = nullptr, *d_w2t = nullptr;
     float *d_z1 = nullptr, *d_z2 = nullptr, *d_dz1 = nullptr, *d_dz2 = nullptr, *d_y = nullptr, *d_xb = nullptr, *d_tb = nullptr;
+    float *d_rowp = nullptr, *d_wgp = nullptr;  // gradient partials (per wavefront / per workgroup)
     uint m_bcap = 0;
     bool m_ok = false;  // every array of the constructor was allocated
     float m_out_scale = 1.0f, m_out_bias = 0.0f;

@@ -75,3 +75,21 @@ def test_training_reduces_loss():
     assert hist[-1] < 0.1 * hist[0], hist
     y = net.forward(X[:2000])
     assert np.mean((y - T[:2000]) ** 2) < 0.05
+
+
+def test_training_is_bitwise_reproducible():
+    # the gradient sums run over per-wavefront and per-workgroup partials in a fixed order (no float atomics),
+    # so two trainings on the same data and seed end with the same parameters, bit for bit
+    rng = np.random.default_rng(3)
+    n = 8192
+    X = rng.normal(size=(n, 16)).astype(np.float32)
+    X[:, 12:] = 0.0
+    T = (np.tanh(X[:, 0]) + 0.3 * X[:, 1] * X[:, 4]).astype(np.float32)
+    params = []
+    for _ in range(2):
+        net = Network(seed=2)
+        net.set_normalization(np.zeros(12), np.ones(12), 1.0, 0.0)
+        net.train(X, T, epochs=3, batch=1024, lr=3e-3)
+        params.append(net.params())
+        net.close()
+    assert np.array_equal(params[0].view(np.uint32), params[1].view(np.uint32))
