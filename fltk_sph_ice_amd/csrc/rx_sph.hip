@@ -115,10 +115,11 @@ __global__ __launch_bounds__(kTpb) void k_sph_force(const float* __restrict__ de
         [&](uint, const PosIdx& a, const Pair& b) {
             if (!b.in) return;
             float3 rij = p0 - f3(a.x);
-            float r = sqrtf(dot3(rij, rij));
+            float r2 = dot3(rij, rij);
+            float ri = __builtin_amdgcn_rsqf(r2);
             float prsj = b.pr / (b.d * b.d);
-            float q = k.h - r;
-            fp += (k.mass * (prsi + prsj) * k.gwspiky * q * q / r) * rij;
+            float q = k.h - r2 * ri;
+            fp += (k.mass * (prsi + prsj) * k.gwspiky * q * q * ri) * rij;
             fv += (k.mass / b.d * k.lwvisc * q) * (f3(b.v) - v0);
         });
     float3 a = (-1.0f) * fp + k.visc * fv + k.gravity;
@@ -288,7 +289,8 @@ __global__ __launch_bounds__(kTpb) void k_pbf_ext_force(const float* __restrict_
         [&](uint, const float4& x, const Pair& b) {
             if (!b.in) return;
             float3 rij = p0 - f3(x);
-            float r = sqrtf(dot3(rij, rij));
+            float r2 = dot3(rij, rij);
+            float r = r2 * __builtin_amdgcn_rsqf(r2);  // r2 > 0 (the pair test above)
             fv += (k.lwvisc * (k.h - r)) * (f3(b.v) - v0);
         });
     // rx_sph_pbd.cpp:676: viscosity uses the particle's own density
@@ -313,9 +315,9 @@ __global__ __launch_bounds__(kTpb) void k_pbf_lambda(float* __restrict__ dens, f
                 float q2 = k.h2 - r2;
                 d += k.mass * k.wpoly6 * q2 * q2 * q2;
                 if (r2 > 0.0f) {
-                    float r = sqrtf(r2);
-                    float q = k.h - r;
-                    float3 g = (k.gwspiky * q * q / r * k.inv_rho0) * rij;
+                    float ri = __builtin_amdgcn_rsqf(r2);
+                    float q = k.h - r2 * ri;
+                    float3 g = (k.gwspiky * k.inv_rho0 * q * q * ri) * rij;
                     gs += g;
                     sd += dot3(g, g);
                 }
@@ -354,15 +356,15 @@ __global__ __launch_bounds__(kTpb) void k_pbf_dp(const float* __restrict__ scl, 
             if (!b.in) return;
             float3 rij = p - f3(a.x);
             float r2 = dot3(rij, rij);
-            float r = sqrtf(r2);
+            float ri = __builtin_amdgcn_rsqf(r2);
             float sc = 0.0f;
             if (k.ap) {
                 float q2 = k.h2 - r2;
                 float ww = k.wpoly6 * q2 * q2 * q2 * k.inv_ap_wq;
                 sc = -k.ap_k * pow_n(ww, k.ap_n) * dt2;
             }
-            float q = k.h - r;
-            acc += ((si + b.s + sc) * k.gwspiky * q * q / r * k.inv_rho0) * rij;
+            float q = k.h - r2 * ri;
+            acc += ((si + b.s + sc) * k.gwspiky * k.inv_rho0 * q * q * ri) * rij;
         });
     dp[c.dSortedIndex[i]] = make_float4(acc.x, acc.y, acc.z, 0.0f);
 }
@@ -756,9 +758,9 @@ __device__ inline void boundary_lambda_terms(float3 p, const float* __restrict__
             float w = kb.h2 - r2;
             d += kb.rho0 * q.v * kb.wpoly6 * w * w * w;
             if (r2 > 0.0f) {
-                float r = sqrtf(r2);
-                float t = kb.h - r;
-                float3 g = (kb.rho0 * q.v / kb.mass * kb.gwspiky * t * t / r * kb.inv_rho0) * rij;
+                float ri = __builtin_amdgcn_rsqf(r2);
+                float t = kb.h - r2 * ri;
+                float3 g = (kb.rho0 * q.v / kb.mass * kb.gwspiky * kb.inv_rho0 * t * t * ri) * rij;
                 sd += dot3(g, g);
             }
         });
@@ -785,9 +787,9 @@ __global__ __launch_bounds__(kTpb) void k_pbf_lambda_wb(float* __restrict__ dens
                 float q2 = k.h2 - r2;
                 d += k.mass * k.wpoly6 * q2 * q2 * q2;
                 if (r2 > 0.0f) {
-                    float r = sqrtf(r2);
-                    float q = k.h - r;
-                    float3 g = (k.gwspiky * q * q / r * k.inv_rho0) * rij;
+                    float ri = __builtin_amdgcn_rsqf(r2);
+                    float q = k.h - r2 * ri;
+                    float3 g = (k.gwspiky * k.inv_rho0 * q * q * ri) * rij;
                     gs += g;
                     sd += dot3(g, g);
                 }
@@ -820,9 +822,9 @@ __global__ __launch_bounds__(kTpb) void k_pbf_boundary_lambda(float* __restrict_
                 float q2 = k.h2 - r2;
                 d += k.mass * k.wpoly6 * q2 * q2 * q2;
                 if (r2 > 0.0f) {
-                    float r = sqrtf(r2);
-                    float q = k.h - r;
-                    float3 g = (k.gwspiky * q * q / r * k.inv_rho0) * rij;
+                    float ri = __builtin_amdgcn_rsqf(r2);
+                    float q = k.h - r2 * ri;
+                    float3 g = (k.gwspiky * k.inv_rho0 * q * q * ri) * rij;
                     sd += dot3(g, g);
                 }
             }
@@ -861,15 +863,15 @@ __global__ __launch_bounds__(kTpb) void k_pbf_dp_wb(const float* __restrict__ sc
             if (!q.in) return;
             float3 rij = p - f3(a.x);
             float r2 = dot3(rij, rij);
-            float r = sqrtf(r2);
+            float ri = __builtin_amdgcn_rsqf(r2);
             float sc = 0.0f;
             if (k.ap) {
                 float q2 = k.h2 - r2;
                 float ww = k.wpoly6 * q2 * q2 * q2 * k.inv_ap_wq;
                 sc = -k.ap_k * pow_n(ww, k.ap_n) * dt2;
             }
-            float t = k.h - r;
-            acc += ((si + q.s + sc) * k.gwspiky * t * t / r * k.inv_rho0) * rij;
+            float t = k.h - r2 * ri;
+            acc += ((si + q.s + sc) * k.gwspiky * k.inv_rho0 * t * t * ri) * rij;
         });
     for_each_candidate(
         p, kb, b.dCellStart, b.dCellEnd, [&](uint j) { return PosIdx{b.dSortedPos[j], b.dSortedIndex[j]}; },
@@ -888,15 +890,15 @@ __global__ __launch_bounds__(kTpb) void k_pbf_dp_wb(const float* __restrict__ sc
             if (!q.in) return;
             float3 rij = p - f3(a.x);
             float r2 = dot3(rij, rij);
-            float r = sqrtf(r2);
+            float ri = __builtin_amdgcn_rsqf(r2);
             float sc = 0.0f;
             if (kb.ap) {
                 float q2 = kb.h2 - r2;
                 float ww = kb.rho0 * q.v / kb.mass * kb.wpoly6 * q2 * q2 * q2 * kb.inv_ap_wq;
                 sc = -kb.ap_k * pow_n(ww, kb.ap_n) * dt2;
             }
-            float t = kb.h - r;
-            acc += ((si + q.s + sc) * kb.gwspiky * t * t / r * kb.inv_rho0) * rij;
+            float t = kb.h - r2 * ri;
+            acc += ((si + q.s + sc) * kb.gwspiky * kb.inv_rho0 * t * t * ri) * rij;
         });
     dp[c.dSortedIndex[i]] = make_float4(acc.x, acc.y, acc.z, 0.0f);
 }

@@ -116,9 +116,9 @@ __global__ __launch_bounds__(kTpb) void k_force_integrate(const float4* __restri
                 float3 r = p - f3(q.x);
                 float r2 = dot3(r, r);
                 if (r2 <= k.s.h2 && r2 > 0.0f) {
-                    float rr = sqrtf(r2);
-                    float qq = k.s.h - rr;
-                    fp += ((prsi + q.d.y) * qq * qq / rr) * r;
+                    float ri = __builtin_amdgcn_rsqf(r2);
+                    float qq = k.s.h - r2 * ri;
+                    fp += ((prsi + q.d.y) * qq * qq * ri) * r;
                     fv += (qq / q.d.x) * (f3(q.v) - vi);
                 }
             });
