@@ -117,10 +117,11 @@ __global__ __launch_bounds__(kTpb) void k_sph_force(const float* __restrict__ de
             float3 rij = p0 - f3(a.x);
             float r2 = dot3(rij, rij);
             float ri = __builtin_amdgcn_rsqf(r2);
-            float prsj = b.pr / (b.d * b.d);
+            float id = __builtin_amdgcn_rcpf(b.d);  // 1 / rho_j, rho_j > 0 (the particle's own term)
+            float prsj = b.pr * id * id;
             float q = k.h - r2 * ri;
             fp += (k.mass * (prsi + prsj) * k.gwspiky * q * q * ri) * rij;
-            fv += (k.mass / b.d * k.lwvisc * q) * (f3(b.v) - v0);
+            fv += (k.mass * id * k.lwvisc * q) * (f3(b.v) - v0);
         });
     float3 a = (-1.0f) * fp + k.visc * fv + k.gravity;
     frc[o] = make_float4(a.x, a.y, a.z, 0.0f);
@@ -150,7 +151,7 @@ __global__ __launch_bounds__(kTpb) void k_sph_normal(float4* __restrict__ nrm, c
             if (!b.in) return;
             float3 rij = p0 - f3(a.x);
             float q = k.h2 - dot3(rij, rij);
-            n += (k.mass / b.d * k.gwpoly6 * q * q) * rij;
+            n += (k.mass * __builtin_amdgcn_rcpf(b.d) * k.gwpoly6 * q * q) * rij;
         });
     float l = sqrtf(dot3(n, n));
     if (l > 0.0f) n = (1.0f / l) * n;
@@ -423,7 +424,7 @@ __global__ __launch_bounds__(kTpb) void k_xsph(const float4* __restrict__ vel, f
             if (!b.in) return;
             float3 rij = p - f3(a.x);
             float q = k.h2 - dot3(rij, rij);
-            acc += (k.mass / b.d * k.wpoly6 * q * q * q) * (f3(b.v) - v0);
+            acc += (k.mass * __builtin_amdgcn_rcpf(b.d) * k.wpoly6 * q * q * q) * (f3(b.v) - v0);
         });
     float3 v = v0 + cx * acc;
     nvel[o] = make_float4(v.x, v.y, v.z, 0.0f);
@@ -726,9 +727,10 @@ __global__ __launch_bounds__(kTpb) void k_sph_boundary_force(const float* __rest
         [&](uint, const PosIdx& a, const Pair& q) {
             if (!q.in) return;
             float3 rij = p - f3(a.x);
-            float r = sqrtf(dot3(rij, rij));
-            float w = kb.h - r;
-            f += (-kb.rho0 * q.v * prsi * kb.gwspiky * w * w / r) * rij;
+            float r2 = dot3(rij, rij);
+            float ri = __builtin_amdgcn_rsqf(r2);  // r > 1e-4 (the pair test above)
+            float w = kb.h - r2 * ri;
+            f += (-kb.rho0 * q.v * prsi * kb.gwspiky * w * w * ri) * rij;
         });
     float4 a = frc[i];
     frc[i] = make_float4(a.x + f.x, a.y + f.y, a.z + f.z, a.w);
@@ -738,6 +740,7 @@ __global__ __launch_bounds__(kTpb) void k_sph_boundary_force(const float* __rest
 // (rx_pbdsph_kernel.cu:533-560, rx_sph_pbd.cpp:790-812).
 __device__ inline void boundary_lambda_terms(float3 p, const float* __restrict__ vol, const rxParticleCell& b, const SphK& kb,
                                              float& d, float& sd) {
+    const float gv = kb.rho0 / kb.mass * kb.gwspiky * kb.inv_rho0;  // gradient factor per unit boundary volume
     struct Pair {
         bool in;
         float v;
@@ -760,7 +763,7 @@ __device__ inline void boundary_lambda_terms(float3 p, const float* __restrict__
             if (r2 > 0.0f) {
                 float ri = __builtin_amdgcn_rsqf(r2);
                 float t = kb.h - r2 * ri;
-                float3 g = (kb.rho0 * q.v / kb.mass * kb.gwspiky * kb.inv_rho0 * t * t * ri) * rij;
+                float3 g = (q.v * gv * t * t * ri) * rij;
                 sd += dot3(g, g);
             }
         });
@@ -844,6 +847,7 @@ __global__ __launch_bounds__(kTpb) void k_pbf_dp_wb(const float* __restrict__ sc
     float3 p = f3(c.dSortedPos[i]);
     float si = scl[c.dSortedIndex[i]];
     float dt2 = k.dt * k.dt;
+    const float wv = kb.rho0 / kb.mass * kb.wpoly6 * kb.inv_ap_wq;  // s_corr's W ratio per unit boundary volume
     float3 acc = f3(0, 0, 0);
     struct Pair {
         bool in;
@@ -894,7 +898,7 @@ __global__ __launch_bounds__(kTpb) void k_pbf_dp_wb(const float* __restrict__ sc
             float sc = 0.0f;
             if (kb.ap) {
                 float q2 = kb.h2 - r2;
-                float ww = kb.rho0 * q.v / kb.mass * kb.wpoly6 * q2 * q2 * q2 * kb.inv_ap_wq;
+                float ww = q.v * wv * q2 * q2 * q2;
                 sc = -kb.ap_k * pow_n(ww, kb.ap_n) * dt2;
             }
             float t = kb.h - r2 * ri;
