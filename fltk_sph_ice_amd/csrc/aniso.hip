@@ -11,8 +11,8 @@
 //
 // Summation order follows the reference: cells z, y, x over the clamped cell box, each cell's terms summed on
 // their own and the cell sum added to the total (the covariance entries are summed straight into the total,
-// as there).  Every multiply, add and divide rounds on its own (no contraction); with IEEE division and square
-// root the results are bit-identical to the CPU restatement.
+// as there).  Every multiply, add and divide rounds on its own (no contraction); with correctly rounded division
+// (IEEE, or div_h below) and square root the results are bit-identical to the CPU restatement.
 //
 // The 3x3 decomposition is a cyclic Jacobi iteration in FP64 on the symmetric C (the reference runs a general
 // SVD; for a symmetric positive semi-definite C the singular values are the eigenvalues and U = V).  The
@@ -106,6 +106,15 @@ __device__ inline void for_cells(float3 p, int3 lo, int3 hi, const AnisoK& k, co
 // r2 above this has sqrtf(r2) >= h: the square root can be skipped for candidates beyond it
 __device__ inline float sqrt_skip2(float h) { return h * h * 1.000001f; }
 
+// r / h correctly rounded, from inv_h = 1 / h (correctly rounded) and one correction with the exact remainder
+// (Markstein, "Computation of elementary functions on the IBM RISC System/6000 processor", IBM J. Res. Dev. 34(1),
+// 1990): q0 = r inv_h is faithful, e = r - q0 h is exact in one fma, and q0 + e inv_h rounds to r / h.  The
+// remainder must not underflow, which holds for r = 0 and for every r = sqrtf(r2) of a float r2 > 0 (r >= 2^-75).
+__device__ inline float div_h(float r, float h, float inv_h) {
+    const float q0 = r * inv_h;
+    return fmaf(fmaf(-q0, h, r), inv_h, q0);
+}
+
 // rx_aniso_kernel.cu:33-63 (calWeighedAvgPositionCell), summed over the cells of p +- h
 __device__ inline float4 weighted_mean_sum(float3 p, float h, const float4* __restrict__ sp, const uint* __restrict__ cs,
                                            const uint* __restrict__ ce, const AnisoK& k) {
@@ -113,6 +122,7 @@ __device__ inline float4 weighted_mean_sum(float3 p, float h, const float4* __re
     const int3 hi = cell_of(make_float3(p.x + h, p.y + h, p.z + h), k);
     const float reach2 = h * h * 1.0001f;  // a cell farther than h (with margin) has no particle with r < h
     const float far2 = sqrt_skip2(h);
+    const float inv_h = 1.0f / h;
     float4 tot = make_float4(0.0f, 0.0f, 0.0f, 0.0f), part = tot;
     for_cells(
         p, lo, hi, k, cs, ce, [&](uint j) { return sp[j]; },
@@ -122,7 +132,7 @@ __device__ inline float4 weighted_mean_sum(float3 p, float h, const float4* __re
             if (r2 > far2) return;
             const float r = sqrtf(r2);
             if (r < h) {
-                const float a = 1.0f - r / h;
+                const float a = 1.0f - div_h(r, h, inv_h);
                 const float w = a * a * a;
                 part.x = part.x + q.x * w;
                 part.y = part.y + q.y * w;
@@ -169,6 +179,7 @@ __global__ __launch_bounds__(kTpb) void k_aniso_cov(float4* __restrict__ posw, f
     const int3 hi = cell_of(make_float3(p.x + h, p.y + h, p.z + h), k);
     const float reach2 = h * h * 1.0001f;
     const float far2 = sqrt_skip2(h);
+    const float inv_h = 1.0f / h;
     float cv[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     float wsum = 0.0f, cnt = 0.0f, pw = 0.0f, pc = 0.0f;
     for_cells(
@@ -179,7 +190,7 @@ __global__ __launch_bounds__(kTpb) void k_aniso_cov(float4* __restrict__ posw, f
             if (r2 > far2) return;
             const float r = sqrtf(r2);
             if (r < h) {
-                const float a = 1.0f - r / h;
+                const float a = 1.0f - div_h(r, h, inv_h);
                 const float w = a * a * a;
                 const float d[3] = {q.x - m.x, q.y - m.y, q.z - m.z};
 #pragma unroll
