@@ -154,7 +154,7 @@ __device__ inline float4 weighted_mean_sum(float3 p, float h, const float4* __re
 // rx_aniso_kernel.cu:72-108 (sphCalUpdatedPosition)
 __global__ __launch_bounds__(kTpb) void k_aniso_smooth(float4* __restrict__ up, float4* __restrict__ wpos, float lambda, float h,
                                                        rxParticleCell c, AnisoK k) {
-    const uint i = blockIdx.x * kTpb + threadIdx.x;
+    const uint i = xcd_thread();
     if (i >= c.uNumParticles) return;
     const float4 p4 = c.dSortedPos[i];
     const float3 p = make_float3(p4.x, p4.y, p4.z);
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(kTpb) void k_aniso_smooth(float4* __restrict__ up, 
 // rx_aniso_kernel.cu:118-229 (calCovarianceMatrixCell, sphCalCovarianceMatrix)
 __global__ __launch_bounds__(kTpb) void k_aniso_cov(float4* __restrict__ posw, float* __restrict__ cmat, float h, rxParticleCell c,
                                                     AnisoK k) {
-    const uint i = blockIdx.x * kTpb + threadIdx.x;
+    const uint i = xcd_thread();
     if (i >= c.uNumParticles) return;
     const float4 p4 = c.dSortedPos[i];
     const float3 p = make_float3(p4.x, p4.y, p4.z);

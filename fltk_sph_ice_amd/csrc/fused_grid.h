@@ -52,15 +52,6 @@ __device__ inline void fg_for_stencil(int3 c, const FGridK& g, const uint* __res
         }
 }
 
-// Blocks are dealt round-robin to the 8 XCDs of the MI355X; this remap gives each XCD a contiguous run of
-// blocks (hence of sorted particles) so that neighbour re-reads hit the XCD's own L2.
-__device__ inline uint xcd_block(uint b, uint nb) {
-    uint per = nb >> 3, rem = nb & 7u;
-    uint x = b & 7u, k = b >> 3;
-    // blocks of XCD x get the logical range starting at x * per + min(x, rem)
-    return x * per + min(x, rem) + k;
-}
-
 struct FusedGrid {
     FGridK k{};
     uint n = 0;

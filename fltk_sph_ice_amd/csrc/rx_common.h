@@ -90,6 +90,18 @@ __device__ inline void row_range(const uint* __restrict__ cs, const uint* __rest
 // --------------------------------------------------------------------------------------------- launch
 inline unsigned rx_blocks(size_t n, unsigned tpb) { return (unsigned)((n + tpb - 1) / tpb); }
 
+// Blocks are dealt round-robin to the 8 XCDs of the MI355X; this remap gives each XCD a contiguous run of
+// blocks (hence of sorted particles) so that neighbour re-reads hit the XCD's own L2.
+__device__ inline uint xcd_block(uint b, uint nb) {
+    uint per = nb >> 3, rem = nb & 7u;
+    uint x = b & 7u, k = b >> 3;
+    // blocks of XCD x get the logical range starting at x * per + min(x, rem)
+    return x * per + min(x, rem) + k;
+}
+// Thread index of a one-dimensional launch under the remap above.  For passes over sorted particles, whose work
+// per slot is even; a lattice pass keeps the round-robin order, as its empty regions would idle whole XCDs.
+__device__ inline uint xcd_thread() { return xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; }
+
 // Generic grid-stride fill used instead of hipMemsetAsync (graph-safe).
 template <class T> __global__ void k_fill(T* __restrict__ p, T v, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;

@@ -63,7 +63,7 @@ __device__ inline float pow_n(float x, float n) { return (n == 4.0f) ? (x * x) *
 // ------------------------------------------------------------------------------------------ WCSPH
 // rx_sph_kernel.cu:216-288 (sphCalDensity)
 __global__ __launch_bounds__(kTpb) void k_sph_density(float* __restrict__ dens, float* __restrict__ pres, rxParticleCell c, SphK k, int write_pres) {
-    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint i = xcd_thread();
     if (i >= c.uNumParticles) return;
     float3 p = f3(c.dSortedPos[i]);
     float d = 0.0f;
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(kTpb) void k_sph_density(float* __restrict__ dens, 
 // rx_sph_kernel.cu:394-488 (sphCalForces) with the CPU path's pair rule (rx_sph_cpu.cpp:584: j != i, r > 0).
 __global__ __launch_bounds__(kTpb) void k_sph_force(const float* __restrict__ dens, const float* __restrict__ pres,
                                                     float4* __restrict__ frc, rxParticleCell c, SphK k) {
-    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint i = xcd_thread();
     if (i >= c.uNumParticles) return;
     float3 p0 = f3(c.dSortedPos[i]);
     float3 v0 = f3(c.dSortedVel[i]);
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(kTpb) void k_sph_force(const float* __restrict__ de
 
 // rx_sph_kernel.cu:301-376 (sphCalNormal)
 __global__ __launch_bounds__(kTpb) void k_sph_normal(float4* __restrict__ nrm, const float* __restrict__ dens, rxParticleCell c, SphK k) {
-    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint i = xcd_thread();
     if (i >= c.uNumParticles) return;
     float3 p0 = f3(c.dSortedPos[i]);
     float3 n = f3(0, 0, 0);
@@ -200,7 +200,7 @@ __global__ void k_check_delete(float4* pos, float4* vel, int* attr, float3 mn, f
 // computed without contraction so that membership at r == h is the plain IEEE result.
 __global__ __launch_bounds__(kTpb) void k_detect_neighbors(int* __restrict__ nb, rxParticleCell c, SphK k, int nmax) {
 #pragma clang fp contract(off)
-    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint i = xcd_thread();
     if (i >= c.uNumParticles) return;
     uint o = c.dSortedIndex[i];
     float3 p = f3(c.dSortedPos[i]);
@@ -266,7 +266,7 @@ __global__ void k_detect_surface(const int* __restrict__ nb, int* __restrict__ s
 // ------------------------------------------------------------------------------------------ PBF
 // rx_pbdsph_kernel.cu:71-101
 __global__ __launch_bounds__(kTpb) void k_pbf_ext_force(const float* __restrict__ dens, float4* __restrict__ frc, rxParticleCell c, SphK k) {
-    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint i = xcd_thread();
     if (i >= c.uNumParticles) return;
     float3 p0 = f3(c.dSortedPos[i]);
     float3 v0 = f3(c.dSortedVel[i]);
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(kTpb) void k_pbf_ext_force(const float* __restrict_
 // rx_pbdsph_kernel.cu:253-303 with the self term of rx_sph_pbd.cpp:767-784:
 // sd = sum_{k != i} |grad W_ik / rho0|^2 + |sum_j grad W_ij / rho0|^2.
 __global__ __launch_bounds__(kTpb) void k_pbf_lambda(float* __restrict__ dens, float* __restrict__ scl, float eps, rxParticleCell c, SphK k) {
-    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint i = xcd_thread();
     if (i >= c.uNumParticles) return;
     float3 p = f3(c.dSortedPos[i]);
     float d = 0.0f, sd = 0.0f;
@@ -333,7 +333,7 @@ __global__ __launch_bounds__(kTpb) void k_pbf_lambda(float* __restrict__ dens, f
 
 // rx_pbdsph_kernel.cu:374-407 / rx_sph_pbd.cpp:925-1026
 __global__ __launch_bounds__(kTpb) void k_pbf_dp(const float* __restrict__ scl, float4* __restrict__ dp, rxParticleCell c, SphK k) {
-    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint i = xcd_thread();
     if (i >= c.uNumParticles) return;
     float3 p = f3(c.dSortedPos[i]);
     float si = scl[c.dSortedIndex[i]];
@@ -397,7 +397,7 @@ __global__ void k_pbf_update_position(float4* __restrict__ pos, const float4* __
 // rx_pbdsph_kernel.cu:1130-1162 (xsphVisocosity)
 __global__ __launch_bounds__(kTpb) void k_xsph(const float4* __restrict__ vel, float4* __restrict__ nvel, const float* __restrict__ dens,
                                                float cx, rxParticleCell c, SphK k) {
-    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint i = xcd_thread();
     if (i >= c.uNumParticles) return;
     float3 p = f3(c.dSortedPos[i]);
     uint o = c.dSortedIndex[i];
@@ -645,7 +645,7 @@ SphK bgrid_consts(const rxSimParams& p) {
 // rx_sph_kernel.cu:500-564 (sphCalBoundaryVolume): V_b = m / sum_{b'} m W(r_bb') over boundary particles,
 // itself included (rx_sph_cpu.cpp:667-693).
 __global__ __launch_bounds__(kTpb) void k_boundary_volume(float* __restrict__ vol, rxParticleCell b, SphK kb) {
-    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint i = xcd_thread();
     if (i >= b.uNumParticles) return;
     float3 p = f3(b.dSortedPos[i]);
     float mw = 0.0f;
@@ -691,7 +691,7 @@ __device__ inline float boundary_density(float3 p, const float* __restrict__ vol
 __global__ __launch_bounds__(kTpb) void k_sph_boundary_density(float* __restrict__ dens, float* __restrict__ pres,
                                                                const float4* __restrict__ pos, const float* __restrict__ vol,
                                                                rxParticleCell b, SphK kb, uint n, int write_pres) {
-    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint i = xcd_thread();
     if (i >= n) return;
     float d = boundary_density(f3(pos[i]), vol, b, kb) + dens[i];
     dens[i] = d;
@@ -704,7 +704,7 @@ __global__ __launch_bounds__(kTpb) void k_sph_boundary_density(float* __restrict
 __global__ __launch_bounds__(kTpb) void k_sph_boundary_force(const float* __restrict__ dens, const float* __restrict__ pres,
                                                              const float4* __restrict__ pos, const float* __restrict__ vol,
                                                              float4* __restrict__ frc, rxParticleCell b, SphK kb, uint n) {
-    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint i = xcd_thread();
     if (i >= n) return;
     float3 p = f3(pos[i]);
     float d0 = dens[i];
@@ -776,7 +776,7 @@ __device__ inline void boundary_lambda_terms(float3 p, const float* __restrict__
 // term included, rx_sph_pbd.cpp:699-822): rho_i and sd_i gain the boundary terms.
 __global__ __launch_bounds__(kTpb) void k_pbf_lambda_wb(float* __restrict__ dens, float* __restrict__ scl, float eps, rxParticleCell c,
                                                         SphK k, const float* __restrict__ vol, rxParticleCell b, SphK kb) {
-    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint i = xcd_thread();
     if (i >= c.uNumParticles) return;
     float3 p = f3(c.dSortedPos[i]);
     float d = 0.0f, sd = 0.0f;
@@ -812,7 +812,7 @@ __global__ __launch_bounds__(kTpb) void k_pbf_lambda_wb(float* __restrict__ dens
 // within h counts (the GPU kernel skips the fluid slot whose number equals the boundary slot's).
 __global__ __launch_bounds__(kTpb) void k_pbf_boundary_lambda(float* __restrict__ sclb, float eps, rxParticleCell c, SphK k,
                                                               const float* __restrict__ vol, rxParticleCell b, SphK kb) {
-    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint i = xcd_thread();
     if (i >= b.uNumParticles) return;
     float3 p = f3(b.dSortedPos[i]);
     float d = 0.0f, sd = 0.0f;
@@ -842,7 +842,7 @@ __global__ __launch_bounds__(kTpb) void k_pbf_boundary_lambda(float* __restrict_
 __global__ __launch_bounds__(kTpb) void k_pbf_dp_wb(const float* __restrict__ scl, float4* __restrict__ dp, rxParticleCell c, SphK k,
                                                     const float* __restrict__ vol, const float* __restrict__ sclb,
                                                     rxParticleCell b, SphK kb) {
-    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint i = xcd_thread();
     if (i >= c.uNumParticles) return;
     float3 p = f3(c.dSortedPos[i]);
     float si = scl[c.dSortedIndex[i]];
