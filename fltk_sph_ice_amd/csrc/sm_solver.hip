@@ -283,7 +283,7 @@ __global__ __launch_bounds__(256) void k_sm_clusters(const int2* __restrict__ un
                                                      const int* __restrict__ act, SmK k) {
     const int wpb = blockDim.x >> 6;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int u = blockIdx.x * wpb + wave; u < nunits; u += gridDim.x * wpb) {
+    for (int u = (int)xcd_block(blockIdx.x, gridDim.x) * wpb + wave; u < nunits; u += gridDim.x * wpb) {
         int2 un = units[u];
         int T = tiles[un.x];
         int idx = un.y + lane / T;
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(256) void k_sm_clusters(const int2* __restrict__ un
 __global__ void k_sm_average(const int* __restrict__ ps, const int* __restrict__ pc, const float4* __restrict__ tr,
                              const float4* __restrict__ rest, const float4* __restrict__ xs, float4* __restrict__ xo,
                              const int* __restrict__ act, SmK k) {
-    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint i = xcd_thread();
     if (i >= k.n) return;
     int s = ps[i], e = ps[i + 1];
     float3 x = f3(xs[i]);
