@@ -13,6 +13,8 @@
 namespace {
 
 constexpr int kTpb = 256;
+// Library scratch slots (rx_scratch) of this file: the slot-order copies that the pair loops read.
+constexpr int kSlotScalars = 13, kSlotBoundary = 14, kSlotVelDens = 15;
 
 // Visits the slots of all particles in the cells overlapped by [p - h, p + h] (the cell range of
 // rx_sph_kernel.cu:266-278), row by row and in slot order.  Candidates go in batches of four: `load(j)` for all
@@ -83,15 +85,16 @@ __global__ __launch_bounds__(kTpb) void k_sph_density(float* __restrict__ dens, 
 }
 
 // rx_sph_kernel.cu:394-488 (sphCalForces) with the CPU path's pair rule (rx_sph_cpu.cpp:584: j != i, r > 0).
-__global__ __launch_bounds__(kTpb) void k_sph_force(const float* __restrict__ dens, const float* __restrict__ pres,
-                                                    float4* __restrict__ frc, rxParticleCell c, SphK k) {
+// sdp: (rho, p) in slot order (k_slot_pairs).
+__global__ __launch_bounds__(kTpb) void k_sph_force(const float2* __restrict__ sdp, float4* __restrict__ frc, rxParticleCell c, SphK k) {
     uint i = xcd_thread();
     if (i >= c.uNumParticles) return;
     float3 p0 = f3(c.dSortedPos[i]);
     float3 v0 = f3(c.dSortedVel[i]);
     uint o = c.dSortedIndex[i];
-    float d0 = dens[o];
-    float prsi = pres[o] / (d0 * d0);
+    const float2 own = sdp[i];
+    float d0 = own.x;
+    float prsi = own.y / (d0 * d0);
     float3 fp = f3(0, 0, 0), fv = f3(0, 0, 0);
     struct Pair {
         bool in;
@@ -99,22 +102,23 @@ __global__ __launch_bounds__(kTpb) void k_sph_force(const float* __restrict__ de
         float d, pr;
     };
     for_each_candidate(
-        p0, k, c.dCellStart, c.dCellEnd, [&](uint j) { return PosIdx{c.dSortedPos[j], c.dSortedIndex[j]}; },
-        [&](uint j, const PosIdx& a) {
+        p0, k, c.dCellStart, c.dCellEnd, [&](uint j) { return c.dSortedPos[j]; },
+        [&](uint j, const float4& x) {
             Pair b{};
-            float3 rij = p0 - f3(a.x);
+            float3 rij = p0 - f3(x);
             float r2 = dot3(rij, rij);
             b.in = j != i && r2 <= k.h2 && r2 > 0.0f;
             if (b.in) {
                 b.v = c.dSortedVel[j];
-                b.d = dens[a.o];
-                b.pr = pres[a.o];
+                const float2 t = sdp[j];
+                b.d = t.x;
+                b.pr = t.y;
             }
             return b;
         },
-        [&](uint, const PosIdx& a, const Pair& b) {
+        [&](uint, const float4& x, const Pair& b) {
             if (!b.in) return;
-            float3 rij = p0 - f3(a.x);
+            float3 rij = p0 - f3(x);
             float r2 = dot3(rij, rij);
             float ri = __builtin_amdgcn_rsqf(r2);
             float id = __builtin_amdgcn_rcpf(b.d);  // 1 / rho_j, rho_j > 0 (the particle's own term)
@@ -128,7 +132,8 @@ __global__ __launch_bounds__(kTpb) void k_sph_force(const float* __restrict__ de
 }
 
 // rx_sph_kernel.cu:301-376 (sphCalNormal)
-__global__ __launch_bounds__(kTpb) void k_sph_normal(float4* __restrict__ nrm, const float* __restrict__ dens, rxParticleCell c, SphK k) {
+// sd: rho in slot order (k_slot_scalars).
+__global__ __launch_bounds__(kTpb) void k_sph_normal(float4* __restrict__ nrm, const float* __restrict__ sd, rxParticleCell c, SphK k) {
     uint i = xcd_thread();
     if (i >= c.uNumParticles) return;
     float3 p0 = f3(c.dSortedPos[i]);
@@ -138,18 +143,18 @@ __global__ __launch_bounds__(kTpb) void k_sph_normal(float4* __restrict__ nrm, c
         float d;
     };
     for_each_candidate(
-        p0, k, c.dCellStart, c.dCellEnd, [&](uint j) { return PosIdx{c.dSortedPos[j], c.dSortedIndex[j]}; },
-        [&](uint j, const PosIdx& a) {
+        p0, k, c.dCellStart, c.dCellEnd, [&](uint j) { return c.dSortedPos[j]; },
+        [&](uint j, const float4& x) {
             Pair b{};
-            float3 rij = p0 - f3(a.x);
+            float3 rij = p0 - f3(x);
             float r = sqrtf(dot3(rij, rij));
             b.in = j != i && r <= k.h && r > 0.0001f;
-            if (b.in) b.d = dens[a.o];
+            if (b.in) b.d = sd[j];
             return b;
         },
-        [&](uint, const PosIdx& a, const Pair& b) {
+        [&](uint, const float4& x, const Pair& b) {
             if (!b.in) return;
-            float3 rij = p0 - f3(a.x);
+            float3 rij = p0 - f3(x);
             float q = k.h2 - dot3(rij, rij);
             n += (k.mass * __builtin_amdgcn_rcpf(b.d) * k.gwpoly6 * q * q) * rij;
         });
@@ -331,12 +336,36 @@ __global__ __launch_bounds__(kTpb) void k_pbf_lambda(float* __restrict__ dens, f
     dens[o] = d;
 }
 
-// rx_pbdsph_kernel.cu:374-407 / rx_sph_pbd.cpp:925-1026
-__global__ __launch_bounds__(kTpb) void k_pbf_dp(const float* __restrict__ scl, float4* __restrict__ dp, rxParticleCell c, SphK k) {
+// Per-particle values in slot (sorted) order, so that a pair loop reads its neighbours' values from contiguous
+// slots instead of gathering them by original index: one gather per particle instead of one per pair.
+__global__ void k_slot_scalars(const float* __restrict__ src, const uint* __restrict__ idx, float* __restrict__ dst, uint n) {
+    uint i = xcd_thread();
+    if (i < n) dst[i] = src[idx[i]];
+}
+__global__ void k_slot_pairs(const float* __restrict__ a, const float* __restrict__ b, const uint* __restrict__ idx,
+                             float2* __restrict__ dst, uint n) {
+    uint i = xcd_thread();
+    if (i < n) {
+        const uint o = idx[i];
+        dst[i] = make_float2(a[o], b[o]);
+    }
+}
+__global__ void k_slot_vel_dens(const float4* __restrict__ vel, const float* __restrict__ dens, const uint* __restrict__ idx,
+                                float4* __restrict__ dst, uint n) {
+    uint i = xcd_thread();
+    if (i < n) {
+        const uint o = idx[i];
+        const float4 v = vel[o];
+        dst[i] = make_float4(v.x, v.y, v.z, dens[o]);
+    }
+}
+
+// rx_pbdsph_kernel.cu:374-407 / rx_sph_pbd.cpp:925-1026.  ss: the multipliers in slot order (k_slot_scalars).
+__global__ __launch_bounds__(kTpb) void k_pbf_dp(const float* __restrict__ ss, float4* __restrict__ dp, rxParticleCell c, SphK k) {
     uint i = xcd_thread();
     if (i >= c.uNumParticles) return;
     float3 p = f3(c.dSortedPos[i]);
-    float si = scl[c.dSortedIndex[i]];
+    float si = ss[i];
     float dt2 = k.dt * k.dt;
     float3 acc = f3(0, 0, 0);
     struct Pair {
@@ -344,18 +373,18 @@ __global__ __launch_bounds__(kTpb) void k_pbf_dp(const float* __restrict__ scl, 
         float s;
     };
     for_each_candidate(
-        p, k, c.dCellStart, c.dCellEnd, [&](uint j) { return PosIdx{c.dSortedPos[j], c.dSortedIndex[j]}; },
-        [&](uint, const PosIdx& a) {
+        p, k, c.dCellStart, c.dCellEnd, [&](uint j) { return c.dSortedPos[j]; },
+        [&](uint j, const float4& x) {
             Pair b{};
-            float3 rij = p - f3(a.x);
+            float3 rij = p - f3(x);
             float r2 = dot3(rij, rij);
             b.in = r2 <= k.h2 && r2 > 0.0f;
-            if (b.in) b.s = scl[a.o];
+            if (b.in) b.s = ss[j];
             return b;
         },
-        [&](uint, const PosIdx& a, const Pair& b) {
+        [&](uint, const float4& x, const Pair& b) {
             if (!b.in) return;
-            float3 rij = p - f3(a.x);
+            float3 rij = p - f3(x);
             float r2 = dot3(rij, rij);
             float ri = __builtin_amdgcn_rsqf(r2);
             float sc = 0.0f;
@@ -395,36 +424,33 @@ __global__ void k_pbf_update_position(float4* __restrict__ pos, const float4* __
 }
 
 // rx_pbdsph_kernel.cu:1130-1162 (xsphVisocosity)
-__global__ __launch_bounds__(kTpb) void k_xsph(const float4* __restrict__ vel, float4* __restrict__ nvel, const float* __restrict__ dens,
-                                               float cx, rxParticleCell c, SphK k) {
+// sv: (v, rho) in slot order (k_slot_vel_dens).
+__global__ __launch_bounds__(kTpb) void k_xsph(const float4* __restrict__ sv, float4* __restrict__ nvel, float cx, rxParticleCell c,
+                                               SphK k) {
     uint i = xcd_thread();
     if (i >= c.uNumParticles) return;
     float3 p = f3(c.dSortedPos[i]);
     uint o = c.dSortedIndex[i];
-    float3 v0 = f3(vel[o]);
+    float3 v0 = f3(sv[i]);
     float3 acc = f3(0, 0, 0);
     struct Pair {
         bool in;
-        float d;
         float4 v;
     };
     for_each_candidate(
-        p, k, c.dCellStart, c.dCellEnd, [&](uint j) { return PosIdx{c.dSortedPos[j], c.dSortedIndex[j]}; },
-        [&](uint, const PosIdx& a) {
+        p, k, c.dCellStart, c.dCellEnd, [&](uint j) { return c.dSortedPos[j]; },
+        [&](uint j, const float4& x) {
             Pair b{};
-            float3 rij = p - f3(a.x);
+            float3 rij = p - f3(x);
             b.in = dot3(rij, rij) <= k.h2;
-            if (b.in) {
-                b.d = dens[a.o];
-                b.v = vel[a.o];
-            }
+            if (b.in) b.v = sv[j];
             return b;
         },
-        [&](uint, const PosIdx& a, const Pair& b) {
+        [&](uint, const float4& x, const Pair& b) {
             if (!b.in) return;
-            float3 rij = p - f3(a.x);
+            float3 rij = p - f3(x);
             float q = k.h2 - dot3(rij, rij);
-            acc += (k.mass * __builtin_amdgcn_rcpf(b.d) * k.wpoly6 * q * q * q) * (f3(b.v) - v0);
+            acc += (k.mass * __builtin_amdgcn_rcpf(b.v.w) * k.wpoly6 * q * q * q) * (f3(b.v) - v0);
         });
     float3 v = v0 + cx * acc;
     nvel[o] = make_float4(v.x, v.y, v.z, 0.0f);
@@ -663,24 +689,24 @@ __global__ __launch_bounds__(kTpb) void k_boundary_volume(float* __restrict__ vo
 }
 
 // Boundary term of a fluid density at p: sum_b rho0 V_b W(r).
-__device__ inline float boundary_density(float3 p, const float* __restrict__ vol, const rxParticleCell& b, const SphK& kb) {
+__device__ inline float boundary_density(float3 p, const float* __restrict__ vs, const rxParticleCell& b, const SphK& kb) {
     struct Pair {
         bool in;
         float v;
     };
     float d = 0.0f;
     for_each_candidate(
-        p, kb, b.dCellStart, b.dCellEnd, [&](uint j) { return PosIdx{b.dSortedPos[j], b.dSortedIndex[j]}; },
-        [&](uint, const PosIdx& a) {
+        p, kb, b.dCellStart, b.dCellEnd, [&](uint j) { return b.dSortedPos[j]; },
+        [&](uint j, const float4& x) {
             Pair q{};
-            float3 r = p - f3(a.x);
+            float3 r = p - f3(x);
             q.in = dot3(r, r) <= kb.h2;
-            if (q.in) q.v = vol[a.o];
+            if (q.in) q.v = vs[j];
             return q;
         },
-        [&](uint, const PosIdx& a, const Pair& q) {
+        [&](uint, const float4& x, const Pair& q) {
             if (!q.in) return;
-            float3 r = p - f3(a.x);
+            float3 r = p - f3(x);
             float w = kb.h2 - dot3(r, r);
             d += kb.rho0 * q.v * kb.wpoly6 * w * w * w;
         });
@@ -689,11 +715,11 @@ __device__ inline float boundary_density(float3 p, const float* __restrict__ vol
 
 // rx_sph_kernel.cu:575-650 (sphCalBoundaryDensity): rho_i += boundary term, p_i = k (rho_i - rho0); original order.
 __global__ __launch_bounds__(kTpb) void k_sph_boundary_density(float* __restrict__ dens, float* __restrict__ pres,
-                                                               const float4* __restrict__ pos, const float* __restrict__ vol,
+                                                               const float4* __restrict__ pos, const float* __restrict__ vs,
                                                                rxParticleCell b, SphK kb, uint n, int write_pres) {
     uint i = xcd_thread();
     if (i >= n) return;
-    float d = boundary_density(f3(pos[i]), vol, b, kb) + dens[i];
+    float d = boundary_density(f3(pos[i]), vs, b, kb) + dens[i];
     dens[i] = d;
     if (write_pres) pres[i] = kb.gas_k * (d - kb.rho0);
 }
@@ -702,7 +728,7 @@ __global__ __launch_bounds__(kTpb) void k_sph_boundary_density(float* __restrict
 // for 1e-4 < r <= h.  This is the GPU form: an acceleration like the rest of dFrc.  The CPU loop
 // (rx_sph_cpu.cpp:617-651) divides the same sum by the particle mass, which is not an acceleration.
 __global__ __launch_bounds__(kTpb) void k_sph_boundary_force(const float* __restrict__ dens, const float* __restrict__ pres,
-                                                             const float4* __restrict__ pos, const float* __restrict__ vol,
+                                                             const float4* __restrict__ pos, const float* __restrict__ vs,
                                                              float4* __restrict__ frc, rxParticleCell b, SphK kb, uint n) {
     uint i = xcd_thread();
     if (i >= n) return;
@@ -715,18 +741,18 @@ __global__ __launch_bounds__(kTpb) void k_sph_boundary_force(const float* __rest
     };
     float3 f = f3(0, 0, 0);
     for_each_candidate(
-        p, kb, b.dCellStart, b.dCellEnd, [&](uint j) { return PosIdx{b.dSortedPos[j], b.dSortedIndex[j]}; },
-        [&](uint, const PosIdx& a) {
+        p, kb, b.dCellStart, b.dCellEnd, [&](uint j) { return b.dSortedPos[j]; },
+        [&](uint j, const float4& x) {
             Pair q{};
-            float3 rij = p - f3(a.x);
+            float3 rij = p - f3(x);
             float r = sqrtf(dot3(rij, rij));
             q.in = r <= kb.h && r > 0.0001f;
-            if (q.in) q.v = vol[a.o];
+            if (q.in) q.v = vs[j];
             return q;
         },
-        [&](uint, const PosIdx& a, const Pair& q) {
+        [&](uint, const float4& x, const Pair& q) {
             if (!q.in) return;
-            float3 rij = p - f3(a.x);
+            float3 rij = p - f3(x);
             float r2 = dot3(rij, rij);
             float ri = __builtin_amdgcn_rsqf(r2);  // r > 1e-4 (the pair test above)
             float w = kb.h - r2 * ri;
@@ -738,7 +764,7 @@ __global__ __launch_bounds__(kTpb) void k_sph_boundary_force(const float* __rest
 
 // Boundary terms of a PBF multiplier at p: the density sum (as above) and sum_b |(rho0 V_b / m) grad W / rho0|^2
 // (rx_pbdsph_kernel.cu:533-560, rx_sph_pbd.cpp:790-812).
-__device__ inline void boundary_lambda_terms(float3 p, const float* __restrict__ vol, const rxParticleCell& b, const SphK& kb,
+__device__ inline void boundary_lambda_terms(float3 p, const float* __restrict__ vs, const rxParticleCell& b, const SphK& kb,
                                              float& d, float& sd) {
     const float gv = kb.rho0 / kb.mass * kb.gwspiky * kb.inv_rho0;  // gradient factor per unit boundary volume
     struct Pair {
@@ -746,17 +772,17 @@ __device__ inline void boundary_lambda_terms(float3 p, const float* __restrict__
         float v;
     };
     for_each_candidate(
-        p, kb, b.dCellStart, b.dCellEnd, [&](uint j) { return PosIdx{b.dSortedPos[j], b.dSortedIndex[j]}; },
-        [&](uint, const PosIdx& a) {
+        p, kb, b.dCellStart, b.dCellEnd, [&](uint j) { return b.dSortedPos[j]; },
+        [&](uint j, const float4& x) {
             Pair q{};
-            float3 r = p - f3(a.x);
+            float3 r = p - f3(x);
             q.in = dot3(r, r) <= kb.h2;
-            if (q.in) q.v = vol[a.o];
+            if (q.in) q.v = vs[j];
             return q;
         },
-        [&](uint, const PosIdx& a, const Pair& q) {
+        [&](uint, const float4& x, const Pair& q) {
             if (!q.in) return;
-            float3 rij = p - f3(a.x);
+            float3 rij = p - f3(x);
             float r2 = dot3(rij, rij);
             float w = kb.h2 - r2;
             d += kb.rho0 * q.v * kb.wpoly6 * w * w * w;
@@ -775,7 +801,7 @@ __device__ inline void boundary_lambda_terms(float3 p, const float* __restrict__
 // rx_pbdsph_kernel.cu:577-657 (pbdsphCalScalingFactorWithBoundary) with the fluid terms of k_pbf_lambda (self
 // term included, rx_sph_pbd.cpp:699-822): rho_i and sd_i gain the boundary terms.
 __global__ __launch_bounds__(kTpb) void k_pbf_lambda_wb(float* __restrict__ dens, float* __restrict__ scl, float eps, rxParticleCell c,
-                                                        SphK k, const float* __restrict__ vol, rxParticleCell b, SphK kb) {
+                                                        SphK k, const float* __restrict__ vs, rxParticleCell b, SphK kb) {
     uint i = xcd_thread();
     if (i >= c.uNumParticles) return;
     float3 p = f3(c.dSortedPos[i]);
@@ -799,7 +825,7 @@ __global__ __launch_bounds__(kTpb) void k_pbf_lambda_wb(float* __restrict__ dens
             }
         });
     sd += dot3(gs, gs);
-    boundary_lambda_terms(p, vol, b, kb, d, sd);
+    boundary_lambda_terms(p, vs, b, kb, d, sd);
     float C = d * k.inv_rho0 - 1.0f;
     uint o = c.dSortedIndex[i];
     scl[o] = -C / (sd + eps);
@@ -811,7 +837,7 @@ __global__ __launch_bounds__(kTpb) void k_pbf_lambda_wb(float* __restrict__ dens
 // sd_b = sum_fluid |grad W / rho0|^2 + sum_b' |(rho0 V_b' / m) grad W / rho0|^2 (r > 0).  Every fluid particle
 // within h counts (the GPU kernel skips the fluid slot whose number equals the boundary slot's).
 __global__ __launch_bounds__(kTpb) void k_pbf_boundary_lambda(float* __restrict__ sclb, float eps, rxParticleCell c, SphK k,
-                                                              const float* __restrict__ vol, rxParticleCell b, SphK kb) {
+                                                              const float* __restrict__ vs, rxParticleCell b, SphK kb) {
     uint i = xcd_thread();
     if (i >= b.uNumParticles) return;
     float3 p = f3(b.dSortedPos[i]);
@@ -832,20 +858,20 @@ __global__ __launch_bounds__(kTpb) void k_pbf_boundary_lambda(float* __restrict_
                 }
             }
         });
-    boundary_lambda_terms(p, vol, b, kb, d, sd);
+    boundary_lambda_terms(p, vs, b, kb, d, sd);
     float C = d * k.inv_rho0 - 1.0f;
     sclb[b.dSortedIndex[i]] = -C / (sd + eps);
 }
 
 // rx_pbdsph_kernel.cu:760-860 (pbdsphPositionCorrectionWithBoundary), rx_sph_pbd.cpp:978-1018: the fluid terms of
 // k_pbf_dp plus sum_b (lambda_i + lambda_b + s_corr) grad W_spiky / rho0 with s_corr's W scaled by rho0 V_b / m.
-__global__ __launch_bounds__(kTpb) void k_pbf_dp_wb(const float* __restrict__ scl, float4* __restrict__ dp, rxParticleCell c, SphK k,
-                                                    const float* __restrict__ vol, const float* __restrict__ sclb,
-                                                    rxParticleCell b, SphK kb) {
+// ss: the fluid multipliers in slot order; sb: (lambda_b, V_b) in the boundary grid's slot order.
+__global__ __launch_bounds__(kTpb) void k_pbf_dp_wb(const float* __restrict__ ss, float4* __restrict__ dp, rxParticleCell c, SphK k,
+                                                    const float2* __restrict__ sb, rxParticleCell b, SphK kb) {
     uint i = xcd_thread();
     if (i >= c.uNumParticles) return;
     float3 p = f3(c.dSortedPos[i]);
-    float si = scl[c.dSortedIndex[i]];
+    float si = ss[i];
     float dt2 = k.dt * k.dt;
     const float wv = kb.rho0 / kb.mass * kb.wpoly6 * kb.inv_ap_wq;  // s_corr's W ratio per unit boundary volume
     float3 acc = f3(0, 0, 0);
@@ -854,18 +880,18 @@ __global__ __launch_bounds__(kTpb) void k_pbf_dp_wb(const float* __restrict__ sc
         float s, v;
     };
     for_each_candidate(
-        p, k, c.dCellStart, c.dCellEnd, [&](uint j) { return PosIdx{c.dSortedPos[j], c.dSortedIndex[j]}; },
-        [&](uint, const PosIdx& a) {
+        p, k, c.dCellStart, c.dCellEnd, [&](uint j) { return c.dSortedPos[j]; },
+        [&](uint j, const float4& x) {
             Pair q{};
-            float3 rij = p - f3(a.x);
+            float3 rij = p - f3(x);
             float r2 = dot3(rij, rij);
             q.in = r2 <= k.h2 && r2 > 0.0f;
-            if (q.in) q.s = scl[a.o];
+            if (q.in) q.s = ss[j];
             return q;
         },
-        [&](uint, const PosIdx& a, const Pair& q) {
+        [&](uint, const float4& x, const Pair& q) {
             if (!q.in) return;
-            float3 rij = p - f3(a.x);
+            float3 rij = p - f3(x);
             float r2 = dot3(rij, rij);
             float ri = __builtin_amdgcn_rsqf(r2);
             float sc = 0.0f;
@@ -878,21 +904,22 @@ __global__ __launch_bounds__(kTpb) void k_pbf_dp_wb(const float* __restrict__ sc
             acc += ((si + q.s + sc) * k.gwspiky * k.inv_rho0 * t * t * ri) * rij;
         });
     for_each_candidate(
-        p, kb, b.dCellStart, b.dCellEnd, [&](uint j) { return PosIdx{b.dSortedPos[j], b.dSortedIndex[j]}; },
-        [&](uint, const PosIdx& a) {
+        p, kb, b.dCellStart, b.dCellEnd, [&](uint j) { return b.dSortedPos[j]; },
+        [&](uint j, const float4& x) {
             Pair q{};
-            float3 rij = p - f3(a.x);
+            float3 rij = p - f3(x);
             float r2 = dot3(rij, rij);
             q.in = r2 <= kb.h2 && r2 > 0.0f;
             if (q.in) {
-                q.s = sclb[a.o];
-                q.v = vol[a.o];
+                const float2 t = sb[j];
+                q.s = t.x;
+                q.v = t.y;
             }
             return q;
         },
-        [&](uint, const PosIdx& a, const Pair& q) {
+        [&](uint, const float4& x, const Pair& q) {
             if (!q.in) return;
-            float3 rij = p - f3(a.x);
+            float3 rij = p - f3(x);
             float r2 = dot3(rij, rij);
             float ri = __builtin_amdgcn_rsqf(r2);
             float sc = 0.0f;
@@ -945,13 +972,21 @@ void CuSphDensity(float* dDens, float* dPres, rxParticleCell cell) {
 
 void CuSphForces(float* dDens, float* dPres, float* dFrc, rxParticleCell cell, float dt) {
     if (!need_params("CuSphForces") || !cell.uNumParticles) return;
-    k_sph_force<<<rx_blocks(cell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>(dDens, dPres, (float4*)dFrc, cell, rx_make_consts(rx_params()));
+    const uint n = cell.uNumParticles;
+    float2* sdp = (float2*)rx_scratch(kSlotScalars, sizeof(float2) * n);
+    if (!sdp) return;
+    k_slot_pairs<<<rx_blocks(n, kTpb), kTpb, 0, rx_stream()>>>(dDens, dPres, cell.dSortedIndex, sdp, n);
+    k_sph_force<<<rx_blocks(n, kTpb), kTpb, 0, rx_stream()>>>(sdp, (float4*)dFrc, cell, rx_make_consts(rx_params()));
     finish("CuSphForces");
 }
 
 void CuSphNormal(float* dNrms, float* dDens, rxParticleCell cell) {
     if (!need_params("CuSphNormal") || !cell.uNumParticles) return;
-    k_sph_normal<<<rx_blocks(cell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>((float4*)dNrms, dDens, cell, rx_make_consts(rx_params()));
+    const uint n = cell.uNumParticles;
+    float* sd = (float*)rx_scratch(kSlotScalars, sizeof(float) * n);
+    if (!sd) return;
+    k_slot_scalars<<<rx_blocks(n, kTpb), kTpb, 0, rx_stream()>>>(dDens, cell.dSortedIndex, sd, n);
+    k_sph_normal<<<rx_blocks(n, kTpb), kTpb, 0, rx_stream()>>>((float4*)dNrms, sd, cell, rx_make_consts(rx_params()));
     finish("CuSphNormal");
 }
 
@@ -1016,7 +1051,11 @@ void CuPbdSphScalingFactor(float* dPos, float* dDens, float* dScl, float eps, rx
 
 void CuPbdSphPositionCorrection(float* dPos, float* dScl, float* dDp, rxParticleCell cell) {
     if (!need_params("CuPbdSphPositionCorrection") || !cell.uNumParticles) return;
-    k_pbf_dp<<<rx_blocks(cell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>(dScl, (float4*)dDp, cell, rx_make_consts(rx_params()));
+    const uint n = cell.uNumParticles;
+    float* ss = (float*)rx_scratch(kSlotScalars, sizeof(float) * n);
+    if (!ss) return;
+    k_slot_scalars<<<rx_blocks(n, kTpb), kTpb, 0, rx_stream()>>>(dScl, cell.dSortedIndex, ss, n);
+    k_pbf_dp<<<rx_blocks(n, kTpb), kTpb, 0, rx_stream()>>>(ss, (float4*)dDp, cell, rx_make_consts(rx_params()));
     finish("CuPbdSphPositionCorrection");
 }
 
@@ -1040,8 +1079,11 @@ void CuPbdSphUpdateVelocity(float* pos, float* new_pos, float* new_vel, float dt
 
 void CuXSphViscosity(float* dPos, float* dVel, float* dNewVel, float* dDens, float c, rxParticleCell cell) {
     if (!need_params("CuXSphViscosity") || !cell.uNumParticles) return;
-    k_xsph<<<rx_blocks(cell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>((const float4*)dVel, (float4*)dNewVel, dDens, c, cell,
-                                                                         rx_make_consts(rx_params()));
+    const uint n = cell.uNumParticles;
+    float4* sv = (float4*)rx_scratch(kSlotVelDens, sizeof(float4) * n);
+    if (!sv) return;
+    k_slot_vel_dens<<<rx_blocks(n, kTpb), kTpb, 0, rx_stream()>>>((const float4*)dVel, dDens, cell.dSortedIndex, sv, n);
+    k_xsph<<<rx_blocks(n, kTpb), kTpb, 0, rx_stream()>>>(sv, (float4*)dNewVel, c, cell, rx_make_consts(rx_params()));
     finish("CuXSphViscosity");
 }
 
@@ -1098,10 +1140,20 @@ void CuSphBoundaryVolume(float* dVolB, float mass, rxParticleCell cell) {
     finish("CuSphBoundaryVolume");
 }
 
+// dVolB in the boundary grid's slot order (library scratch), or null after a failed allocation.
+float* boundary_volumes(const float* vol, const rxParticleCell& b) {
+    const uint nb = b.uNumParticles;
+    float* vs = (float*)rx_scratch(kSlotBoundary, sizeof(float) * (nb ? nb : 1));
+    if (vs && nb) k_slot_scalars<<<rx_blocks(nb, kTpb), kTpb, 0, rx_stream()>>>(vol, b.dSortedIndex, vs, nb);
+    return vs;
+}
+
 // rx_sph.cu:520-532
 void CuSphBoundaryDensity(float* dDens, float* dPres, float* dPos, float* dVolB, rxParticleCell bcell, uint pnum) {
     if (!need_params("CuSphBoundaryDensity") || !boundary_grid_ok(bcell, "CuSphBoundaryDensity") || !pnum) return;
-    k_sph_boundary_density<<<rx_blocks(pnum, kTpb), kTpb, 0, rx_stream()>>>(dDens, dPres, (const float4*)dPos, dVolB, bcell,
+    const float* vs = boundary_volumes(dVolB, bcell);
+    if (!vs) return;
+    k_sph_boundary_density<<<rx_blocks(pnum, kTpb), kTpb, 0, rx_stream()>>>(dDens, dPres, (const float4*)dPos, vs, bcell,
                                                                             bgrid_consts(rx_params()), pnum, 1);
     finish("CuSphBoundaryDensity");
 }
@@ -1109,7 +1161,9 @@ void CuSphBoundaryDensity(float* dDens, float* dPres, float* dPos, float* dVolB,
 // rx_sph.cu:541-553
 void CuSphBoundaryForces(float* dDens, float* dPres, float* dPos, float* dVolB, float* dFrc, rxParticleCell bcell, uint pnum) {
     if (!need_params("CuSphBoundaryForces") || !boundary_grid_ok(bcell, "CuSphBoundaryForces") || !pnum) return;
-    k_sph_boundary_force<<<rx_blocks(pnum, kTpb), kTpb, 0, rx_stream()>>>(dDens, dPres, (const float4*)dPos, dVolB, (float4*)dFrc,
+    const float* vs = boundary_volumes(dVolB, bcell);
+    if (!vs) return;
+    k_sph_boundary_force<<<rx_blocks(pnum, kTpb), kTpb, 0, rx_stream()>>>(dDens, dPres, (const float4*)dPos, vs, (float4*)dFrc,
                                                                           bcell, bgrid_consts(rx_params()), pnum);
     finish("CuSphBoundaryForces");
 }
@@ -1117,7 +1171,9 @@ void CuSphBoundaryForces(float* dDens, float* dPres, float* dPos, float* dVolB, 
 // rx_sph.cu:754-766
 void CuPbdSphBoundaryDensity(float* dDens, float* dPos, float* dVolB, rxParticleCell bcell, uint pnum) {
     if (!need_params("CuPbdSphBoundaryDensity") || !boundary_grid_ok(bcell, "CuPbdSphBoundaryDensity") || !pnum) return;
-    k_sph_boundary_density<<<rx_blocks(pnum, kTpb), kTpb, 0, rx_stream()>>>(dDens, nullptr, (const float4*)dPos, dVolB, bcell,
+    const float* vs = boundary_volumes(dVolB, bcell);
+    if (!vs) return;
+    k_sph_boundary_density<<<rx_blocks(pnum, kTpb), kTpb, 0, rx_stream()>>>(dDens, nullptr, (const float4*)dPos, vs, bcell,
                                                                             bgrid_consts(rx_params()), pnum, 0);
     finish("CuPbdSphBoundaryDensity");
 }
@@ -1127,10 +1183,12 @@ void CuPbdSphScalingFactorWithBoundary(float* dPos, float* dDens, float* dScl, f
                                        float* dSclB, rxParticleCell bcell) {
     if (!need_params("CuPbdSphScalingFactorWithBoundary") || !boundary_grid_ok(bcell, "CuPbdSphScalingFactorWithBoundary")) return;
     SphK k = rx_make_consts(rx_params()), kb = bgrid_consts(rx_params());
+    const float* vs = boundary_volumes(dVolB, bcell);
+    if (!vs) return;
     if (cell.uNumParticles)
-        k_pbf_lambda_wb<<<rx_blocks(cell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>(dDens, dScl, eps, cell, k, dVolB, bcell, kb);
+        k_pbf_lambda_wb<<<rx_blocks(cell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>(dDens, dScl, eps, cell, k, vs, bcell, kb);
     if (bcell.uNumParticles)
-        k_pbf_boundary_lambda<<<rx_blocks(bcell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>(dSclB, eps, cell, k, dVolB, bcell, kb);
+        k_pbf_boundary_lambda<<<rx_blocks(bcell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>(dSclB, eps, cell, k, vs, bcell, kb);
     finish("CuPbdSphScalingFactorWithBoundary");
 }
 
@@ -1140,8 +1198,14 @@ void CuPbdSphPositionCorrectionWithBoundary(float* dPos, float* dScl, float* dDp
     if (!need_params("CuPbdSphPositionCorrectionWithBoundary") || !boundary_grid_ok(bcell, "CuPbdSphPositionCorrectionWithBoundary") ||
         !cell.uNumParticles)
         return;
-    k_pbf_dp_wb<<<rx_blocks(cell.uNumParticles, kTpb), kTpb, 0, rx_stream()>>>(dScl, (float4*)dDp, cell, rx_make_consts(rx_params()),
-                                                                                dVolB, dSclB, bcell, bgrid_consts(rx_params()));
+    const uint n = cell.uNumParticles, nb = bcell.uNumParticles;
+    float* ss = (float*)rx_scratch(kSlotScalars, sizeof(float) * n);
+    float2* sb = (float2*)rx_scratch(kSlotBoundary, sizeof(float2) * (nb ? nb : 1));
+    if (!ss || !sb) return;
+    k_slot_scalars<<<rx_blocks(n, kTpb), kTpb, 0, rx_stream()>>>(dScl, cell.dSortedIndex, ss, n);
+    if (nb) k_slot_pairs<<<rx_blocks(nb, kTpb), kTpb, 0, rx_stream()>>>(dSclB, dVolB, bcell.dSortedIndex, sb, nb);
+    k_pbf_dp_wb<<<rx_blocks(n, kTpb), kTpb, 0, rx_stream()>>>(ss, (float4*)dDp, cell, rx_make_consts(rx_params()), sb, bcell,
+                                                               bgrid_consts(rx_params()));
     finish("CuPbdSphPositionCorrectionWithBoundary");
 }
 

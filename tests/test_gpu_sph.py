@@ -70,6 +70,37 @@ def test_forces_against_pairwise_scale(lib):
     assert err <= TOL, err
 
 
+def test_normals_against_pairwise_sums(lib):
+    """CuSphNormal (rx_sph_kernel.cu:301-376): n_i = sum_{j != i, 1e-4 < r <= h} (m / rho_j) GWpoly6 (h^2 - r^2)^2
+    (x_i - x_j), normalised; against the same sum over all pairs in FP64, on a jittered block whose densities
+    differ per particle (so that rho_j must come from the neighbour, not the particle)."""
+    sc = scene.wcsph_dam_break(11, 9, 10)
+    pos, vel = sc.particles()
+    pos[:, :3] += np.random.default_rng(5).uniform(-0.15, 0.15, (sc.n, 3)).astype(np.float32) * np.float32(sc.spacing)
+    sp, g, c, dpos, dvel = _grid(lib, sc, pos, vel)
+    n = sc.n
+    rho = (sp.Density * (1.0 + 0.2 * np.random.default_rng(6).random(n))).astype(np.float32)
+    dd, dn = DevBuf.from_array(rho), DevBuf(16 * n)
+    lib.CuSphNormal(dn.ptr, dd.ptr, c)
+    check(lib)
+    got = dn.download(np.float32).reshape(n, 4)[:, :3]
+    x = pos[:, :3].astype(np.float64)
+    h = float(sp.EffectiveRadius)
+    d = x[:, None, :] - x[None, :, :]
+    r = np.sqrt((d * d).sum(-1))
+    w = np.where((r <= h) & (r > 1e-4), (float(sp.Mass) / rho.astype(np.float64))[None, :] * float(sp.GWpoly6) * (h * h - r * r) ** 2, 0.0)
+    np.fill_diagonal(w, 0.0)
+    ref = (w[:, :, None] * d).sum(1)
+    scale = (np.abs(w)[:, :, None] * np.abs(d)).sum(1).max(1)  # sum_j |n_ij|, before the normalisation
+    ln = np.linalg.norm(ref, axis=1)
+    ok = ln > 1e-3 * scale  # directions of near-cancelling sums are not determined by the data
+    assert ok.sum() > n // 4
+    err = np.abs(got[ok] - ref[ok] / ln[ok, None]).max(1) * (ln[ok] / scale[ok])
+    print(f"WCSPH normals: max per-particle error {err.max():.2e} of sum_j |n_ij| / |n_i|")
+    assert err.max() <= TOL
+    assert np.allclose(np.linalg.norm(got[ok], axis=1), 1.0, atol=1e-5)
+
+
 def test_integrate_free_fall_and_walls(lib):
     sc = scene.wcsph_dam_break(4, 4, 4)
     sp = sc.params()
