@@ -21,6 +21,7 @@ constexpr int kRWaves = kRThreads / 64;
 constexpr int kRBatch = 4;   // rounds whose loads are issued together
 constexpr int kRDig = 1 << kRBits;
 constexpr int kRSplit = kRThreads / kRDig;  // threads per digit in the offset step
+constexpr int kRCntLoads = kRMaxTiles / kRSplit / 2;  // count-matrix loads in flight per thread in the offset step
 
 // Lanes of the wavefront that are valid and hold digit d (nbits bits).
 __device__ inline unsigned long long digit_match(uint d, bool valid, int nbits) {
@@ -111,15 +112,15 @@ __global__ __launch_bounds__(kRThreads) void k_radix_scatter(const uint* __restr
         // sum interleaved tiles (the counts of earlier tiles, and of all tiles for the digit totals)
         const int d = threadIdx.x % kRDig, q = threadIdx.x / kRDig;
         uint below = 0, all = 0;
-        for (uint t0 = q; t0 < rs.ntiles; t0 += 16 * kRSplit) {  // 16 loads in flight per thread
-            uint c[16];
+        for (uint t0 = q; t0 < rs.ntiles; t0 += kRCntLoads * kRSplit) {  // two batches of loads per thread
+            uint c[kRCntLoads];
 #pragma unroll
-            for (int u = 0; u < 16; ++u) {
+            for (int u = 0; u < kRCntLoads; ++u) {
                 const uint t = t0 + u * kRSplit;
                 c[u] = t < rs.ntiles ? cnt[(size_t)t * kRDig + d] : 0u;
             }
 #pragma unroll
-            for (int u = 0; u < 16; ++u) {
+            for (int u = 0; u < kRCntLoads; ++u) {
                 all += c[u];
                 below += t0 + u * kRSplit < blockIdx.x ? c[u] : 0u;
             }
@@ -161,28 +162,36 @@ __global__ __launch_bounds__(kRThreads) void k_radix_scatter(const uint* __restr
     const uint mask = (1u << nbits) - 1u;
     const unsigned long long below = (1ull << lane) - 1ull;
     uint* mine = wcnt + (w << nbits);
-    for (size_t r = wbase; r < wend; r += 64) {
-        const size_t i = r + lane;
-        const bool valid = i < wend;
-        const uint key = valid ? keys[i] : 0u;
-        const uint val = valid ? (vals ? vals[i] : (uint)i) : 0u;
-        const uint d = (key >> shift) & mask;
-        const unsigned long long m = digit_match(d, valid, nbits);
-        // every lane reads the running slot before the lowest lane of its digit advances it (a wavefront's LDS
-        // accesses complete in program order)
-        const uint base = mine[d];
-        if (valid) {
-            const uint pos = base + (uint)__builtin_popcountll(m & below);
-            okeys[pos] = key;
-            ovals[pos] = val;
-            if (gs.n4) {
+    for (size_t r0 = wbase; r0 < wend; r0 += 64 * kRBatch) {
+        uint kb[kRBatch], vb[kRBatch];  // the loads of kRBatch rounds are issued before the first round is placed
 #pragma unroll
-                for (int a = 0; a < kRGatherMax; ++a)
-                    if (a < gs.n4) gs.dst4[a][pos] = gs.src4[a][val];
-                gs.dstu[pos] = gs.srcu[val];
-                gs.zero[pos] = 0.0f;
+        for (int u = 0; u < kRBatch; ++u) {
+            const size_t i = r0 + (size_t)u * 64 + lane;
+            kb[u] = i < wend ? keys[i] : 0u;
+            vb[u] = i < wend ? (vals ? vals[i] : (uint)i) : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < kRBatch; ++u) {
+            const bool valid = r0 + (size_t)u * 64 + lane < wend;
+            const uint key = kb[u], val = vb[u];
+            const uint d = (key >> shift) & mask;
+            const unsigned long long m = digit_match(d, valid, nbits);
+            // every lane reads the running slot before the lowest lane of its digit advances it (a wavefront's
+            // LDS accesses complete in program order)
+            const uint base = mine[d];
+            if (valid) {
+                const uint pos = base + (uint)__builtin_popcountll(m & below);
+                okeys[pos] = key;
+                ovals[pos] = val;
+                if (gs.n4) {
+#pragma unroll
+                    for (int a = 0; a < kRGatherMax; ++a)
+                        if (a < gs.n4) gs.dst4[a][pos] = gs.src4[a][val];
+                    gs.dstu[pos] = gs.srcu[val];
+                    gs.zero[pos] = 0.0f;
+                }
+                if (lane == __builtin_ctzll(m)) mine[d] = base + (uint)__builtin_popcountll(m);
             }
-            if (lane == __builtin_ctzll(m)) mine[d] = base + (uint)__builtin_popcountll(m);
         }
     }
 }
