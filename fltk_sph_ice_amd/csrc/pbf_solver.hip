@@ -1350,8 +1350,13 @@ __global__ __launch_bounds__(kTpb) void k_xsph_deferred(const float4* __restrict
 // HeatTransfar::heatParticleAndParticle (HeatTransfar.cpp:429-473): over the neighbours within h at the
 // final positions, dT_i = td sum_{j != i} dt (T_j - T_i) / max(rho_j, 0.1) W_spline(r_ij, h), with the
 // cubic spline of HeatTransfar.cpp:200-236 (sigma = 10 / (7 pi h^2)).  therm = (T, heat, phase, changed).
+// hp: (T_j, max(rho_j, 0.1)) per slot (k_heat_pack), read for the neighbours within h.
+__global__ void k_heat_pack(const float4* __restrict__ therm, const float* __restrict__ dens, float2* __restrict__ hp, uint n) {
+    uint i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) hp[i] = make_float2(therm[i].x, fmaxf(dens[i], 0.1f));
+}
 template <bool kCoop, class Walk>
-__device__ inline bool heat_slot(uint i, const float4* __restrict__ x, const float* __restrict__ dens,
+__device__ inline bool heat_slot(uint i, const float4* __restrict__ x, const float2* __restrict__ hp,
                                  const float4* __restrict__ therm, float* __restrict__ dtemp, const PbfK& k, float td, float hdt,
                                  Walk&& walk) {
     float3 p = f3(x[i]);
@@ -1363,10 +1368,10 @@ __device__ inline bool heat_slot(uint i, const float4* __restrict__ x, const flo
                   float3 r = p - f3(qj);
                   float r2 = dot3(r, r);
                   if (r2 <= k.s.h2 && j != i) {
+                      const float2 t = hp[j];  // one gather for the neighbour's temperature and density
                       float q = sqrtf(r2) * inv_h;
                       float w = q < 1.0f ? ws * (1.0f - 1.5f * q * q + 0.75f * q * q * q) : ws * 0.25f * (2.0f - q) * (2.0f - q) * (2.0f - q);
-                      float d = fmaxf(dens[j], 0.1f);
-                      acc += hdt * (therm[j].x - ti) / d * w;
+                      acc += hdt * (t.x - ti) / t.y * w;
                   }
               }))
         return false;
@@ -1375,7 +1380,7 @@ __device__ inline bool heat_slot(uint i, const float4* __restrict__ x, const flo
     return true;
 }
 
-__global__ __launch_bounds__(kTpb) void k_heat_diffuse(const float4* __restrict__ x, const float* __restrict__ dens,
+__global__ __launch_bounds__(kTpb) void k_heat_diffuse(const float4* __restrict__ x, const float2* __restrict__ hp,
                                                        const float4* __restrict__ therm, float* __restrict__ dtemp,
                                                        const float* __restrict__ stats, PbfK k, const uint* __restrict__ list,
                                                        const uint4* __restrict__ cnt, const uint* __restrict__ bhash,
@@ -1386,12 +1391,12 @@ __global__ __launch_bounds__(kTpb) void k_heat_diffuse(const float4* __restrict_
     float disp2 = stats[ST_DISP2];
     const NearMovers nm = stage_movers(i, valid, k, disp2, stats, xb, x, bhash, false);
     if (!valid) return;
-    heat_slot<false>(i, x, dens, therm, dtemp, k, td, hdt, [&](float3 p, auto&& fetch, auto&& body) {
+    heat_slot<false>(i, x, hp, therm, dtemp, k, td, hdt, [&](float3 p, auto&& fetch, auto&& body) {
         return for_neighbors(i, 0, k, list, cnt, bhash, off, disp2, p, xb, nm, fetch, body);
     });
 }
 
-__global__ __launch_bounds__(kTpb) void k_heat_deferred(const float4* __restrict__ x, const float* __restrict__ dens,
+__global__ __launch_bounds__(kTpb) void k_heat_deferred(const float4* __restrict__ x, const float2* __restrict__ hp,
                                                         const float4* __restrict__ therm, float* __restrict__ dtemp,
                                                         float* __restrict__ stats, PbfK k, const uint* __restrict__ bhash,
                                                         const uint* __restrict__ off, const float4* __restrict__ xb, float td,
@@ -1401,7 +1406,7 @@ __global__ __launch_bounds__(kTpb) void k_heat_deferred(const float4* __restrict
     for (uint q = wave_index(); q < nq; q += wave_count()) {
         const uint i = k.defer[q];
         const float di = sqrtf(len2_fixed(f3(x[i]) - f3(xb[i])));
-        heat_slot<true>(i, x, dens, therm, dtemp, k, td, hdt, [&](float3, auto&& fetch, auto&& body) {
+        heat_slot<true>(i, x, hp, therm, dtemp, k, td, hdt, [&](float3, auto&& fetch, auto&& body) {
             coop_neighbors(i, di, k, bhash, off, disp2, xb, nmov, fetch, body);
             return true;
         });
@@ -2020,10 +2025,11 @@ int rxPBDSPH_GPU::enqueue_step(float dt) {
     std::swap(m_x, m_xs);
     if (m_therm) {  // heat exchange at the final positions
         mk = begin_mark(RX_PROF_HEAT);
-        k_heat_diffuse<<<nb, kTpb, 0, s>>>(m_x, m_dens, m_therm, m_ftmp, m_stats, k, m_list, m_cnt, m_grid.hash, m_grid.off, m_xb,
+        k_heat_pack<<<rx_blocks(n, 256), 256, 0, s>>>(m_therm, m_dens, (float2*)m_t, n);  // m_t is free once the velocity is final
+        k_heat_diffuse<<<nb, kTpb, 0, s>>>(m_x, (const float2*)m_t, m_therm, m_ftmp, m_stats, k, m_list, m_cnt, m_grid.hash, m_grid.off, m_xb,
                                            m_heat_td, m_heat_dt > 0.0f ? m_heat_dt : dt);
         if (m_defer_on)
-            k_heat_deferred<<<kDeferBlocks, kTpb, 0, s>>>(m_x, m_dens, m_therm, m_ftmp, m_stats, k, m_grid.hash, m_grid.off, m_xb,
+            k_heat_deferred<<<kDeferBlocks, kTpb, 0, s>>>(m_x, (const float2*)m_t, m_therm, m_ftmp, m_stats, k, m_grid.hash, m_grid.off, m_xb,
                                                           m_heat_td, m_heat_dt > 0.0f ? m_heat_dt : dt);
         k_heat_phase<<<rx_blocks(n, 256), 256, 0, s>>>(m_therm, m_ftmp, n);
         end_mark(mk);
