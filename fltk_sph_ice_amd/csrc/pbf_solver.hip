@@ -685,14 +685,14 @@ __device__ inline bool predict_slot(uint i, const float4* __restrict__ x, const 
     float3 p = f3(x[i]), vi = f3(v[i]);
     float d = 0.0f;
     float3 fv = f3(0, 0, 0);
-    if (!walk(p, [&](uint j) { return XV{x[j], v[j]}; },
-              [&](uint, const XV& q) {
-                  float3 r = p - f3(q.x);
+    if (!walk(p, [&](uint j) { return x[j]; },
+              [&](uint j, const float4& xj) {
+                  float3 r = p - f3(xj);
                   float r2 = dot3(r, r);
                   if (r2 <= k.s.h2) {
                       float qq = k.s.h2 - r2;
                       d += qq * qq * qq;
-                      if (r2 > 0.0f) fv += (k.s.h - r2 * rinv_of(r2)) * (f3(q.v) - vi);
+                      if (r2 > 0.0f) fv += (k.s.h - r2 * rinv_of(r2)) * (f3(v[j]) - vi);  // velocity read within h only
                   }
               }))
         return false;
@@ -1299,13 +1299,14 @@ __device__ inline bool xsph_slot(uint i, const float4* __restrict__ xa, const fl
     float3 p = f3(xa[i]);
     float3 vi = f3(v[i]);
     float3 acc = f3(0, 0, 0);
-    if (!walk(p, [&](uint j) { return XV{xa[j], v[j]}; },
-              [&](uint, const XV& q4) {
-                  float3 r = p - f3(q4.x);
+    if (!walk(p, [&](uint j) { return xa[j]; },
+              [&](uint j, const float4& x4) {
+                  float3 r = p - f3(x4);
                   float r2 = dot3(r, r);
                   if (r2 <= k.s.h2) {
+                      const float4 vj = v[j];  // read for the neighbours within h only
                       float q = k.s.h2 - r2;
-                      acc += (q * q * q / q4.v.w) * (f3(q4.v) - vi);
+                      acc += (q * q * q / vj.w) * (f3(vj) - vi);
                   }
               }))
         return false;
