@@ -91,11 +91,6 @@ __global__ __launch_bounds__(kTpb) void k_density(const float4* __restrict__ x, 
     dp[i] = make_float2(d, pr / (d * d));
 }
 
-struct XVD {  // neighbour position, velocity and (rho, p / rho^2)
-    float4 x, v;
-    float2 d;
-};
-
 // rxSPH::calForce + integrate (rx_sph_cpu.cpp:574-614, 1040-1077); writes the new state.
 __global__ __launch_bounds__(kTpb) void k_force_integrate(const float4* __restrict__ x, const float4* __restrict__ v,
                                                           const float2* __restrict__ dp, const uint* __restrict__ bhash,
@@ -111,15 +106,19 @@ __global__ __launch_bounds__(kTpb) void k_force_integrate(const float4* __restri
         vi = f3(v[i]);
         const float prsi = dp[i].y;
         stencil_split(
-            fg_unhash(bhash[i], k.g), k.g, off, sub, [&](uint j) { return XVD{x[j], v[j], dp[j]}; },
-            [&](uint, const XVD& q) {
-                float3 r = p - f3(q.x);
+            fg_unhash(bhash[i], k.g), k.g, off, sub, [&](uint j) { return x[j]; },
+            [&](uint j, const float4& xj) {
+                float3 r = p - f3(xj);
                 float r2 = dot3(r, r);
                 if (r2 <= k.s.h2 && r2 > 0.0f) {
+                    // velocity and (rho, p / rho^2) are read for the neighbours within h only (about one candidate
+                    // in six of the 27-cell stencil)
+                    const float4 vj = v[j];
+                    const float2 dj = dp[j];
                     float ri = __builtin_amdgcn_rsqf(r2);
                     float qq = k.s.h - r2 * ri;
-                    fp += ((prsi + q.d.y) * qq * qq * ri) * r;
-                    fv += (qq / q.d.x) * (f3(q.v) - vi);
+                    fp += ((prsi + dj.y) * qq * qq * ri) * r;
+                    fv += (qq / dj.x) * (f3(vj) - vi);
                 }
             });
     }
