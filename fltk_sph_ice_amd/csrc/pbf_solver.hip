@@ -1291,8 +1291,9 @@ __global__ void k_velocity(const float4* __restrict__ xa, const float4* __restri
     v[i] = make_float4(vv.x, vv.y, vv.z, dens[i]);
 }
 
-// XSPH (rx_pbdsph_kernel.cu:1090-1162 form): reads v = (v, rho), writes vout.  Held to 80 registers (six
-// wavefronts per SIMD, no spills): the pass waits on neighbour gathers, and the extra wavefront hides more of it.
+// XSPH (rx_pbdsph_kernel.cu:1090-1162 form): reads v = (v, rho), writes vout.  Held to at least six wavefronts
+// per SIMD (it needs 68 registers, so seven run): the pass waits on neighbour gathers, and more wavefronts hide
+// more of it.  Eight (64 registers) spill and run slower.
 template <bool kCoop, class Walk>
 __device__ inline bool xsph_slot(uint i, const float4* __restrict__ xa, const float4* __restrict__ v, float4* __restrict__ vout,
                                  const PbfK& k, Walk&& walk) {
