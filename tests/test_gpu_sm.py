@@ -213,3 +213,38 @@ def test_graded_block_fills_every_size_bin():
     assert np.abs(gp[:, :3] - rp[:, :3]).max() <= 2e-5 * np.abs(rp[:, :3]).max()
     assert np.abs(gv[:, :3] - rv[:, :3]).max() <= 2e-3 * max(np.abs(rv[:, :3]).max(), 1e-3)
     sm.close()
+
+
+def test_ice_block_at_131072_particles():
+    """The batched shape-matching configuration at its full size: 131,072 particles, clusters within 1.6
+    spacings, alpha = 0.9, beta = 0.3, 2 iterations.  One step matches the oracle to the parity bar of the
+    smaller blocks.  Over 50 more steps (the block falls 0.5 onto the floor): the state stays finite, the block's
+    centre falls, every cluster keeps its members, the fastest particle stays below three times the free-fall
+    speed of the block's top, and no particle sinks a lattice spacing into a wall (the wall rule acts on the
+    prediction; the shape-matching goal can carry a particle slightly past it, as in the reference)."""
+    sc = scene.ice_block()
+    assert sc.n == 131072
+    sm, pos, vel = _make(sc)
+    cs, mem, hist = sm.clusters()
+    rp, rv = _oracle_steps(sc, pos, vel, cs, mem, 1)
+    sm.step(1)
+    gp, gv = sm.state()
+    ep = np.abs(gp[:, :3] - rp[:, :3]).max() / np.abs(rp[:, :3]).max()
+    print(f"ice block, 131,072 particles: one-step position error {ep:.2e} of the box scale; cluster sizes "
+          f"{ {i: int(c) for i, c in enumerate(hist) if c} }")
+    assert ep <= 2e-5
+    assert np.abs(gv[:, :3] - rv[:, :3]).max() <= 2e-3 * max(np.abs(rv[:, :3]).max(), 1e-3)
+    sm.step(50)
+    gp, gv = sm.state()
+    cs2, mem2, _ = sm.clusters()
+    lo, hi = np.array(sc.box_min, np.float32), np.array(sc.box_max, np.float32)
+    sink = float(max((lo - gp[:, :3]).max(), (gp[:, :3] - hi).max(), 0.0))
+    top = float(pos[:, 1].max() - lo[1])
+    vmax = float(np.abs(gv[:, :3]).max())
+    print(f"after 51 steps: centre height {gp[:, 1].mean():.3f} (from {pos[:, 1].mean():.3f}), deepest past a wall "
+          f"{sink:.4f}, max |v| {vmax:.2f}")
+    assert np.isfinite(gp).all() and np.isfinite(gv).all()
+    assert gp[:, 1].mean() < pos[:, 1].mean()
+    assert np.array_equal(cs, cs2) and np.array_equal(mem, mem2)
+    assert vmax < 3.0 * np.sqrt(2.0 * 9.8 * top)
+    assert sink < sc.spacing
