@@ -94,6 +94,12 @@ def test_ext_force_and_xsph(lib):
     check(lib)
     nv = dnv.download(np.float32).reshape(n, 4)
     assert np.abs(nv[:, :3] - rv[:, :3]).max() <= TOL * np.abs(rv[:, :3]).max()
+    # in place (dNewVel == dVel): every neighbour velocity is read from the slot-order copy made before any is
+    # overwritten, so the result is the out-of-place one, bit for bit
+    dvin = DevBuf.from_array(vel)
+    lib.CuXSphViscosity(dpos.ptr, dvin.ptr, dvin.ptr, drd.ptr, np.float32(0.01), c)
+    check(lib)
+    assert np.array_equal(dvin.download(np.float32).reshape(n, 4).view(np.uint32), nv.view(np.uint32))
 
 
 def test_abi_step_parity(lib):
